@@ -1,0 +1,139 @@
+"""Benchmark: elements ranked per second for one random weighted list of
+n = 2^27 elements on one MI355X (sparse ruling set through the C ABI).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--algorithm srs|pd]
+
+Inputs: a seeded random list (successors of a Feistel permutation, int64
+weights uniform in [1,100], 0 on the tail) generated on the device and
+resident in HBM before the timed region.  One step = one kc_rank_list call
+(pack, ruler chase, recursion, propagation), synchronous.  Prints one JSON
+line.  The CPU baseline is the oracle's restatement of the reference's sync
+pointer doubling on emulated MPI ranks (one thread per rank), timed on a
+bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_ELEMENT = 32  # succ 8 in + weight 8 in + root 8 out + rank 8 out
+CHASE_BYTES_PER_ELEMENT = 32  # chase: read (succ, w) 16 B + write (ruler, dist) 16 B per element
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 27)
+    ap.add_argument("--seed", type=int, default=2027)
+    ap.add_argument("--algorithm", default="srs")
+    ap.add_argument("--spacing", type=int, default=0)
+    ap.add_argument("--spacing-deep", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--check", action="store_true", help="verify the last step against the closed form")
+    return ap.parse_args()
+
+
+def cpu_baseline(n_sample, seed):
+    """Oracle restatement of the reference's sync pointer doubling
+    (pointer_doubling_generic.hpp:608-700) on emulated ranks, one thread each."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as o
+    ranks = max(1, min(16, os.cpu_count() or 1))
+    succ, w = o.gen_list(n_sample, seed, True)
+    t0 = time.perf_counter()
+    r, k, rounds = o.pointer_doubling(succ, w, ranks)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "elements/s", "cores": ranks, "kind": "port",
+            "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}), sync pointer "
+                      f"doubling on {ranks} emulated ranks, {rounds} rounds, {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    import torch
+    import kascade_amd
+
+    n = args.n
+    ctx = kascade_amd.Context(0)
+    succ, w = ctx.gen_list(n, args.seed, weighted=True)
+    root = torch.empty(n, dtype=torch.uint64, device="cuda")
+    rank = torch.empty(n, dtype=torch.int64, device="cuda")
+    cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep)
+    for _ in range(args.warmup):
+        ctx.rank_list(succ, w, root, rank, **cfg)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.rank_list(succ, w, root, rank, **cfg)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    phases = ctx.profile_read()
+    ctx.profile(False)
+    ms_per_step = dt * 1e3 / args.steps
+    value = n / (dt / args.steps)
+
+    # dominant kernel and its roofline (HIP events on the library's stream)
+    kernel_phases = {k: v for k, v in phases.items() if k not in ("unclaimed",)}
+    dom = max(kernel_phases, key=lambda k: kernel_phases[k][0])
+    dom_ms = phases[dom][0] / max(1, phases[dom][1])
+    if dom.startswith("chase_ruler_L0"):
+        alg_bytes = CHASE_BYTES_PER_ELEMENT * n
+    elif dom.startswith("ruler_propagation_L0") or dom == "pack":
+        alg_bytes = 32 * n
+    else:
+        alg_bytes = BYTES_PER_ELEMENT * n
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    whole_gbps = BYTES_PER_ELEMENT * n / (ms_per_step * 1e-3) / 1e9
+
+    check = None
+    if args.check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as o
+        er, ek = o.list_closed_form(n, args.seed, w.cpu().numpy())
+        check = bool((root.cpu().numpy() == er).all() and (rank.cpu().numpy() == ek).all())
+
+    out = {
+        "metric": "elements ranked/s @ n=2^27",
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic: seeded random permutation list, int64 weights uniform in [1,100], tail weight 0",
+        "config": {"workload": f"single random list n={n}, int64 weights, {args.algorithm} "
+                               f"(C2: uint64 successors, weighted ranks bit-exact)",
+                   "n": n, "algorithm": args.algorithm, "seed": args.seed},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
+        "whole_call_effective_gbps": whole_gbps,
+        "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
+    }
+    if check is not None:
+        out["bit_exact_vs_closed_form"] = check
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(min(args.cpu_sample, n), args.seed)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,236 @@
+/*
+ * kascade_amd.h -- C ABI of the MI355X list-ranking / tree-rooting library.
+ *
+ * Every entry point is a drop-in for one of KAscade's lib/kascade entry points
+ * (cited per declaration; paths are relative to the niklas-uhl/kascade tree).
+ * The MPI communicator and Distribution arguments of the reference disappear:
+ * one call ranks the whole array on one GPU.
+ *
+ * Conventions (lib/kascade/types.hpp:8-9):
+ *   ids are uint64_t (idx_t), ranks are int64_t (rank_t); ids must be < 2^63
+ *   because bit 63 is the root flag (lib/kascade/bits.hpp:9-21).
+ *
+ * Pointers may be device memory (hipMalloc / torch tensors) or host memory;
+ * host arrays are staged through device buffers by the library.  All entry
+ * points are synchronous with respect to the host: they enqueue on `stream`
+ * (NULL = the context's stream) and return when the results are written, like
+ * the reference's collective calls.  A context is externally synchronised,
+ * like a communicator: one host thread at a time.
+ */
+#ifndef KASCADE_AMD_H
+#define KASCADE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes replace the reference's std::runtime_error / KASSERT aborts
+ * (pointer_doubling_generic.hpp:446-447, sparse_ruling_set.cpp:374,
+ * eulertour.cpp:318). */
+typedef enum kc_status {
+  KC_OK = 0,
+  KC_EINVAL = 1,    /* bad argument, id >= n, in-degree > 1 where a list is required,
+                       or nonzero weight on a root (pointer_doubling_generic.hpp:431) */
+  KC_ECYCLE = 2,    /* a cycle without a root (the reference would not terminate) */
+  KC_ENOMEM = 3,    /* device allocation failed */
+  KC_EDEVICE = 4,   /* HIP runtime error */
+  KC_EBUDGET = 5,   /* job larger than the server's memory budget */
+  KC_EAGAIN = 6,    /* result not ready yet (kc_poll) */
+  KC_ENOTFOUND = 7  /* unknown ticket */
+} kc_status;
+
+/* Mirrors kascade::Algorithm (lib/kascade/configuration.hpp:9-18) for the
+ * members that exist on one GPU. */
+typedef enum kc_algorithm {
+  KC_ALGO_AUTO = 0,
+  KC_ALGO_POINTER_DOUBLING = 1, /* sync Wyllie doubling, pointer_doubling_generic.hpp:608-700 */
+  KC_ALGO_SPARSE_RULING_SET = 2 /* ruler chase + recursion, sparse_ruling_set.cpp:45-393 */
+} kc_algorithm;
+
+/* Knobs of kascade::SparseRulingSetConfig / PointerDoublingConfig
+ * (configuration.hpp:39-47, :65-95) that mean something on one GPU.  Zero
+ * means "choose" (the per-size autotuned table). */
+typedef struct kc_rank_cfg {
+  int32_t algorithm;        /* kc_algorithm */
+  uint32_t ruler_spacing;   /* elements per ruler at the top level (power of two), 0 = auto */
+  uint32_t ruler_spacing_deep; /* elements per ruler below the top level, 0 = auto */
+  uint32_t max_levels;      /* recursion depth cap (sparse_ruling_set_rounds), 0 = auto */
+  uint32_t validate;        /* 1: reject in-degree > 1 before ranking (is_list, successor_utils.cpp:31-63) */
+  uint32_t reserved[3];
+} kc_rank_cfg;
+
+/* ---- context ------------------------------------------------------------ */
+kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void **ctx_out);
+void kc_context_destroy(void *ctx);
+/* Human-readable text of the last error on this context. */
+const char *kc_last_error(void *ctx);
+/* Device memory currently held by the context's scratch pool, in bytes. */
+uint64_t kc_context_scratch_bytes(void *ctx);
+/* Releases the scratch pool. */
+kc_status kc_context_trim(void *ctx);
+
+/* Per-phase device timing (HIP events on the launching stream), the
+ * counterpart of the reference's kamping::measurements::timer phases
+ * (e.g. "chase_ruler", "ruler_propagation", sparse_ruling_set.cpp:147,315).
+ * kc_profile_read returns the number of phases and fills up to max_entries
+ * (name, total ms, launches); names stay valid until the next reset. */
+kc_status kc_profile_enable(void *ctx, int on);
+kc_status kc_profile_reset(void *ctx);
+uint32_t kc_profile_read(void *ctx, uint32_t max_entries, const char **names, double *ms,
+                         uint64_t *counts);
+
+/* ---- list ranking ------------------------------------------------------- */
+
+/* Replaces kascade::set_initial_ranking_state (list_ranking.hpp:47-50,
+ * list_ranking.cpp:60-80): root := succ, rank := 1, rank := 0 where succ[i]==i. */
+kc_status kc_set_initial_ranking_state(void *ctx, const uint64_t *succ, uint64_t n,
+                                       uint64_t *root_out, int64_t *rank_out, void *stream);
+
+/* Ranks a forest of lists.  succ[i] is the successor of i, a tail points to
+ * itself.  weight == NULL means unit weights as set_initial_ranking_state
+ * (rank 1, 0 on tails); otherwise weight[i] is the distance from i to succ[i]
+ * and must be 0 on tails.  Outputs: root_out[i] = the tail of i's list,
+ * rank_out[i] = sum of weights from i (inclusive) to the tail (exclusive).
+ * root_out/rank_out may alias succ/weight (the reference's in-place contract,
+ * pointer_doubling.hpp:14-17).  Replaces kascade::pointer_doubling
+ * (pointer_doubling.hpp:18-23) and kascade::sparse_ruling_set
+ * (sparse_ruling_set.hpp:13-18); cfg->algorithm selects which. */
+kc_status kc_rank_list(void *ctx, const uint64_t *succ, const int64_t *weight, uint64_t n,
+                       uint64_t *root_out, int64_t *rank_out, const kc_rank_cfg *cfg,
+                       void *stream);
+
+/* In-place forms with the reference signatures: succ_array in = successors,
+ * out = roots; rank_array in = distance to successor (0 on roots), out =
+ * distance to root.  kascade::pointer_doubling (pointer_doubling.hpp:18-23). */
+kc_status kc_pointer_doubling(void *ctx, const kc_rank_cfg *cfg, uint64_t *succ_array,
+                              int64_t *rank_array, uint64_t n, void *stream);
+/* kascade::sparse_ruling_set (sparse_ruling_set.hpp:13-18). */
+kc_status kc_sparse_ruling_set(void *ctx, const kc_rank_cfg *cfg, uint64_t *succ_array,
+                               int64_t *rank_array, uint64_t n, void *stream);
+
+/* State after exactly `rounds` synchronous doubling rounds (each a Jacobi step
+ * of DoublingWithoutAggregation::execute_doubling_step,
+ * pointer_doubling_generic.hpp:187-243).  succ_out keeps the root flag
+ * (bit 63) on entries that point at a root, as the reference's arrays do
+ * inside the doubling loop (:418-438, :677-688).  Exposed so the per-round
+ * state can be checked; not a ranking entry point. */
+kc_status kc_pointer_doubling_rounds(void *ctx, const uint64_t *succ, const int64_t *weight,
+                                     uint64_t n, uint32_t rounds, uint64_t *succ_out,
+                                     int64_t *rank_out, void *stream);
+
+/* Order-preserving compaction of the active elements (pack.hpp:23-80): the
+ * packed id of active[k] is k; psucc_out[k] = packed id of succ[active[k]],
+ * prank_out[k] = rank[active[k]].  `active` must be ascending and closed under
+ * succ.  kc_unpack writes packed results back (pack.hpp:81-175). */
+kc_status kc_pack(void *ctx, const uint64_t *succ, const int64_t *rank, uint64_t n,
+                  const uint64_t *active, uint64_t nactive, uint64_t *psucc_out,
+                  int64_t *prank_out, void *stream);
+kc_status kc_unpack(void *ctx, const uint64_t *psucc, const int64_t *prank,
+                    const uint64_t *active, uint64_t nactive, uint64_t *succ_inout,
+                    int64_t *rank_inout, uint64_t n, void *stream);
+
+/* ---- batched segmented ranking ----------------------------------------- */
+/* nseg independent segments; segment s owns elements [seg_off[s], seg_off[s+1])
+ * and its successors are local ids in [0, size_s).  Equivalent to ranking the
+ * forest with global ids seg_off[s] + local.  root_local receives local ids.
+ * weight == NULL: unit weights as set_initial_ranking_state.  The per-size
+ * autotuned ruler spacing applies per segment size class. */
+kc_status kc_rank_lists_segmented(void *ctx, const uint32_t *succ_local, const uint64_t *seg_off,
+                                  uint64_t nseg, const int64_t *weight, uint32_t *root_local,
+                                  int64_t *rank_out, const kc_rank_cfg *cfg, void *stream);
+
+/* ---- tree rooting ------------------------------------------------------- */
+typedef struct kc_tree_cfg {
+  kc_rank_cfg rank; /* how the Euler tour is ranked (eulertour.cpp:282-321) */
+  uint32_t reserved[4];
+} kc_tree_cfg;
+
+/* parent[v] = parent of v, roots are their own parent.  Outputs root_out[v]
+ * (the tree root) and depth_out[v] (edges to the root).  Replaces
+ * kascade::rank_via_euler_tour (eulertour.hpp:22-27, eulertour.cpp:337-377)
+ * after set_initial_ranking_state, as the harness calls it
+ * (bench/detail/algorithm_impl.hpp:191-222).  May alias (in-place). */
+kc_status kc_root_tree(void *ctx, const uint64_t *parent, uint64_t n, uint64_t *root_out,
+                       int64_t *depth_out, const kc_tree_cfg *cfg, void *stream);
+/* In-place form with the reference signature: succ_array in = parents, out =
+ * roots; rank_array out = depth (input values ignored, eulertour.cpp:45-53). */
+kc_status kc_rank_via_euler_tour(void *ctx, const kc_tree_cfg *cfg, uint64_t *succ_array,
+                                 int64_t *rank_array, uint64_t n, void *stream);
+/* The tour successor array alone (kascade::compute_euler_tour,
+ * eulertour.cpp:379-392), in this library's element numbering: element v < n
+ * is the upward edge (v, parent v) or, for a root, the tour's terminal dummy;
+ * element n + v is the downward edge (parent v, v).  tour_succ_out has 2n
+ * entries, tour_weight_out (may be NULL) the +1/-1/0 weights. */
+kc_status kc_compute_euler_tour(void *ctx, const uint64_t *parent, uint64_t n,
+                                uint64_t *tour_succ_out, int64_t *tour_weight_out, void *stream);
+
+/* ---- synthetic inputs (seeded; identical to the CPU generators) ---------- */
+/* Random list: order pi = seeded Feistel permutation, succ[pi(i)] = pi(i+1);
+ * weighted: weights uniform in [1,100], 0 on the tail; else 1 / 0.
+ * local != 0: blocks of 10 consecutive ids visited in permuted block order. */
+kc_status kc_gen_list(void *ctx, uint64_t n, uint64_t seed, int weighted, int local,
+                      uint64_t *succ_out, int64_t *weight_out, void *stream);
+/* Random recursive tree (parent of generation index i uniform in [0,i)),
+ * relabelled by a seeded permutation. */
+kc_status kc_gen_tree(void *ctx, uint64_t n, uint64_t seed, uint64_t *parent_out, void *stream);
+/* Segment-local variant for batches: segment s is a list of size
+ * seg_off[s+1]-seg_off[s] generated with seed seeds[s] (local ids, u32). */
+kc_status kc_gen_lists_segmented(void *ctx, const uint64_t *seg_off, uint64_t nseg,
+                                 const uint64_t *seeds, const uint8_t *local_flags, int weighted,
+                                 uint32_t *succ_out, int64_t *weight_out, void *stream);
+
+/* ---- serving ------------------------------------------------------------ */
+typedef enum kc_job_kind { KC_JOB_LIST = 0, KC_JOB_TREE = 1 } kc_job_kind;
+
+/* A job's arrays live in host memory owned by the caller until the ticket
+ * completes.  LIST: succ/weight in, root/rank out.  TREE: succ = parent in,
+ * root/rank(depth) out, weight ignored. */
+typedef struct kc_job {
+  int32_t kind; /* kc_job_kind */
+  uint32_t reserved;
+  uint64_t n;
+  const uint64_t *succ;
+  const int64_t *weight; /* may be NULL */
+  uint64_t *root_out;
+  int64_t *rank_out;
+} kc_job;
+
+typedef struct kc_server_cfg {
+  uint64_t mem_budget_bytes;   /* device-memory admission budget */
+  uint32_t max_batch_jobs;     /* jobs coalesced into one segmented launch, 0 = auto */
+  uint32_t num_slots;          /* batches in flight on separate streams, 0 = auto */
+  uint64_t max_batch_elements; /* elements per batch, 0 = auto */
+  uint32_t one_job_per_launch; /* 1: no coalescing (the baseline mode) */
+  uint32_t reserved[3];
+} kc_server_cfg;
+
+typedef struct kc_server_stats {
+  uint64_t jobs_completed;
+  uint64_t elements_completed;
+  uint64_t batches_launched;
+  double p50_latency_ms;
+  double p99_latency_ms;
+  double mean_latency_ms;
+  double jobs_per_s;       /* completions / (last completion - first submit) */
+  double elements_per_s;
+  uint64_t peak_device_bytes;
+} kc_server_stats;
+
+kc_status kc_server_create(void *ctx, const kc_server_cfg *cfg, void **server_out);
+/* Stops admitting, drains in-flight work, frees everything. */
+void kc_server_destroy(void *server);
+/* Enqueue a job; returns at once.  KC_EBUDGET if the job can never fit. */
+kc_status kc_submit(void *server, const kc_job *job, uint64_t *ticket_out);
+/* KC_OK once the job's outputs are written, KC_EAGAIN before, or its error. */
+kc_status kc_poll(void *server, uint64_t ticket);
+kc_status kc_wait(void *server, uint64_t ticket);
+kc_status kc_server_stats_get(void *server, kc_server_stats *out);
+kc_status kc_server_reset_stats(void *server);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KASCADE_AMD_H */

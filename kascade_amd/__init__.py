@@ -1,0 +1,416 @@
+"""Python binding of libkascade_amd (ctypes over the C ABI in include/kascade_amd.h).
+
+Arrays may be torch tensors (device or host) or numpy arrays (host); host arrays
+are staged by the library.  Every call raises KascadeError on a non-OK status.
+The HIP library must be built (python -m kascade_amd._build); importing this
+module on a machine without it raises ImportError: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Context", "KascadeError", "Server", "lib_path", "load"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libkascade_amd.so")
+_lib = None
+
+KC_OK, KC_EINVAL, KC_ECYCLE, KC_ENOMEM, KC_EDEVICE, KC_EBUDGET, KC_EAGAIN, KC_ENOTFOUND = range(8)
+STATUS_NAMES = {0: "KC_OK", 1: "KC_EINVAL", 2: "KC_ECYCLE", 3: "KC_ENOMEM", 4: "KC_EDEVICE",
+                5: "KC_EBUDGET", 6: "KC_EAGAIN", 7: "KC_ENOTFOUND"}
+ALGORITHMS = {"auto": 0, "pointer_doubling": 1, "pd": 1, "sparse_ruling_set": 2, "srs": 2}
+
+
+class KascadeError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class RankCfg(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_int32), ("ruler_spacing", ctypes.c_uint32),
+                ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
+                ("validate", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class TreeCfg(ctypes.Structure):
+    _fields_ = [("rank", RankCfg), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_uint32), ("n", ctypes.c_uint64),
+                ("succ", ctypes.c_void_p), ("weight", ctypes.c_void_p),
+                ("root_out", ctypes.c_void_p), ("rank_out", ctypes.c_void_p)]
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint32),
+                ("num_slots", ctypes.c_uint32), ("max_batch_elements", ctypes.c_uint64),
+                ("one_job_per_launch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("jobs_completed", ctypes.c_uint64), ("elements_completed", ctypes.c_uint64),
+                ("batches_launched", ctypes.c_uint64), ("p50_latency_ms", ctypes.c_double),
+                ("p99_latency_ms", ctypes.c_double), ("mean_latency_ms", ctypes.c_double),
+                ("jobs_per_s", ctypes.c_double), ("elements_per_s", ctypes.c_double),
+                ("peak_device_bytes", ctypes.c_uint64)]
+
+
+def lib_path():
+    return _LIB_PATH
+
+
+# name -> (restype, argtypes); every symbol declared in include/kascade_amd.h
+_P = ctypes.c_void_p
+_U64 = ctypes.c_uint64
+_SIGS = {
+    "kc_context_create": (ctypes.c_int, [ctypes.c_int, _U64, ctypes.POINTER(_P)]),
+    "kc_context_destroy": (None, [_P]),
+    "kc_last_error": (ctypes.c_char_p, [_P]),
+    "kc_context_scratch_bytes": (_U64, [_P]),
+    "kc_context_trim": (ctypes.c_int, [_P]),
+    "kc_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
+    "kc_profile_reset": (ctypes.c_int, [_P]),
+    "kc_profile_read": (ctypes.c_uint32, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
+    "kc_set_initial_ranking_state": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
+    "kc_rank_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, ctypes.POINTER(RankCfg), _P]),
+    "kc_pointer_doubling": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
+    "kc_sparse_ruling_set": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
+    "kc_pointer_doubling_rounds": (ctypes.c_int, [_P, _P, _P, _U64, ctypes.c_uint32, _P, _P, _P]),
+    "kc_pack": (ctypes.c_int, [_P, _P, _P, _U64, _P, _U64, _P, _P, _P]),
+    "kc_unpack": (ctypes.c_int, [_P, _P, _P, _P, _U64, _P, _P, _U64, _P]),
+    "kc_rank_lists_segmented": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, _P, ctypes.POINTER(RankCfg), _P]),
+    "kc_root_tree": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.POINTER(TreeCfg), _P]),
+    "kc_rank_via_euler_tour": (ctypes.c_int, [_P, ctypes.POINTER(TreeCfg), _P, _P, _U64, _P]),
+    "kc_compute_euler_tour": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
+    "kc_gen_list": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
+    "kc_gen_tree": (ctypes.c_int, [_P, _U64, _U64, _P, _P]),
+    "kc_gen_lists_segmented": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.c_int, _P, _P, _P]),
+    "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
+    "kc_server_destroy": (None, [_P]),
+    "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
+    "kc_poll": (ctypes.c_int, [_P, _U64]),
+    "kc_wait": (ctypes.c_int, [_P, _U64]),
+    "kc_server_stats_get": (ctypes.c_int, [_P, ctypes.POINTER(ServerStats)]),
+    "kc_server_reset_stats": (ctypes.c_int, [_P]),
+}
+
+
+def load():
+    """Loads the HIP library; raises ImportError if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise ImportError(f"{_LIB_PATH} is missing: run `python -m kascade_amd._build`")
+        L = ctypes.CDLL(_LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, dtype):
+    """Data pointer of a torch tensor or numpy array of the given numpy dtype."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        if a.dtype != dtype:
+            raise TypeError(f"expected {np.dtype(dtype)}, got {a.dtype}")
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be contiguous")
+        return a.ctypes.data
+    import torch  # torch tensors (device or host)
+    tmap = {np.uint64: torch.uint64, np.int64: torch.int64, np.uint32: torch.uint32,
+            np.uint8: torch.uint8}
+    if a.dtype != tmap[np.dtype(dtype).type]:
+        raise TypeError(f"expected {tmap[np.dtype(dtype).type]}, got {a.dtype}")
+    if not a.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return a.data_ptr()
+
+
+def _numel(a):
+    return a.size if isinstance(a, np.ndarray) else a.numel()
+
+
+def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False):
+    c = RankCfg()
+    c.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    c.ruler_spacing = ruler_spacing
+    c.ruler_spacing_deep = ruler_spacing_deep
+    c.max_levels = max_levels
+    c.validate = int(bool(validate))
+    return c
+
+
+def _like(a, dtype, n=None):
+    n = _numel(a) if n is None else n
+    if isinstance(a, np.ndarray):
+        return np.empty(n, dtype)
+    import torch
+    tmap = {np.uint64: torch.uint64, np.int64: torch.int64, np.uint32: torch.uint32}
+    return torch.empty(n, dtype=tmap[np.dtype(dtype).type], device=a.device)
+
+
+class Context:
+    """One device context (kc_context_create).  Not thread-safe, like a communicator."""
+
+    def __init__(self, device=0, mem_budget_bytes=0):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        self._check(self._lib.kc_context_create(device, mem_budget_bytes, ctypes.byref(h)), None)
+        self.handle = h
+        self.device = device
+
+    def _check(self, status, handle=True):
+        if status != KC_OK:
+            msg = ""
+            if handle is not None and getattr(self, "handle", None):
+                raw = self._lib.kc_last_error(self.handle)
+                msg = raw.decode() if raw else ""
+            raise KascadeError(status, msg)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.kc_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- list ranking ---------------------------------------------------
+    def rank_list(self, succ, weight=None, root_out=None, rank_out=None, stream=None, **cfg):
+        n = _numel(succ)
+        root_out = _like(succ, np.uint64) if root_out is None else root_out
+        rank_out = _like(succ, np.int64) if rank_out is None else rank_out
+        c = _cfg(**cfg)
+        self._check(self._lib.kc_rank_list(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64), n,
+                                           _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
+                                           ctypes.byref(c), stream))
+        return root_out, rank_out
+
+    def pointer_doubling(self, succ_array, rank_array, stream=None, **cfg):
+        """In place: succ -> root, rank (weights) -> rank (pointer_doubling.hpp:18-23)."""
+        c = _cfg(**cfg)
+        self._check(self._lib.kc_pointer_doubling(self.handle, ctypes.byref(c), _ptr(succ_array, np.uint64),
+                                                  _ptr(rank_array, np.int64), _numel(succ_array), stream))
+
+    def sparse_ruling_set(self, succ_array, rank_array, stream=None, **cfg):
+        """In place, as kascade::sparse_ruling_set (sparse_ruling_set.hpp:13-18)."""
+        c = _cfg(**cfg)
+        self._check(self._lib.kc_sparse_ruling_set(self.handle, ctypes.byref(c), _ptr(succ_array, np.uint64),
+                                                   _ptr(rank_array, np.int64), _numel(succ_array), stream))
+
+    def set_initial_ranking_state(self, succ, root_out=None, rank_out=None, stream=None):
+        root_out = _like(succ, np.uint64) if root_out is None else root_out
+        rank_out = _like(succ, np.int64) if rank_out is None else rank_out
+        self._check(self._lib.kc_set_initial_ranking_state(self.handle, _ptr(succ, np.uint64), _numel(succ),
+                                                           _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
+                                                           stream))
+        return root_out, rank_out
+
+    def pd_rounds(self, succ, weight, rounds, stream=None):
+        so = _like(succ, np.uint64)
+        ro = _like(succ, np.int64)
+        self._check(self._lib.kc_pointer_doubling_rounds(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64),
+                                                         _numel(succ), rounds, _ptr(so, np.uint64),
+                                                         _ptr(ro, np.int64), stream))
+        return so, ro
+
+    def pack(self, succ, rank, active, stream=None):
+        na = _numel(active)
+        ps = _like(active, np.uint64, na)
+        pr = _like(active, np.int64, na)
+        self._check(self._lib.kc_pack(self.handle, _ptr(succ, np.uint64), _ptr(rank, np.int64), _numel(succ),
+                                      _ptr(active, np.uint64), na, _ptr(ps, np.uint64), _ptr(pr, np.int64),
+                                      stream))
+        return ps, pr
+
+    def unpack(self, psucc, prank, active, succ, rank, stream=None):
+        self._check(self._lib.kc_unpack(self.handle, _ptr(psucc, np.uint64), _ptr(prank, np.int64),
+                                        _ptr(active, np.uint64), _numel(active), _ptr(succ, np.uint64),
+                                        _ptr(rank, np.int64), _numel(succ), stream))
+
+    def rank_lists_segmented(self, succ_local, seg_off, weight=None, root_out=None, rank_out=None,
+                             stream=None, **cfg):
+        total = _numel(succ_local)
+        root_out = _like(succ_local, np.uint32) if root_out is None else root_out
+        rank_out = _like(succ_local, np.int64, total) if rank_out is None else rank_out
+        c = _cfg(**cfg)
+        self._check(self._lib.kc_rank_lists_segmented(self.handle, _ptr(succ_local, np.uint32),
+                                                       _ptr(seg_off, np.uint64), _numel(seg_off) - 1,
+                                                       _ptr(weight, np.int64), _ptr(root_out, np.uint32),
+                                                       _ptr(rank_out, np.int64), ctypes.byref(c), stream))
+        return root_out, rank_out
+
+    # ---- trees --------------------------------------------------------------
+    def root_tree(self, parent, root_out=None, depth_out=None, stream=None, **cfg):
+        root_out = _like(parent, np.uint64) if root_out is None else root_out
+        depth_out = _like(parent, np.int64) if depth_out is None else depth_out
+        t = TreeCfg()
+        t.rank = _cfg(**cfg)
+        self._check(self._lib.kc_root_tree(self.handle, _ptr(parent, np.uint64), _numel(parent),
+                                           _ptr(root_out, np.uint64), _ptr(depth_out, np.int64),
+                                           ctypes.byref(t), stream))
+        return root_out, depth_out
+
+    def rank_via_euler_tour(self, succ_array, rank_array, stream=None, **cfg):
+        """In place: parents -> roots, rank -> depth (eulertour.hpp:22-27)."""
+        t = TreeCfg()
+        t.rank = _cfg(**cfg)
+        self._check(self._lib.kc_rank_via_euler_tour(self.handle, ctypes.byref(t), _ptr(succ_array, np.uint64),
+                                                     _ptr(rank_array, np.int64), _numel(succ_array), stream))
+
+    def compute_euler_tour(self, parent, stream=None):
+        n = _numel(parent)
+        s = _like(parent, np.uint64, 2 * n)
+        w = _like(parent, np.int64, 2 * n)
+        self._check(self._lib.kc_compute_euler_tour(self.handle, _ptr(parent, np.uint64), n,
+                                                    _ptr(s, np.uint64), _ptr(w, np.int64), stream))
+        return s, w
+
+    # ---- synthetic inputs --------------------------------------------------------
+    def gen_list(self, n, seed, weighted=True, local=False, device="cuda", stream=None):
+        succ, w = _alloc_pair(n, device)
+        self._check(self._lib.kc_gen_list(self.handle, n, seed, int(weighted), int(local),
+                                          _ptr(succ, np.uint64), _ptr(w, np.int64), stream))
+        return succ, w
+
+    def gen_tree(self, n, seed, device="cuda", stream=None):
+        parent, _ = _alloc_pair(n, device, weights=False)
+        self._check(self._lib.kc_gen_tree(self.handle, n, seed, _ptr(parent, np.uint64), stream))
+        return parent
+
+    def gen_lists_segmented(self, seg_off, seeds, local_flags=None, weighted=True, device="cuda",
+                            stream=None):
+        total = int(seg_off[-1])
+        if device is None or device == "cpu":
+            succ = np.empty(total, np.uint32)
+            w = np.empty(total, np.int64)
+        else:
+            import torch
+            succ = torch.empty(total, dtype=torch.uint32, device=device)
+            w = torch.empty(total, dtype=torch.int64, device=device)
+        self._check(self._lib.kc_gen_lists_segmented(self.handle, _ptr(seg_off, np.uint64), _numel(seg_off) - 1,
+                                                     _ptr(seeds, np.uint64),
+                                                     _ptr(local_flags, np.uint8) if local_flags is not None else None,
+                                                     int(weighted), _ptr(succ, np.uint32), _ptr(w, np.int64),
+                                                     stream))
+        return succ, w
+
+    # ---- phase timing -----------------------------------------------------------
+    def profile(self, on=True):
+        self._check(self._lib.kc_profile_enable(self.handle, int(on)))
+
+    def profile_reset(self):
+        self._check(self._lib.kc_profile_reset(self.handle))
+
+    def profile_read(self):
+        """{phase: (total_ms, launches)} accumulated since the last reset."""
+        k = self._lib.kc_profile_read(self.handle, 0, None, None, None)
+        names = (ctypes.c_char_p * max(k, 1))()
+        ms = (ctypes.c_double * max(k, 1))()
+        cnt = (ctypes.c_uint64 * max(k, 1))()
+        k = self._lib.kc_profile_read(self.handle, k, names, ms, cnt)
+        return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
+
+    def scratch_bytes(self):
+        return self._lib.kc_context_scratch_bytes(self.handle)
+
+    def trim(self):
+        self._check(self._lib.kc_context_trim(self.handle))
+
+
+def _alloc_pair(n, device, weights=True):
+    if device is None or device == "cpu":
+        return np.empty(n, np.uint64), (np.empty(n, np.int64) if weights else None)
+    import torch
+    return (torch.empty(n, dtype=torch.uint64, device=device),
+            torch.empty(n, dtype=torch.int64, device=device) if weights else None)
+
+
+class Server:
+    """Continuous-batching request server (kc_server_*); jobs use host arrays."""
+
+    def __init__(self, ctx, mem_budget_bytes=200 << 30, max_batch_jobs=0, num_slots=0,
+                 max_batch_elements=0, one_job_per_launch=False):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        cfg = ServerCfg()
+        cfg.mem_budget_bytes = mem_budget_bytes
+        cfg.max_batch_jobs = max_batch_jobs
+        cfg.num_slots = num_slots
+        cfg.max_batch_elements = max_batch_elements
+        cfg.one_job_per_launch = int(bool(one_job_per_launch))
+        h = ctypes.c_void_p()
+        ctx._check(self._lib.kc_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, kind, succ, weight=None, root_out=None, rank_out=None):
+        n = succ.size
+        root_out = np.empty(n, np.uint64) if root_out is None else root_out
+        rank_out = np.empty(n, np.int64) if rank_out is None else rank_out
+        j = Job()
+        j.kind = {"list": 0, "tree": 1}[kind] if isinstance(kind, str) else int(kind)
+        j.n = n
+        j.succ = _ptr(succ, np.uint64)
+        j.weight = _ptr(weight, np.int64)
+        j.root_out = _ptr(root_out, np.uint64)
+        j.rank_out = _ptr(rank_out, np.int64)
+        t = ctypes.c_uint64()
+        st = self._lib.kc_submit(self.handle, ctypes.byref(j), ctypes.byref(t))
+        if st != KC_OK:
+            raise KascadeError(st, "submit failed")
+        self._keep[t.value] = (succ, weight, root_out, rank_out)
+        return t.value, root_out, rank_out
+
+    def poll(self, ticket):
+        st = self._lib.kc_poll(self.handle, ticket)
+        if st == KC_EAGAIN:
+            return False
+        if st != KC_OK:
+            raise KascadeError(st, f"job {ticket} failed")
+        self._keep.pop(ticket, None)
+        return True
+
+    def wait(self, ticket):
+        st = self._lib.kc_wait(self.handle, ticket)
+        self._keep.pop(ticket, None)
+        if st != KC_OK:
+            raise KascadeError(st, f"job {ticket} failed")
+
+    def stats(self):
+        s = ServerStats()
+        st = self._lib.kc_server_stats_get(self.handle, ctypes.byref(s))
+        if st != KC_OK:
+            raise KascadeError(st, "stats")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        self._lib.kc_server_reset_stats(self.handle)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.kc_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

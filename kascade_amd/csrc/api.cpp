@@ -1,0 +1,551 @@
+// C ABI of libkascade_amd: contexts, host/device pointer staging, status
+// mapping.  The kernels live in the .hip sources.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <type_traits>
+
+#include "common.hpp"
+
+namespace kc {
+
+// ---------------------------------------------------------------- Arena ---
+Arena::~Arena() { trim(); }
+
+void* Arena::alloc(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t{255};
+  if (bytes == 0) bytes = 256;
+  // Next free position after the top block.
+  size_t chunk = 0, offset = 0;
+  if (top_ > 0) {
+    const Block& b = blocks_[top_ - 1];
+    chunk = b.chunk;
+    offset = b.offset + b.size;
+  }
+  while (chunk < chunks_.size() && offset + bytes > chunks_[chunk].size) {
+    chunk++;
+    offset = 0;
+  }
+  if (chunk == chunks_.size()) {
+    size_t want = std::max(bytes, size_t{64} << 20);
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    chunks_.push_back(Chunk{static_cast<char*>(p), want});
+  }
+  Block nb{chunk, offset, bytes};
+  if (top_ < blocks_.size()) blocks_[top_] = nb; else blocks_.push_back(nb);
+  top_++;
+  return chunks_[chunk].base + offset;
+}
+
+void Arena::release(size_t mark) { top_ = std::min(top_, mark); }
+
+void Arena::trim() {
+  if (top_ != 0) return;  // never free live blocks
+  for (auto& c : chunks_) (void)hipFree(c.base);
+  chunks_.clear();
+  blocks_.clear();
+}
+
+uint64_t Arena::held() const {
+  uint64_t s = 0;
+  for (auto& c : chunks_) s += c.size;
+  return s;
+}
+
+// -------------------------------------------------------------- Profiler ---
+hipEvent_t Profiler::get() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  KC_HIP(hipEventCreate(&e));
+  return e;
+}
+void Profiler::begin(const char* name, hipStream_t st) {
+  hipEvent_t a = get();
+  KC_HIP(hipEventRecord(a, st));
+  open_.emplace_back(name, a);
+}
+void Profiler::end(hipStream_t st) {
+  if (open_.empty()) return;
+  hipEvent_t b = get();
+  (void)hipEventRecord(b, st);
+  pending_.push_back(Pending{open_.back().first, open_.back().second, b});
+  open_.pop_back();
+}
+void Profiler::flush() {
+  for (auto& p : pending_) {
+    float ms = 0;
+    (void)hipEventSynchronize(p.b);
+    if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) (void)hipGetLastError();
+    bool found = false;
+    for (auto& t : totals)
+      if (t.name == p.name) {
+        t.ms += ms;
+        t.count++;
+        found = true;
+      }
+    if (!found) totals.push_back(Acc{p.name, ms, 1});
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+void Profiler::reset() {
+  flush();
+  totals.clear();
+}
+Profiler::~Profiler() {
+  for (auto& p : pending_) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+  for (auto& o : open_) (void)hipEventDestroy(o.second);
+}
+
+void check_err(Context* ctx, hipStream_t st) {
+  uint32_t e = 0;
+  KC_HIP(hipMemcpyAsync(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost, st));
+  KC_HIP(hipStreamSynchronize(st));
+  if (e == 0) return;
+  KC_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), st));
+  KC_HIP(hipStreamSynchronize(st));
+  if (e & kErrRange) throw KcError{KC_EINVAL, "successor id out of range (must be < n)"};
+  if (e & kErrRootWeight) throw KcError{KC_EINVAL, "a root (succ[i] == i) carries a nonzero weight"};
+  if (e & kErrNotList) throw KcError{KC_EINVAL, "not a list: an element has more than one predecessor"};
+  if (e & kErrCycle) throw KcError{KC_ECYCLE, "cycle without a root"};
+  throw KcError{KC_EDEVICE, "unknown device error"};
+}
+
+void read_counters(Context* ctx, hipStream_t st, int first, int count, uint64_t* out) {
+  KC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_counters + first, count * sizeof(uint64_t),
+                        hipMemcpyDeviceToHost, st));
+  KC_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, ctx->h_pinned, count * sizeof(uint64_t));
+}
+
+// Host arrays are staged through device buffers; device arrays pass through.
+bool is_device_ptr(const void* p) {
+  if (!p) return true;
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+template <typename T>
+class Staged {
+  using M = std::remove_const_t<T>;
+
+ public:
+  // in: copy host -> device before the call; out: device -> host after.
+  Staged(Context* ctx, Scratch& scratch, T* user, uint64_t count, bool in, bool out,
+         hipStream_t st)
+      : user_(user), count_(count), out_(out), st_(st) {
+    (void)ctx;
+    if (user == nullptr || is_device_ptr(user)) {
+      dev_ = const_cast<M*>(user);
+      return;
+    }
+    dev_ = scratch.get<M>(count ? count : 1);
+    staged_ = true;
+    if (in && count)
+      KC_HIP(hipMemcpyAsync(dev_, user, count * sizeof(T), hipMemcpyHostToDevice, st));
+  }
+  M* dev() const { return dev_; }
+  void finish() {
+    if (staged_ && out_ && count_)
+      KC_HIP(hipMemcpyAsync(const_cast<M*>(user_), dev_, count_ * sizeof(T), hipMemcpyDeviceToHost, st_));
+  }
+
+ private:
+  T* user_;
+  M* dev_ = nullptr;
+  uint64_t count_;
+  bool out_;
+  bool staged_ = false;
+  hipStream_t st_;
+};
+
+}  // namespace kc
+
+using kc::Context;
+using kc::KcError;
+using kc::Scratch;
+using kc::Staged;
+
+namespace {
+
+template <typename F>
+kc_status guarded(void* vctx, F&& body) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return KC_EINVAL;
+  try {
+    KC_HIP(hipSetDevice(ctx->device));
+    body(ctx);
+    if (ctx->prof.on) ctx->prof.flush();
+    ctx->last_error.clear();
+    return KC_OK;
+  } catch (const KcError& e) {
+    ctx->last_error = e.what;
+    if (e.status != KC_EDEVICE) {
+      // leave the context usable: drain the stream and clear the error word
+      (void)hipStreamSynchronize(ctx->stream);
+      (void)hipMemset(ctx->d_err, 0, sizeof(uint32_t));
+    }
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    ctx->last_error = "device allocation failed";
+    return KC_ENOMEM;
+  }
+}
+
+hipStream_t pick(Context* ctx, void* stream) {
+  return stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+}
+
+}  // namespace
+
+extern "C" {
+
+kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void** ctx_out) {
+  if (!ctx_out) return KC_EINVAL;
+  *ctx_out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return KC_EDEVICE;
+  auto* ctx = new (std::nothrow) Context();
+  if (!ctx) return KC_ENOMEM;
+  ctx->device = device;
+  ctx->budget = mem_budget_bytes;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&ctx->d_err, 256) != hipSuccess || hipMalloc(&ctx->d_counters, 64 * sizeof(uint64_t)) != hipSuccess ||
+      hipHostMalloc(&ctx->h_pinned, 64 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
+      hipMemset(ctx->d_err, 0, 256) != hipSuccess || hipMemset(ctx->d_counters, 0, 64 * sizeof(uint64_t)) != hipSuccess) {
+    kc_context_destroy(ctx);
+    return KC_EDEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  *ctx_out = ctx;
+  return KC_OK;
+}
+
+void kc_context_destroy(void* vctx) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->arena.release(0);
+  ctx->arena.trim();
+  if (ctx->d_err) (void)hipFree(ctx->d_err);
+  if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+  if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* kc_last_error(void* vctx) {
+  Context* ctx = static_cast<Context*>(vctx);
+  return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+uint64_t kc_context_scratch_bytes(void* vctx) {
+  Context* ctx = static_cast<Context*>(vctx);
+  return ctx ? ctx->arena.held() : 0;
+}
+
+kc_status kc_context_trim(void* vctx) {
+  return guarded(vctx, [&](Context* ctx) {
+    KC_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->arena.trim();
+  });
+}
+
+kc_status kc_rank_list(void* vctx, const uint64_t* succ, const int64_t* weight, uint64_t n,
+                       uint64_t* root_out, int64_t* rank_out, const kc_rank_cfg* cfg,
+                       void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!succ || !root_out || !rank_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<const int64_t> w(ctx, scratch, weight, n, true, false, st);
+    Staged<uint64_t> ro(ctx, scratch, root_out, n, false, true, st);
+    Staged<int64_t> ra(ctx, scratch, rank_out, n, false, true, st);
+    kc::rank_list_device(ctx, s.dev(), w.dev(), n, ro.dev(), ra.dev(), cfg, st);
+    ro.finish();
+    ra.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_pointer_doubling(void* vctx, const kc_rank_cfg* cfg, uint64_t* succ_array,
+                              int64_t* rank_array, uint64_t n, void* stream) {
+  kc_rank_cfg c{};
+  if (cfg) c = *cfg;
+  c.algorithm = KC_ALGO_POINTER_DOUBLING;
+  return kc_rank_list(vctx, succ_array, rank_array, n, succ_array, rank_array, &c, stream);
+}
+
+kc_status kc_sparse_ruling_set(void* vctx, const kc_rank_cfg* cfg, uint64_t* succ_array,
+                               int64_t* rank_array, uint64_t n, void* stream) {
+  kc_rank_cfg c{};
+  if (cfg) c = *cfg;
+  c.algorithm = KC_ALGO_SPARSE_RULING_SET;
+  return kc_rank_list(vctx, succ_array, rank_array, n, succ_array, rank_array, &c, stream);
+}
+
+kc_status kc_pointer_doubling_rounds(void* vctx, const uint64_t* succ, const int64_t* weight,
+                                     uint64_t n, uint32_t rounds, uint64_t* succ_out,
+                                     int64_t* rank_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!succ || !succ_out || !rank_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<const int64_t> w(ctx, scratch, weight, n, true, false, st);
+    Staged<uint64_t> so(ctx, scratch, succ_out, n, false, true, st);
+    Staged<int64_t> ro(ctx, scratch, rank_out, n, false, true, st);
+    kc::pd_rounds_device(ctx, s.dev(), w.dev(), n, rounds, so.dev(), ro.dev(), st);
+    so.finish();
+    ro.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_rank_lists_segmented(void* vctx, const uint32_t* succ_local, const uint64_t* seg_off,
+                                  uint64_t nseg, const int64_t* weight, uint32_t* root_local,
+                                  int64_t* rank_out, const kc_rank_cfg* cfg, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    hipStream_t st = pick(ctx, stream);
+    if (!seg_off) throw KcError{KC_EINVAL, "null seg_off"};
+    Scratch scratch(ctx);
+    Staged<const uint64_t> off(ctx, scratch, seg_off, nseg + 1, true, false, st);
+    uint64_t total = 0, first = 0;
+    KC_HIP(hipMemcpyAsync(&first, off.dev(), sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    KC_HIP(hipMemcpyAsync(&total, off.dev() + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    KC_HIP(hipStreamSynchronize(st));
+    if (first != 0) throw KcError{KC_EINVAL, "seg_off[0] must be 0"};
+    if (total && (!succ_local || !root_local || !rank_out)) throw KcError{KC_EINVAL, "null array"};
+    Staged<const uint32_t> s(ctx, scratch, succ_local, total, true, false, st);
+    Staged<const int64_t> w(ctx, scratch, weight, total, true, false, st);
+    Staged<uint32_t> ro(ctx, scratch, root_local, total, false, true, st);
+    Staged<int64_t> ra(ctx, scratch, rank_out, total, false, true, st);
+    kc::rank_segmented_device(ctx, s.dev(), off.dev(), nseg, total, w.dev(), ro.dev(), ra.dev(), cfg, st);
+    ro.finish();
+    ra.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_root_tree(void* vctx, const uint64_t* parent, uint64_t n, uint64_t* root_out,
+                       int64_t* depth_out, const kc_tree_cfg* cfg, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!parent || !root_out || !depth_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> p(ctx, scratch, parent, n, true, false, st);
+    Staged<uint64_t> ro(ctx, scratch, root_out, n, false, true, st);
+    Staged<int64_t> de(ctx, scratch, depth_out, n, false, true, st);
+    kc::root_tree_device(ctx, p.dev(), n, ro.dev(), de.dev(), cfg, st);
+    ro.finish();
+    de.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_rank_via_euler_tour(void* vctx, const kc_tree_cfg* cfg, uint64_t* succ_array,
+                                 int64_t* rank_array, uint64_t n, void* stream) {
+  return kc_root_tree(vctx, succ_array, n, succ_array, rank_array, cfg, stream);
+}
+
+kc_status kc_compute_euler_tour(void* vctx, const uint64_t* parent, uint64_t n,
+                                uint64_t* tour_succ_out, int64_t* tour_weight_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!parent || !tour_succ_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> p(ctx, scratch, parent, n, true, false, st);
+    Staged<uint64_t> so(ctx, scratch, tour_succ_out, 2 * n, false, true, st);
+    Staged<int64_t> wo(ctx, scratch, tour_weight_out, 2 * n, false, true, st);
+    kc::Rec* tour = scratch.get<kc::Rec>(2 * n + 1);
+    kc::euler_tour_device(ctx, p.dev(), n, tour, st);
+    KC_HIP(hipMemcpy2DAsync(so.dev(), sizeof(uint64_t), &tour[0].s, sizeof(kc::Rec), sizeof(uint64_t),
+                            2 * n, hipMemcpyDeviceToDevice, st));
+    if (wo.dev())
+      KC_HIP(hipMemcpy2DAsync(wo.dev(), sizeof(int64_t), &tour[0].w, sizeof(kc::Rec), sizeof(int64_t),
+                              2 * n, hipMemcpyDeviceToDevice, st));
+    so.finish();
+    wo.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_gen_list(void* vctx, uint64_t n, uint64_t seed, int weighted, int local,
+                      uint64_t* succ_out, int64_t* weight_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<uint64_t> so(ctx, scratch, succ_out, n, false, true, st);
+    Staged<int64_t> wo(ctx, scratch, weight_out, n, false, true, st);
+    kc::gen_list_device(n, seed, weighted, local, so.dev(), wo.dev(), st);
+    so.finish();
+    wo.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_gen_tree(void* vctx, uint64_t n, uint64_t seed, uint64_t* parent_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<uint64_t> po(ctx, scratch, parent_out, n, false, true, st);
+    kc::gen_tree_device(n, seed, po.dev(), st);
+    po.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_gen_lists_segmented(void* vctx, const uint64_t* seg_off, uint64_t nseg,
+                                 const uint64_t* seeds, const uint8_t* local_flags, int weighted,
+                                 uint32_t* succ_out, int64_t* weight_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    hipStream_t st = pick(ctx, stream);
+    if (!seg_off || !seeds) throw KcError{KC_EINVAL, "null array"};
+    Scratch scratch(ctx);
+    Staged<const uint64_t> off(ctx, scratch, seg_off, nseg + 1, true, false, st);
+    Staged<const uint64_t> sd(ctx, scratch, seeds, nseg, true, false, st);
+    Staged<const uint8_t> lf(ctx, scratch, local_flags, nseg, true, false, st);
+    uint64_t total = 0;
+    KC_HIP(hipMemcpyAsync(&total, off.dev() + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    KC_HIP(hipStreamSynchronize(st));
+    Staged<uint32_t> so(ctx, scratch, succ_out, total, false, true, st);
+    Staged<int64_t> wo(ctx, scratch, weight_out, total, false, true, st);
+    kc::gen_lists_segmented_device(off.dev(), nseg, total, sd.dev(), lf.dev(), weighted, so.dev(),
+                                   wo.dev(), st);
+    so.finish();
+    wo.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"
+
+namespace kc {
+void initial_state_device(const uint64_t* succ, uint64_t n, uint64_t* root, int64_t* rank,
+                          hipStream_t st);
+void pack_device(Context* ctx, const uint64_t* succ, const int64_t* rank, uint64_t n,
+                 const uint64_t* active, uint64_t na, uint64_t* psucc, int64_t* prank,
+                 hipStream_t st);
+void unpack_device(Context* ctx, const uint64_t* psucc, const int64_t* prank,
+                   const uint64_t* active, uint64_t na, uint64_t n, uint64_t* succ, int64_t* rank,
+                   hipStream_t st);
+}  // namespace kc
+
+extern "C" {
+
+kc_status kc_set_initial_ranking_state(void* vctx, const uint64_t* succ, uint64_t n,
+                                       uint64_t* root_out, int64_t* rank_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!succ || !root_out || !rank_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<uint64_t> ro(ctx, scratch, root_out, n, false, true, st);
+    Staged<int64_t> ra(ctx, scratch, rank_out, n, false, true, st);
+    kc::initial_state_device(s.dev(), n, ro.dev(), ra.dev(), st);
+    KC_HIP(hipGetLastError());
+    ro.finish();
+    ra.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_pack(void* vctx, const uint64_t* succ, const int64_t* rank, uint64_t n,
+                  const uint64_t* active, uint64_t nactive, uint64_t* psucc_out,
+                  int64_t* prank_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (nactive && (!succ || !rank || !active || !psucc_out || !prank_out))
+      throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<const int64_t> r(ctx, scratch, rank, n, true, false, st);
+    Staged<const uint64_t> a(ctx, scratch, active, nactive, true, false, st);
+    Staged<uint64_t> ps(ctx, scratch, psucc_out, nactive, false, true, st);
+    Staged<int64_t> pr(ctx, scratch, prank_out, nactive, false, true, st);
+    kc::pack_device(ctx, s.dev(), r.dev(), n, a.dev(), nactive, ps.dev(), pr.dev(), st);
+    ps.finish();
+    pr.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_unpack(void* vctx, const uint64_t* psucc, const int64_t* prank,
+                    const uint64_t* active, uint64_t nactive, uint64_t* succ_inout,
+                    int64_t* rank_inout, uint64_t n, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (nactive && (!psucc || !prank || !active || !succ_inout || !rank_inout))
+      throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> ps(ctx, scratch, psucc, nactive, true, false, st);
+    Staged<const int64_t> pr(ctx, scratch, prank, nactive, true, false, st);
+    Staged<const uint64_t> a(ctx, scratch, active, nactive, true, false, st);
+    Staged<uint64_t> s(ctx, scratch, succ_inout, n, true, true, st);
+    Staged<int64_t> r(ctx, scratch, rank_inout, n, true, true, st);
+    kc::unpack_device(ctx, ps.dev(), pr.dev(), a.dev(), nactive, n, s.dev(), r.dev(), st);
+    s.finish();
+    r.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+kc_status kc_profile_enable(void* vctx, int on) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return KC_EINVAL;
+  ctx->prof.on = on != 0;
+  return KC_OK;
+}
+
+kc_status kc_profile_reset(void* vctx) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return KC_EINVAL;
+  ctx->prof.reset();
+  return KC_OK;
+}
+
+uint32_t kc_profile_read(void* vctx, uint32_t max_entries, const char** names, double* ms,
+                         uint64_t* counts) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return 0;
+  ctx->prof.flush();
+  uint32_t k = 0;
+  for (auto& t : ctx->prof.totals) {
+    if (k >= max_entries) break;
+    if (names) names[k] = t.name.c_str();
+    if (ms) ms[k] = t.ms;
+    if (counts) counts[k] = t.count;
+    k++;
+  }
+  return static_cast<uint32_t>(ctx->prof.totals.size());
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+// Internal declarations shared by the HIP sources of libkascade_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/kascade_amd.h"
+
+namespace kc {
+
+constexpr uint64_t kRootFlag = uint64_t{1} << 63;   // lib/kascade/bits.hpp:9
+constexpr uint64_t kAuxFlag = uint64_t{1} << 62;    // marks "unclaimed, packed" records
+constexpr uint64_t kIdMask = kAuxFlag - 1;
+
+// One list element as the kernels see it: successor id (with flags) and the
+// weight / distance that travels with it.  16 bytes so a random access is a
+// single 16-byte load inside one 64-byte HBM burst.
+struct alignas(16) Rec {
+  uint64_t s;
+  int64_t w;
+};
+
+// Device-side error word: kernels OR these bits in; the host maps them to kc_status.
+enum ErrBits : uint32_t {
+  kErrRange = 1u,      // successor id >= n
+  kErrRootWeight = 2u, // nonzero weight on a tail
+  kErrNotList = 4u,    // an element reached twice (in-degree > 1)
+  kErrCycle = 8u,      // a cycle without a tail
+};
+
+// Scratch pool: grow-only chunks reused across calls, handed out as a stack.
+class Arena {
+ public:
+  ~Arena();
+  // Returns nullptr on allocation failure.
+  void* alloc(size_t bytes);
+  size_t mark() const { return top_; }
+  void release(size_t mark);
+  void trim();
+  uint64_t held() const;
+
+ private:
+  struct Chunk {
+    char* base;
+    size_t size;
+  };
+  struct Block {
+    size_t chunk;
+    size_t offset;
+    size_t size;
+  };
+  std::vector<Chunk> chunks_;
+  std::vector<Block> blocks_;
+  size_t top_ = 0;  // number of live blocks
+};
+
+// Per-phase device timing with HIP events on the launching stream (the
+// library's counterpart of the reference's kamping phase timers).
+class Profiler {
+ public:
+  bool on = false;
+  void begin(const char* name, hipStream_t st);
+  void end(hipStream_t st);
+  // Resolves recorded events (the stream must be synchronised).
+  void flush();
+  void reset();
+  struct Acc {
+    std::string name;
+    double ms = 0;
+    uint64_t count = 0;
+  };
+  std::vector<Acc> totals;
+  ~Profiler();
+
+ private:
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  hipEvent_t get();
+  std::vector<hipEvent_t> pool_;
+  std::vector<Pending> pending_;
+  std::vector<std::pair<std::string, hipEvent_t>> open_;
+};
+
+struct Context {
+  Profiler prof;
+  int device = 0;
+  uint64_t budget = 0;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  uint32_t* d_err = nullptr;       // error word
+  uint64_t* d_counters = nullptr;  // small device counters (64 slots)
+  uint64_t* h_pinned = nullptr;    // pinned host mirror of counters
+  std::string last_error;
+  int num_cus = 256;
+};
+
+// Scoped stack allocation from the arena.
+class Scratch {
+ public:
+  explicit Scratch(Context* ctx) : ctx_(ctx), mark_(ctx->arena.mark()) {}
+  ~Scratch() { ctx_->arena.release(mark_); }
+  template <typename T>
+  T* get(uint64_t count) {
+    void* p = ctx_->arena.alloc(count * sizeof(T) + 256);
+    if (!p) throw std::bad_alloc();
+    return static_cast<T*>(p);
+  }
+
+ private:
+  Context* ctx_;
+  size_t mark_;
+};
+
+struct KcError {
+  kc_status status;
+  std::string what;
+};
+
+#define KC_HIP(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw ::kc::KcError{KC_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+// RAII phase timer: no-op unless the context's profiler is on.
+class Phase {
+ public:
+  Phase(Context* ctx, const char* name, hipStream_t st) : ctx_(ctx), st_(st) {
+    if (ctx_->prof.on) ctx_->prof.begin(name, st_);
+  }
+  ~Phase() {
+    if (ctx_->prof.on) ctx_->prof.end(st_);
+  }
+
+ private:
+  Context* ctx_;
+  hipStream_t st_;
+};
+
+inline unsigned blocks_for(uint64_t n, unsigned threads) {
+  uint64_t b = (n + threads - 1) / threads;
+  return static_cast<unsigned>(b ? b : 1);
+}
+
+// Reads the device error word (synchronising `stream`) and throws on error.
+void check_err(Context* ctx, hipStream_t stream);
+// Copies `count` counters starting at slot `first` to the host (synchronising).
+void read_counters(Context* ctx, hipStream_t stream, int first, int count, uint64_t* out);
+
+// ---- tuning ----------------------------------------------------------------
+struct RankTuning {
+  uint32_t spacing_top;   // elements per ruler at the top level
+  uint32_t spacing_deep;  // elements per ruler below
+  uint32_t max_levels;
+  uint32_t small_n;       // sizes at or below this go to the one-workgroup solver
+};
+RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg);
+
+// ---- internal entry points (device pointers only) -----------------------
+// Level-0 ranking of separate succ / weight arrays into root / rank arrays.
+void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
+                      uint64_t* root, int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st);
+// Ranks packed records (ids < n, consumed) into root (ids) / rank; only the
+// first n_out outputs are written.  max_tails bounds the number of lists.
+void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                  int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
+                  hipStream_t st);
+void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
+                      uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
+// Packs succ/weight into records with validation (range, tail weight).
+void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, Rec* rec,
+                  hipStream_t st);
+void validate_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
+void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
+                      int64_t* depth, const kc_tree_cfg* cfg, hipStream_t st);
+void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
+                       hipStream_t st);
+void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* seg_off,
+                           uint64_t nseg, uint64_t total, const int64_t* w, uint32_t* root,
+                           int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st);
+
+// generators
+void gen_list_device(uint64_t n, uint64_t seed, int weighted, int local, uint64_t* succ,
+                     int64_t* w, hipStream_t st);
+void gen_tree_device(uint64_t n, uint64_t seed, uint64_t* parent, hipStream_t st);
+void gen_lists_segmented_device(const uint64_t* seg_off, uint64_t nseg, uint64_t total,
+                                const uint64_t* seeds, const uint8_t* local_flags, int weighted,
+                                uint32_t* succ, int64_t* w, hipStream_t st);
+
+// Device radix sort of (key, value) pairs by key, stable, keys < 2^key_bits.
+void radix_sort_pairs_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t n,
+                          unsigned key_bits, hipStream_t st);
+void radix_sort_pairs_u64(Context* ctx, uint64_t* keys, uint64_t* vals, uint64_t n,
+                          unsigned key_bits, hipStream_t st);
+// Exclusive scan of u64 (returns total on the host).
+uint64_t exclusive_scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n,
+                            hipStream_t st);
+
+}  // namespace kc

@@ -1,0 +1,63 @@
+// Device helpers shared by the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "common.hpp"
+
+namespace kc {
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// Rulers: one per block of K = 2^logK consecutive ids, at a hashed offset inside
+// the block.  Membership and the ruler's compact id (= its block) are computed,
+// never looked up, so a chase tests "is this a ruler" without a memory access.
+__device__ __forceinline__ uint64_t ruler_of_block(uint64_t b, uint64_t n, uint32_t logK,
+                                                   uint32_t seed) {
+  uint64_t start = b << logK;
+  uint64_t full = uint64_t{1} << logK;
+  uint64_t len = n - start < full ? n - start : full;
+  uint32_t h = hash32(static_cast<uint32_t>(b) ^ (static_cast<uint32_t>(b >> 32) * 0x9E3779B9u) ^ seed);
+  uint64_t off = (len == full) ? (h & (full - 1)) : (h % len);
+  return start + off;
+}
+__device__ __forceinline__ bool is_ruler(uint64_t v, uint64_t n, uint32_t logK, uint32_t seed) {
+  return ruler_of_block(v >> logK, n, logK, seed) == v;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Wave-aggregated atomic add of `mine` (0 or 1 per lane) on a 64-bit counter;
+// returns this lane's slot.  Every lane of the wave must call it.
+__device__ __forceinline__ uint64_t wave_append(unsigned long long* ctr, bool mine) {
+  uint64_t mask = __ballot(mine);
+  if (mask == 0) return 0;
+  int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  unsigned long long base = 0;
+  if (static_cast<int>(__lane_id()) == leader) base = atomicAdd(ctr, static_cast<unsigned long long>(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + __popcll(mask & lanemask_lt());
+}
+
+__device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bits) { atomicOr(err, bits); }
+
+}  // namespace kc

@@ -1,0 +1,122 @@
+// Euler-tour tree rooting (lib/kascade/eulertour.cpp:33-127, :235-278,
+// :337-377) on one device.
+//
+// Element numbering of the tour (2n elements, built without a hash map):
+//   element v      (v non-root): upward edge   (v -> parent v), weight +1
+//   element n + v  (v non-root): downward edge (parent v -> v), weight -1
+//   element r      (r a root)  : the tour's terminal dummy (rank 0, self loop),
+//                                the per-root dummy of eulertour.cpp:92-104
+//   element n + r  (r a root)  : unused, an isolated self loop
+// Children are grouped by parent with a stable device radix sort (the CSR of
+// reverse_rooted_tree / make_graph, successor_utils.cpp:236-300).  With the
+// cyclic adjacency order [parent, children...] of each vertex the tour links
+// are (compute_euler_tour_impl, succ[(u_prev, v)] = (v, u_next)):
+//   down(c)   -> down(first child of c), or up(c) if c is a leaf
+//   up(c)     -> down(next sibling of c), or up(parent) after the last child;
+//                for a root parent that is the root's dummy (element = root)
+// Ranking the tour gives, for element v < n, root = the tree root (the dummy's
+// element id is the root's vertex id) and rank = depth(v): exactly the
+// (root, depth) that map_euler_tour_back writes.  Elements >= n are never
+// written out.
+#include "common.hpp"
+#include "device.hpp"
+
+namespace kc {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads)
+tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __restrict__ keys,
+                 uint32_t* __restrict__ vals, Rec* __restrict__ tour, uint32_t* err,
+                 unsigned long long* roots) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool is_root = false;
+  if (v < n) {
+    uint64_t p = parent[v];
+    if (p >= n) {
+      raise_err(err, kErrRange);
+      p = v;
+    }
+    is_root = (p == v);
+    keys[v] = is_root ? static_cast<uint32_t>(n) : static_cast<uint32_t>(p);
+    vals[v] = static_cast<uint32_t>(v);
+    if (is_root) {
+      tour[v] = Rec{v, 0};          // terminal dummy
+      tour[n + v] = Rec{n + v, 0};  // unused slot
+    } else {
+      tour[n + v] = Rec{v, -1};     // leaf default: down(v) -> up(v)
+    }
+  }
+  wave_append(roots, is_root);
+}
+
+__global__ void __launch_bounds__(kThreads)
+tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                 const uint64_t* __restrict__ parent, uint64_t n, Rec* __restrict__ tour) {
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = keys[j];
+  if (p == n) return;  // roots sort last
+  const uint32_t c = vals[j];
+  const bool first = (j == 0) || keys[j - 1] != p;
+  const bool last = (j + 1 == n) || keys[j + 1] != p;
+  // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
+  tour[c] = Rec{last ? static_cast<uint64_t>(p) : n + vals[j + 1], 1};
+  // down(p) -> down(first child); a root p has no down edge
+  if (first && parent[p] != p) tour[n + p] = Rec{n + c, -1};
+}
+
+unsigned bits_for(uint64_t x) {
+  unsigned b = 1;
+  while (b < 64 && (uint64_t{1} << b) <= x) b++;
+  return b;
+}
+
+}  // namespace
+
+// Builds the 2n-element tour into `tour`; returns the number of roots through
+// counter slot 0.
+static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
+                           hipStream_t st) {
+  if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree rooting supports n < 2^32 - 1"};
+  Scratch scratch(ctx);
+  uint32_t* keys = scratch.get<uint32_t>(n);
+  uint32_t* vals = scratch.get<uint32_t>(n);
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+  hipLaunchKernelGGL(tree_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent,
+                     n, keys, vals, tour, ctx->d_err,
+                     reinterpret_cast<unsigned long long*>(ctx->d_counters));
+  uint64_t roots = 0;
+  read_counters(ctx, st, 0, 1, &roots);
+  check_err(ctx, st);
+  radix_sort_pairs_u32(ctx, keys, vals, n, bits_for(n), st);
+  hipLaunchKernelGGL(tour_link_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, keys,
+                     vals, parent, n, tour);
+  return roots;
+}
+
+void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
+                       hipStream_t st) {
+  if (n == 0) return;
+  build_tour(ctx, parent, n, tour, st);
+}
+
+void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
+                      int64_t* depth, const kc_tree_cfg* cfg, hipStream_t st) {
+  if (n == 0) return;
+  Scratch scratch(ctx);
+  Rec* tour = scratch.get<Rec>(2 * n);
+  const uint64_t roots = build_tour(ctx, parent, n, tour, st);
+  const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
+  RankTuning tune = tuning_for(2 * n, rc);
+  int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
+  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
+    throw KcError{KC_EINVAL, "invalid algorithm"};
+  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  // rank_via_euler_tour_select_algorithm (eulertour.cpp:282-321)
+  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st);
+  check_err(ctx, st);
+}
+
+}  // namespace kc

@@ -1,0 +1,566 @@
+// List ranking on one MI355X: sync pointer doubling and the sparse ruling set.
+//
+// Reference algorithms (niklas-uhl/kascade):
+//   pointer doubling     lib/kascade/pointer_doubling_generic.hpp:608-700
+//   sparse ruling set    lib/kascade/sparse_ruling_set.cpp:45-393
+//
+// Layout: the level-0 pass packs succ[] / weight[] into 16-byte records so that
+// every random access of the chase is one 16-byte load.  A claimed record is
+// overwritten in place with {ruler id | root flag, distance from the ruler}; an
+// unclaimed record keeps {successor, weight}.
+#include <algorithm>
+#include <cstdio>
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace kc {
+
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+// Counter slots in Context::d_counters.
+enum Slot : int {
+  kSlotTails = 0,
+  kSlotWork = 1,
+  kSlotExtra = 2,
+  kSlotUnclaimed = 3,
+  kSlotActive = 4,
+  kSlotCount = 8,
+};
+
+__global__ void zero_u64_kernel(unsigned long long* p, int count) {
+  int i = threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// set_initial_ranking_state semantics for NULL weights (list_ranking.cpp:60-80);
+// range and tail-weight validation; tail count.
+__global__ void __launch_bounds__(kThreads)
+pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, uint64_t n,
+            Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool tail = false;
+  if (v < n) {
+    uint64_t s = succ[v];
+    int64_t wt;
+    if (s >= n) {
+      raise_err(err, kErrRange);
+      s = v;
+    }
+    tail = (s == v);
+    if (w) {
+      wt = w[v];
+      if (tail && wt != 0) raise_err(err, kErrRootWeight);
+    } else {
+      wt = tail ? 0 : 1;
+    }
+    rec[v] = Rec{s, wt};
+  }
+  wave_append(tails, tail);
+}
+
+// ---------------------------------------------------------------------------
+// Ruler chase (sparse_ruling_set.cpp:157-265, sync engine
+// ruler_chasing_engine.hpp:57-101).  Persistent lanes: each lane chases one
+// ruler's segment and, when done, takes the next ruler from a global counter
+// (wave-aggregated), so lanes stay busy however uneven the ruler gaps are.
+// A chase claims every element it passes; it stops at the next ruler (a
+// computed predicate) or at a tail, which gets a fresh sublist id.  The
+// sublist link (ruler -> next ruler, gap) is the reversed-orientation message
+// of work_on_item :242-251, written directly in sublist ids.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
+             uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
+             unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
+             uint32_t refill_min) {
+  const int lane = static_cast<int>(__lane_id());
+  const uint64_t lt = lanemask_lt();
+  bool active = false;
+  bool exhausted = false;
+  uint64_t ruler = 0, cur = 0;
+  int64_t dist = 0;
+  for (;;) {
+    const uint64_t need = __ballot(!active && !exhausted);
+    const uint64_t act = __ballot(active);
+    if (need != 0 && (static_cast<uint32_t>(__popcll(need)) >= refill_min || act == 0)) {
+      const int leader = __ffsll(static_cast<long long>(need)) - 1;
+      unsigned long long base = 0;
+      if (lane == leader) base = atomicAdd(work_ctr, static_cast<unsigned long long>(__popcll(need)));
+      base = __shfl(base, leader);
+      if (!active && !exhausted) {
+        const uint64_t item = base + __popcll(need & lt);
+        if (item < nR) {
+          const uint64_t v0 = ruler_of_block(item, n, logK, seed);
+          const Rec r = rec[v0];
+          sub_orig[item] = v0;
+          rec[v0] = Rec{item | kRootFlag, 0};
+          if (r.s == v0) {
+            sub[item] = Rec{item, 0};  // the ruler is a tail
+          } else {
+            ruler = item;
+            cur = r.s;
+            dist = r.w;
+            active = true;
+          }
+        } else {
+          exhausted = true;
+        }
+      }
+      continue;
+    }
+    if (act == 0) break;  // need == 0 here: every lane is exhausted
+    if (active) {
+      if (is_ruler(cur, n, logK, seed)) {
+        const uint64_t id = cur >> logK;
+        if (id == ruler) raise_err(err, kErrCycle);  // came back to its own ruler
+        sub[ruler] = Rec{id, dist};
+        active = false;
+      } else {
+        const Rec r = rec[cur];
+        if (r.s & (kRootFlag | kAuxFlag)) {
+          raise_err(err, kErrNotList);  // reached twice: in-degree > 1
+          active = false;
+        } else if (r.s == cur) {
+          const uint64_t j = nR + atomicAdd(extra_ctr, 1ull);
+          if (j < cap) {
+            sub[j] = Rec{j, 0};
+            sub_orig[j] = cur;
+            rec[cur] = Rec{j | kRootFlag, 0};
+            sub[ruler] = Rec{j, dist};
+          } else {
+            raise_err(err, kErrNotList);
+          }
+          active = false;
+        } else {
+          rec[cur] = Rec{ruler | kRootFlag, dist};
+          dist += r.w;
+          cur = r.s;
+        }
+      }
+    }
+  }
+}
+
+// T[i].s (sublist root id) -> this level's id.
+__global__ void __launch_bounds__(kThreads)
+map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < m) T[i].s = sub_orig[T[i].s];
+}
+
+// Rank propagation (ruler_propagation.hpp:148-257, pull mode): every claimed
+// element reads its ruler's (root, rank) and subtracts its distance from the
+// ruler.  Unclaimed elements (list prefixes no ruler reaches, the analogue of
+// fixup_unreached, post_invert.hpp:195-264) are appended to a compact list and
+// their record is re-marked with that compact id.
+__global__ void __launch_bounds__(kThreads)
+propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
+                 uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride,
+                 uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig,
+                 unsigned long long* u_ctr) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  Rec r{0, 0};
+  bool unclaimed = false;
+  if (v < n) {
+    r = rec[v];
+    if (r.s & kRootFlag) {
+      if (v < n_out) {
+        const Rec t = T[r.s & kIdMask];
+        root[v * stride] = t.s;
+        rank[v * stride] = t.w - r.w;
+      }
+    } else {
+      unclaimed = true;
+    }
+  }
+  const uint64_t k = wave_append(u_ctr, unclaimed);
+  if (unclaimed) {
+    u_ids[k] = v;
+    u_orig[k] = r;
+    rec[v].s = k | kAuxFlag;
+  }
+}
+
+// Unclaimed sub-problem: an unclaimed element's successor is unclaimed (link
+// inside the sub-problem) or already final (it becomes a local tail whose
+// offset/root come from that final element), or it is a true tail.
+__global__ void __launch_bounds__(kThreads)
+unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
+                       const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T,
+                       Rec* __restrict__ urec, Rec* __restrict__ ext) {
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= nu) return;
+  const uint64_t v = u_ids[k];
+  const Rec r0 = u_orig[k];
+  const uint64_t s = r0.s;
+  if (s == v) {
+    urec[k] = Rec{k, 0};
+    ext[k] = Rec{v, 0};
+    return;
+  }
+  const Rec ms = rec[s];
+  if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
+    urec[k] = Rec{ms.s & kIdMask, r0.w};
+    ext[k] = Rec{0, 0};
+  } else {
+    // claimed: its final root / rank follow from its ruler's entry
+    const Rec t = T[ms.s & kIdMask];
+    urec[k] = Rec{k, 0};
+    ext[k] = Rec{t.s, r0.w + t.w - ms.w};
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+unclaimed_fix_kernel(const uint64_t* __restrict__ u_ids, uint64_t nu, uint64_t n_out,
+                     const Rec* __restrict__ ures, const Rec* __restrict__ ext,
+                     uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride) {
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= nu) return;
+  const uint64_t v = u_ids[k];
+  if (v >= n_out) return;
+  const Rec res = ures[k];
+  const Rec e = ext[res.s];
+  root[v * stride] = e.s;
+  rank[v * stride] = res.w + e.w;
+}
+
+// ---------------------------------------------------------------------------
+// One-workgroup solver for small levels: Jacobi pointer doubling in LDS.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSmallMax = 4096;
+constexpr unsigned kSmallThreads = 1024;
+constexpr uint32_t kSmallFlag = 0x80000000u;
+
+__global__ void __launch_bounds__(kSmallThreads)
+small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint64_t* __restrict__ root,
+                  int64_t* __restrict__ rank, uint32_t stride, uint32_t* err) {
+  __shared__ uint32_t s_succ[kSmallMax];
+  __shared__ int64_t s_rank[kSmallMax];
+  constexpr int kPer = kSmallMax / kSmallThreads;
+  for (uint32_t i = threadIdx.x; i < n; i += kSmallThreads) {
+    const Rec r = rec[i];
+    uint32_t s = static_cast<uint32_t>(r.s & kIdMask);
+    if ((r.s & (kRootFlag | kAuxFlag)) || r.s >= n) {
+      raise_err(err, kErrRange);
+      s = i;
+    }
+    if (s == i) {
+      if (r.w != 0) raise_err(err, kErrCycle);  // a self-loop carrying weight closes a cycle
+      s |= kSmallFlag;
+    }
+    s_succ[i] = s;
+    s_rank[i] = r.w;
+  }
+  __syncthreads();
+  uint32_t ns[kPer];
+  int64_t nr[kPer];
+  int rounds = 0;
+  for (;;) {
+    int any = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const uint32_t i = threadIdx.x + q * kSmallThreads;
+      if (i < n) {
+        const uint32_t s = s_succ[i];
+        ns[q] = s;
+        nr[q] = s_rank[i];
+        if (!(s & kSmallFlag)) {
+          ns[q] = s_succ[s];
+          nr[q] += s_rank[s];
+          any |= !(ns[q] & kSmallFlag);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
+      const uint32_t i = threadIdx.x + q * kSmallThreads;
+      if (i < n) {
+        s_succ[i] = ns[q];
+        s_rank[i] = nr[q];
+      }
+    }
+    rounds++;
+    const int more = __syncthreads_or(any);
+    if (!more) break;
+    if (rounds > 14) {
+      if (threadIdx.x == 0) raise_err(err, kErrCycle);
+      break;
+    }
+  }
+  for (uint32_t i = threadIdx.x; i < n_out; i += kSmallThreads) {
+    root[static_cast<uint64_t>(i) * stride] = s_succ[i] & ~kSmallFlag;
+    rank[static_cast<uint64_t>(i) * stride] = s_rank[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sync pointer doubling (Wyllie), Jacobi over double-buffered records: every
+// reply is taken from the state before the round, as the reference builds all
+// replies before any update (request_aggregation_scheme.hpp:156-179 then
+// update_via_read_offsets, pointer_doubling_generic.hpp:62-81).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads)
+pd_flag_roots_kernel(Rec* __restrict__ a, uint64_t n, uint32_t* err) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (v >= n) return;
+  Rec r = a[v];
+  if ((r.s & (kRootFlag | kAuxFlag)) || r.s >= n) {
+    raise_err(err, kErrRange);
+    r.s = v;
+  }
+  if (r.s == v) {  // initialize_active_vertices :418-438
+    if (r.w != 0) raise_err(err, kErrCycle);
+    r.s = v | kRootFlag;
+    a[v] = r;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
+                unsigned long long* active) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool still = false;
+  if (v < n) {
+    const Rec r = a[v];
+    if (r.s & kRootFlag) {
+      b[v] = r;
+    } else {
+      const Rec t = a[r.s];
+      b[v] = Rec{t.s, r.w + t.w};
+      still = !(t.s & kRootFlag);
+    }
+  }
+  wave_append(active, still);
+}
+
+__global__ void __launch_bounds__(kThreads)
+pd_write_kernel(const Rec* __restrict__ a, uint64_t n_out, uint64_t* __restrict__ root,
+                int64_t* __restrict__ rank, uint32_t stride, int keep_flag) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (v >= n_out) return;
+  const Rec r = a[v];
+  root[v * stride] = keep_flag ? r.s : (r.s & ~kRootFlag);
+  rank[v * stride] = r.w;
+}
+
+void zero_counters(Context* ctx, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st,
+                     reinterpret_cast<unsigned long long*>(ctx->d_counters), kSlotCount);
+}
+
+const char* const kChaseNames[] = {"chase_ruler_L0", "chase_ruler_L1", "chase_ruler_L2",
+                                   "chase_ruler_L3", "chase_ruler_L4", "chase_ruler_L5",
+                                   "chase_ruler_L6", "chase_ruler_L7", "chase_ruler_L8"};
+const char* const kPropNames[] = {"ruler_propagation_L0", "ruler_propagation_L1",
+                                  "ruler_propagation_L2", "ruler_propagation_L3",
+                                  "ruler_propagation_L4", "ruler_propagation_L5",
+                                  "ruler_propagation_L6", "ruler_propagation_L7",
+                                  "ruler_propagation_L8"};
+
+unsigned log2_floor(uint64_t x) {
+  unsigned r = 0;
+  while (x > 1) {
+    x >>= 1;
+    r++;
+  }
+  return r;
+}
+
+void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                       int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+                       int algorithm, int level, hipStream_t st);
+
+void pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                     int64_t* rank, uint32_t stride, hipStream_t st) {
+  Phase ph(ctx, "pointer_doubling", st);
+  Scratch scratch(ctx);
+  Rec* other = scratch.get<Rec>(n);
+  const unsigned blocks = blocks_for(n, kThreads);
+  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, rec, n, ctx->d_err);
+  Rec* a = rec;
+  Rec* b = other;
+  unsigned long long* active = reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotActive;
+  const unsigned max_rounds = log2_floor(n) + 2;
+  unsigned rounds = 0;
+  for (;;) {
+    zero_counters(ctx, st);
+    hipLaunchKernelGGL(pd_round_kernel, dim3(blocks), dim3(kThreads), 0, st, a, b, n, active);
+    std::swap(a, b);
+    rounds++;
+    uint64_t left = 0;
+    read_counters(ctx, st, kSlotActive, 1, &left);
+    if (left == 0) break;
+    if (rounds > max_rounds) {
+      check_err(ctx, st);
+      throw KcError{KC_ECYCLE, "pointer doubling did not terminate: cycle without a root"};
+    }
+  }
+  check_err(ctx, st);
+  hipLaunchKernelGGL(pd_write_kernel, dim3(blocks_for(n_out, kThreads)), dim3(kThreads), 0, st, a,
+                     n_out, root, rank, stride, 0);
+}
+
+void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+               int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+               int level, hipStream_t st) {
+  Scratch scratch(ctx);
+  const uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
+  const uint32_t logK = log2_floor(spacing);
+  const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
+  const uint64_t cap = nR + std::min<uint64_t>(max_tails, n);
+  Rec* sub = scratch.get<Rec>(cap);
+  Rec* T = scratch.get<Rec>(cap);
+  uint64_t* sub_orig = scratch.get<uint64_t>(cap);
+  const uint32_t seed = static_cast<uint32_t>(splitmix64(0xC0FFEEull + level * 0x1234567ull + n));
+
+  zero_counters(ctx, st);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  const uint64_t lanes_wanted = nR;
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * 8;
+  const unsigned chase_blocks = static_cast<unsigned>(
+      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
+  {
+    Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
+    hipLaunchKernelGGL(chase_kernel, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed,
+                       nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err, 8u);
+  }
+  check_err(ctx, st);
+  uint64_t extra = 0;
+  read_counters(ctx, st, kSlotExtra, 1, &extra);
+  const uint64_t m = nR + extra;
+
+  // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
+  rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, 2, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
+                    level + 1, st);
+  hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, T,
+                     sub_orig, m);
+
+  // propagation + collection of unclaimed elements
+  zero_counters(ctx, st);
+  uint64_t* u_ids = scratch.get<uint64_t>(n);
+  Rec* u_orig = scratch.get<Rec>(n);
+  {
+    Phase ph(ctx, kPropNames[std::min(level, 8)], st);
+    hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
+                       n, n_out, T, root, rank, stride, u_ids, u_orig, ctr + kSlotUnclaimed);
+  }
+  uint64_t nu = 0;
+  read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
+  if (nu == 0) return;
+  Phase ph(ctx, "unclaimed", st);
+  Rec* urec = scratch.get<Rec>(nu);
+  Rec* ext = scratch.get<Rec>(nu);
+  Rec* ures = scratch.get<Rec>(nu);
+  hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
+                     rec, u_ids, u_orig, nu, T, urec, ext);
+  rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, 2, nu, tune,
+                    nu > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1,
+                    st);
+  hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
+                     u_ids, nu, n_out, ures, ext, root, rank, stride);
+}
+
+void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                       int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+                       int algorithm, int level, hipStream_t st) {
+  if (n == 0) return;
+  if (n <= tune.small_n && n <= kSmallMax) {
+    Phase ph(ctx, "small_rank", st);
+    hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec,
+                       static_cast<uint32_t>(n), static_cast<uint32_t>(n_out), root, rank, stride,
+                       ctx->d_err);
+    return;
+  }
+  if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
+    pd_records_impl(ctx, rec, n, n_out, root, rank, stride, st);
+    return;
+  }
+  srs_level(ctx, rec, n, n_out, root, rank, stride, max_tails, tune, level, st);
+}
+
+}  // namespace
+
+RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg) {
+  RankTuning t{};
+  // Defaults from the spacing sweep recorded in the design notes.
+  t.spacing_top = n >= (uint64_t{1} << 24) ? 32 : 16;
+  t.spacing_deep = 16;
+  t.max_levels = 8;
+  t.small_n = kSmallMax;
+  if (cfg) {
+    auto pow2 = [](uint32_t x) { return x >= 2 && (x & (x - 1)) == 0; };
+    if (cfg->ruler_spacing) {
+      if (!pow2(cfg->ruler_spacing) || cfg->ruler_spacing > (1u << 20))
+        throw KcError{KC_EINVAL, "ruler_spacing must be a power of two in [2, 2^20]"};
+      t.spacing_top = cfg->ruler_spacing;
+    }
+    if (cfg->ruler_spacing_deep) {
+      if (!pow2(cfg->ruler_spacing_deep) || cfg->ruler_spacing_deep > (1u << 20))
+        throw KcError{KC_EINVAL, "ruler_spacing_deep must be a power of two in [2, 2^20]"};
+      t.spacing_deep = cfg->ruler_spacing_deep;
+    }
+    if (cfg->max_levels) t.max_levels = cfg->max_levels;
+  }
+  return t;
+}
+
+void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, Rec* rec,
+                  hipStream_t st) {
+  Phase ph(ctx, "pack", st);
+  zero_counters(ctx, st);
+  hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, succ, w,
+                     n, rec, ctx->d_err,
+                     reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotTails);
+}
+
+void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                  int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
+                  hipStream_t st) {
+  rank_records_impl(ctx, rec, n, n_out, root, rank, 1, max_tails, tune, algorithm, 0, st);
+}
+
+void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
+                      uint64_t* root, int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st) {
+  if (n == 0) return;
+  if (n >= kRootFlag >> 1) throw KcError{KC_EINVAL, "n must be below 2^62"};
+  RankTuning tune = tuning_for(n, cfg);
+  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
+  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
+    throw KcError{KC_EINVAL, "invalid algorithm"};
+  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  if (cfg && cfg->validate) validate_list(ctx, succ, n, st);
+  Scratch scratch(ctx);
+  Rec* rec = scratch.get<Rec>(n);
+  pack_records(ctx, succ, w, n, rec, st);
+  uint64_t tails = 0;
+  read_counters(ctx, st, kSlotTails, 1, &tails);
+  check_err(ctx, st);
+  rank_records_impl(ctx, rec, n, n, root, rank, 1, tails, tune, algo, 0, st);
+  check_err(ctx, st);
+}
+
+void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
+                      uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st) {
+  if (n == 0) return;
+  Scratch scratch(ctx);
+  Rec* a = scratch.get<Rec>(n);
+  Rec* b = scratch.get<Rec>(n);
+  pack_records(ctx, succ, w, n, a, st);
+  check_err(ctx, st);
+  const unsigned blocks = blocks_for(n, kThreads);
+  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, ctx->d_err);
+  unsigned long long* active = reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotActive;
+  for (uint32_t r = 0; r < rounds; r++) {
+    hipLaunchKernelGGL(pd_round_kernel, dim3(blocks), dim3(kThreads), 0, st, a, b, n, active);
+    std::swap(a, b);
+  }
+  hipLaunchKernelGGL(pd_write_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, succ_out, rank_out,
+                     1u, 1);
+  check_err(ctx, st);
+}
+
+}  // namespace kc

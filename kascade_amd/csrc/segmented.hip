@@ -1,0 +1,128 @@
+// Batched segmented ranking: nseg independent lists in one ranking pass.
+//
+// Segment s owns elements [seg_off[s], seg_off[s+1]) and stores successors as
+// local u32 ids.  The batch is ranked as one forest over global ids
+// seg_off[s] + local (the single-device analogue of the reference's block
+// Distribution, distribution.hpp:16-113, with the segment offsets as the
+// block offsets): one pack pass, one persistent chase launch over every
+// segment, one propagation pass.  Roots are written back as local ids.
+#include <algorithm>
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace kc {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+// Segment of element i, searching only the segments that overlap this block.
+__device__ __forceinline__ uint64_t find_segment(const uint64_t* __restrict__ seg_off,
+                                                 uint64_t lo, uint64_t hi, uint64_t i) {
+  // invariant: seg_off[lo] <= i < seg_off[hi + 1]
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) / 2;
+    if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ uint64_t upper_segment(const uint64_t* __restrict__ seg_off, uint64_t nseg, uint64_t i) {
+  // largest s with seg_off[s] <= i (empty segments skipped by taking the last)
+  uint64_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) / 2;
+    if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kThreads)
+seg_pack_kernel(const uint32_t* __restrict__ succ, const uint64_t* __restrict__ seg_off,
+                uint64_t nseg, uint64_t total, const int64_t* __restrict__ w,
+                Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
+  __shared__ uint64_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  const uint64_t lastv = std::min(first + kThreads, total) - 1;
+  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
+  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
+  __syncthreads();
+  const uint64_t i = first + threadIdx.x;
+  bool tail = false;
+  if (i < total) {
+    const uint64_t s = find_segment(seg_off, s_range[0], s_range[1], i);
+    const uint64_t base = seg_off[s];
+    const uint64_t size = seg_off[s + 1] - base;
+    uint64_t loc = succ[i];
+    if (loc >= size) {
+      raise_err(err, kErrRange);
+      loc = i - base;
+    }
+    const uint64_t g = base + loc;
+    tail = (g == i);
+    int64_t wt;
+    if (w) {
+      wt = w[i];
+      if (tail && wt != 0) raise_err(err, kErrRootWeight);
+    } else {
+      wt = tail ? 0 : 1;
+    }
+    rec[i] = Rec{g, wt};
+  }
+  wave_append(tails, tail);
+}
+
+__global__ void __launch_bounds__(kThreads)
+seg_root_local_kernel(const uint64_t* __restrict__ root_global, const uint64_t* __restrict__ seg_off,
+                      uint64_t nseg, uint64_t total, uint32_t* __restrict__ root_local) {
+  __shared__ uint64_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  const uint64_t lastv = std::min(first + kThreads, total) - 1;
+  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
+  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
+  __syncthreads();
+  const uint64_t i = first + threadIdx.x;
+  if (i >= total) return;
+  const uint64_t s = find_segment(seg_off, s_range[0], s_range[1], i);
+  root_local[i] = static_cast<uint32_t>(root_global[i] - seg_off[s]);
+}
+
+__global__ void seg_check_kernel(const uint64_t* __restrict__ seg_off, uint64_t nseg, uint32_t* err) {
+  uint64_t s = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (s >= nseg) return;
+  const uint64_t a = seg_off[s], b = seg_off[s + 1];
+  if (b < a || b - a > 0xFFFFFFFFull) raise_err(err, kErrRange);
+}
+
+}  // namespace
+
+void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* seg_off,
+                           uint64_t nseg, uint64_t total, const int64_t* w, uint32_t* root,
+                           int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st) {
+  if (nseg == 0 || total == 0) return;
+  hipLaunchKernelGGL(seg_check_kernel, dim3(blocks_for(nseg, kThreads)), dim3(kThreads), 0, st,
+                     seg_off, nseg, ctx->d_err);
+  check_err(ctx, st);
+  // size class = mean segment size
+  RankTuning tune = tuning_for(std::max<uint64_t>(1, total / nseg), cfg);
+  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
+  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
+    throw KcError{KC_EINVAL, "invalid algorithm"};
+  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  Scratch scratch(ctx);
+  Rec* rec = scratch.get<Rec>(total);
+  uint64_t* root_global = scratch.get<uint64_t>(total);
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+  hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st, succ,
+                     seg_off, nseg, total, w, rec, ctx->d_err,
+                     reinterpret_cast<unsigned long long*>(ctx->d_counters));
+  uint64_t tails = 0;
+  read_counters(ctx, st, 0, 1, &tails);
+  check_err(ctx, st);
+  rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st);
+  check_err(ctx, st);
+  hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st,
+                     root_global, seg_off, nseg, total, root);
+}
+
+}  // namespace kc

@@ -1,0 +1,147 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.  It checks the product; it is never the
+product path.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_ROOT, "oracle", "liboracle.so")
+_lib = None
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(_ROOT, "oracle")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c = ctypes
+        sig = {
+            "orc_splitmix64": (c.c_uint64, [c.c_uint64]),
+            "orc_perm": (c.c_uint64, [c.c_uint64, c.c_uint64, c.c_uint64]),
+            "orc_perm_inv": (c.c_uint64, [c.c_uint64, c.c_uint64, c.c_uint64]),
+            "orc_gen_list": (None, [c.c_uint64, c.c_uint64, c.c_int, u64p, i64p]),
+            "orc_gen_list_local": (None, [c.c_uint64, c.c_uint64, c.c_int, u64p, i64p]),
+            "orc_list_closed_form": (None, [c.c_uint64, c.c_uint64, i64p, u64p, i64p]),
+            "orc_gen_tree": (None, [c.c_uint64, c.c_uint64, u64p]),
+            "orc_tree_closed_form": (None, [c.c_uint64, c.c_uint64, u64p, i64p]),
+            "orc_local_pointer_chasing": (None, [u64p, i64p, c.c_uint64]),
+            "orc_set_initial_ranking_state": (None, [u64p, u64p, i64p, c.c_uint64]),
+            "orc_pd_rounds": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int]),
+            "orc_pointer_doubling": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int]),
+            "orc_pack": (c.c_int, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
+            "orc_unpack": (None, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
+            "orc_euler_tour_rooting": (c.c_int, [u64p, i64p, c.c_uint64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p64(a):
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.uint64:
+        return a.ctypes.data_as(u64p)
+    assert a.dtype == np.int64
+    return a.ctypes.data_as(i64p)
+
+
+def gen_list(n, seed, weighted=True, local=False):
+    succ = np.empty(n, np.uint64)
+    w = np.empty(n, np.int64)
+    f = lib().orc_gen_list_local if local else lib().orc_gen_list
+    f(n, seed, int(weighted), _p64(succ), _p64(w))
+    return succ, w
+
+
+def list_closed_form(n, seed, w):
+    root = np.empty(n, np.uint64)
+    rank = np.empty(n, np.int64)
+    lib().orc_list_closed_form(n, seed, _p64(np.ascontiguousarray(w)), _p64(root), _p64(rank))
+    return root, rank
+
+
+def gen_tree(n, seed):
+    parent = np.empty(n, np.uint64)
+    lib().orc_gen_tree(n, seed, _p64(parent))
+    return parent
+
+
+def tree_closed_form(n, seed):
+    root = np.empty(n, np.uint64)
+    depth = np.empty(n, np.int64)
+    lib().orc_tree_closed_form(n, seed, _p64(root), _p64(depth))
+    return root, depth
+
+
+def sequential_walk(succ, w):
+    """local_pointer_chasing on copies; returns (root, rank)."""
+    s = np.array(succ, dtype=np.uint64, copy=True)
+    r = np.array(w, dtype=np.int64, copy=True)
+    lib().orc_local_pointer_chasing(_p64(s), _p64(r), len(s))
+    return s, r
+
+
+def initial_ranking_state(succ):
+    succ = np.ascontiguousarray(succ, dtype=np.uint64)
+    root = np.empty_like(succ)
+    rank = np.empty(len(succ), np.int64)
+    lib().orc_set_initial_ranking_state(_p64(succ), _p64(root), _p64(rank), len(succ))
+    return root, rank
+
+
+def pd_rounds(succ, w, rounds):
+    s = np.array(succ, dtype=np.uint64, copy=True)
+    r = np.array(w, dtype=np.int64, copy=True)
+    k = lib().orc_pd_rounds(_p64(s), _p64(r), len(s), rounds)
+    return s, r, k
+
+
+def pointer_doubling(succ, w, p=4):
+    s = np.array(succ, dtype=np.uint64, copy=True)
+    r = np.array(w, dtype=np.int64, copy=True)
+    k = lib().orc_pointer_doubling(_p64(s), _p64(r), len(s), p)
+    assert k >= 0
+    return s, r, k
+
+
+def pack(succ, rank, active):
+    active = np.ascontiguousarray(active, dtype=np.uint64)
+    ps = np.empty(len(active), np.uint64)
+    pr = np.empty(len(active), np.int64)
+    rc = lib().orc_pack(_p64(np.ascontiguousarray(succ, dtype=np.uint64)),
+                        _p64(np.ascontiguousarray(rank, dtype=np.int64)),
+                        _p64(active), len(active), _p64(ps), _p64(pr))
+    assert rc == 0
+    return ps, pr
+
+
+def unpack(psucc, prank, active, succ, rank):
+    lib().orc_unpack(_p64(np.ascontiguousarray(psucc, dtype=np.uint64)),
+                     _p64(np.ascontiguousarray(prank, dtype=np.int64)),
+                     _p64(np.ascontiguousarray(active, dtype=np.uint64)), len(active),
+                     _p64(succ), _p64(rank))
+
+
+def euler_tour_rooting(parent):
+    s = np.array(parent, dtype=np.uint64, copy=True)
+    r = np.empty(len(s), np.int64)
+    rc = lib().orc_euler_tour_rooting(_p64(s), _p64(r), len(s))
+    assert rc == 0
+    return s, r

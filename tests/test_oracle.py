@@ -1,0 +1,227 @@
+"""Pins the CPU oracle (oracle/oracle.c) against checks that do not restate it:
+closed forms from the generators' construction order, brute-force walks,
+BFS depths, the reference's pack known-answer test, and hand-made cases."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def walk_brute(succ, w):
+    """Definition of rank: follow succ to the fixed point, summing weights."""
+    n = len(succ)
+    root = np.empty(n, np.uint64)
+    rank = np.empty(n, np.int64)
+    for v in range(n):
+        cur, acc, steps = v, 0, 0
+        while int(succ[cur]) != cur:
+            acc += int(w[cur])
+            cur = int(succ[cur])
+            steps += 1
+            assert steps <= n, "cycle"
+        root[v] = cur
+        rank[v] = acc
+    return root, rank
+
+
+def random_forest_lists(n, nlists, seed, weighted=True):
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    cuts = np.sort(rng.choice(np.arange(1, n), size=nlists - 1, replace=False)) if nlists > 1 else []
+    succ = np.arange(n, dtype=np.uint64)
+    w = np.zeros(n, np.int64)
+    for seg in np.split(perm, cuts):
+        for a, b in zip(seg[:-1], seg[1:]):
+            succ[a] = b
+            w[a] = rng.integers(1, 101) if weighted else 1
+    return succ, w
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 1000, 4097])
+@pytest.mark.parametrize("weighted", [False, True])
+def test_generator_is_one_list_and_closed_form(n, weighted):
+    succ, w = o.gen_list(n, 11 + n, weighted)
+    assert succ.max() < n
+    heads = np.setdiff1d(np.arange(n), succ[succ != np.arange(n)])
+    assert len(heads) == 1                       # exactly one head
+    tails = np.nonzero(succ == np.arange(n))[0]
+    assert len(tails) == 1 and w[tails[0]] == 0  # one tail, weight 0
+    if weighted and n > 1:
+        nt = w[succ != np.arange(n)]
+        assert nt.min() >= 1 and nt.max() <= 100
+    root, rank = o.list_closed_form(n, 11 + n, w)
+    if n <= 1000:
+        br, bk = walk_brute(succ, w)
+        assert (br == root).all() and (bk == rank).all()
+
+
+def test_local_generator():
+    n = 1003
+    succ, w = o.gen_list(n, 5, True, local=True)
+    br, bk = walk_brute(succ, w)
+    assert len(set(br.tolist())) == 1                       # a single list
+    assert np.mean(succ[:-1] == np.arange(1, n)) > 0.85     # mostly i -> i+1
+
+
+def test_perm_is_bijection_and_inverse():
+    for n in [1, 2, 7, 100, 1025]:
+        p = [o.lib().orc_perm(i, n, 99) for i in range(n)]
+        assert sorted(p) == list(range(n))
+        assert all(o.lib().orc_perm_inv(p[i], n, 99) == i for i in range(n))
+
+
+@pytest.mark.parametrize("n,nlists", [(1, 1), (10, 10), (50, 3), (2000, 17), (3000, 1)])
+def test_sequential_walk_matches_definition(n, nlists):
+    succ, w = random_forest_lists(n, nlists, n + nlists)
+    r, k = o.sequential_walk(succ, w)
+    br, bk = walk_brute(succ, w)
+    assert (r == br).all() and (k == bk).all()
+
+
+def test_sequential_walk_hand_case():
+    # 3 -> 0 -> 2 -> 1 (tail), weights 5, 7, 11; and an isolated 4
+    succ = np.array([2, 1, 1, 0, 4], np.uint64)
+    w = np.array([7, 0, 11, 5, 0], np.int64)
+    r, k = o.sequential_walk(succ, w)
+    assert r.tolist() == [1, 1, 1, 1, 4]
+    assert k.tolist() == [18, 0, 11, 23, 0]
+
+
+def test_initial_state():
+    succ = np.array([1, 1, 0, 3], np.uint64)
+    root, rank = o.initial_ranking_state(succ)
+    assert root.tolist() == [1, 1, 0, 3] and rank.tolist() == [1, 0, 1, 0]
+
+
+@pytest.mark.parametrize("p", [1, 4, 7])
+@pytest.mark.parametrize("n,nlists", [(1, 1), (5, 2), (1000, 1), (4096, 40), (20000, 3)])
+def test_pointer_doubling_ranks(p, n, nlists):
+    succ, w = random_forest_lists(n, nlists, 3 * n + p)
+    r, k, rounds = o.pointer_doubling(succ, w, p)
+    er, ek = o.sequential_walk(succ, w)
+    assert (r == er).all() and (k == ek).all()
+
+
+def test_pointer_doubling_local_lists_contract():
+    # locality-biased lists exercise local contraction / uncontraction
+    succ, w = o.gen_list(30011, 3, True, local=True)
+    r, k, _ = o.pointer_doubling(succ, w, 4)
+    root, rank = o.sequential_walk(succ, w)
+    assert (r == root).all() and (k == rank).all()
+
+
+@pytest.mark.parametrize("n", [1, 9, 200, 1023])
+def test_pd_rounds_closed_form(n):
+    """After k Jacobi rounds: succ_k(v) = 2^k-th successor clamped at the root,
+    rank_k(v) = sum of the 2^k weights passed (brute-force stepping), and the
+    root flag is set once the distance to the root is below 2^k."""
+    succ, w = random_forest_lists(n, max(1, n // 100), n)
+    s64 = succ.astype(np.int64)
+    is_root = s64 == np.arange(n)
+    # distance to the root by stepping
+    dist = np.zeros(n, np.int64)
+    cur = np.arange(n)
+    for _ in range(n):
+        moving = ~is_root[cur]
+        dist += moving
+        cur = s64[cur]
+    cur = np.arange(n)
+    acc = np.zeros(n, np.int64)
+    steps = 0
+    for k in range(0, 12):
+        while steps < (1 << k):
+            acc += w[cur]          # w is 0 on roots, succ[root] = root
+            cur = s64[cur]
+            steps += 1
+        s, r, _ = o.pd_rounds(succ, w, k)
+        flagged = (s >> np.uint64(63)).astype(bool)
+        assert ((s & np.uint64((1 << 63) - 1)).astype(np.int64) == cur).all(), k
+        assert (r == acc).all(), k
+        assert (flagged == (dist < (1 << k))).all(), k
+
+
+def test_pack_kat():
+    kat = json.load(open(os.path.join(GOLDEN, "pack_kat.json")))
+    off = kat["offsets"]
+    succ = np.concatenate([np.array(x, np.uint64) for x in kat["succ_per_rank"]])
+    rank = np.concatenate([np.array(x, np.int64) for x in kat["rank_per_rank"]])
+    active = np.concatenate([np.array(a, np.uint64) + off[r] for r, a in enumerate(kat["active_local_per_rank"])])
+    ps, pr = o.pack(succ, rank, active)
+    assert ps.tolist() == sum(kat["expected_packed_succ_per_rank"], [])
+    assert pr.tolist() == sum(kat["expected_packed_rank_per_rank"], [])
+    # simulated ranking: root 6, rank 6 - i - 2r
+    ps[:] = 6
+    pos = 0
+    for r, a in enumerate(kat["active_local_per_rank"]):
+        for i in range(len(a)):
+            pr[pos] = 6 - i - 2 * r
+            pos += 1
+    o.unpack(ps, pr, active, succ, rank)
+    assert succ.tolist() == sum(kat["expected_succ_after_unpack_per_rank"], [])
+    assert rank.tolist() == sum(kat["expected_rank_after_unpack_per_rank"], [])
+
+
+def test_pack_sanity_identity():
+    succ = np.arange(1, 13, dtype=np.uint64)
+    succ[-1] = 11
+    rank = np.ones(12, np.int64)
+    rank[-1] = 0
+    ps, pr = o.pack(succ, rank, np.arange(12, dtype=np.uint64))
+    assert (ps == succ).all() and (pr == rank).all()
+
+
+def bfs_depth(parent):
+    n = len(parent)
+    children = [[] for _ in range(n)]
+    roots = []
+    for v in range(n):
+        p = int(parent[v])
+        (roots.append(v) if p == v else children[p].append(v))
+    depth = np.full(n, -1, np.int64)
+    root = np.zeros(n, np.uint64)
+    for r in roots:
+        depth[r] = 0
+        root[r] = r
+        stack = [r]
+        while stack:
+            x = stack.pop()
+            for c in children[x]:
+                depth[c] = depth[x] + 1
+                root[c] = r
+                stack.append(c)
+    return root, depth
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 257, 5000])
+def test_tree_generator_and_euler_tour(n):
+    parent = o.gen_tree(n, 4 + n)
+    tr, td = o.tree_closed_form(n, 4 + n)
+    br, bd = bfs_depth(parent)
+    assert (tr == br).all() and (td == bd).all()
+    er, ed = o.euler_tour_rooting(parent)
+    assert (er == br).all() and (ed == bd).all()
+
+
+def test_euler_tour_forest_and_paths():
+    rng = np.random.default_rng(0)
+    # forest: several roots, a star, a path, isolated roots
+    n = 600
+    parent = np.arange(n, dtype=np.uint64)
+    for v in range(1, 200):
+        parent[v] = rng.integers(0, v)          # tree rooted at 0
+    for v in range(201, 300):
+        parent[v] = 200                         # star rooted at 200
+    for v in range(301, 400):
+        parent[v] = v - 1                       # path rooted at 300
+    # 400..599 isolated roots
+    sigma = rng.permutation(n)
+    relab = np.empty(n, np.uint64)
+    relab[sigma] = sigma[parent]
+    br, bd = bfs_depth(relab)
+    er, ed = o.euler_tour_rooting(relab)
+    assert (er == br).all() and (ed == bd).all()
