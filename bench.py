@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--check", action="store_true", help="verify the last step against the closed form")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=300)
+    ap.add_argument("--serve-slots", type=int, default=3)
     return ap.parse_args()
 
 
@@ -54,6 +57,77 @@ def cpu_baseline(n_sample, seed):
     return {"value": n_sample / dt, "unit": "elements/s", "cores": ranks, "kind": "port",
             "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}), sync pointer "
                       f"doubling on {ranks} emulated ranks, {rounds} rounds, {dt:.2f} s"}
+
+
+def serve_bench(ctx, jobs, seed, slots):
+    """Mixed serving stream (60 % lists as C2, 40 % trees as C3, sizes
+    log-uniform in [2^10, 2^24], device-resident arrays): one-job-per-launch
+    capacity, then Poisson arrivals at 0.8 x that capacity through both modes,
+    then the coalescing server at 3 x that offered load.  Every batched
+    result must equal the one-job-per-launch result bit for bit."""
+    import numpy as np
+    import torch
+    import kascade_amd
+
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(jobs):
+        n = int(2 ** rng.uniform(10, 24))
+        if rng.random() < 0.6:
+            s, w = ctx.gen_list(n, seed * 7919 + k, weighted=True)
+            pool.append(("list", s, w))
+        else:
+            pool.append(("tree", ctx.gen_tree(n, seed * 7919 + k), None))
+    outs = {m: [(torch.empty(p[1].numel(), dtype=torch.uint64, device="cuda"),
+                 torch.empty(p[1].numel(), dtype=torch.int64, device="cuda")) for p in pool]
+            for m in ("one", "batched")}
+    torch.cuda.synchronize()
+
+    def run(mode, arrivals):
+        sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one"), num_slots=slots)
+        # warm the slots' scratch pools
+        for k in range(min(8, jobs)):
+            kind, s, w = pool[k]
+            t, _, _ = sv.submit(kind, s, w, *outs[mode][k])
+            sv.wait(t)
+        sv.reset_stats()
+        t0 = time.perf_counter()
+        tickets = []
+        for k, (kind, s, w) in enumerate(pool):
+            if arrivals is not None:
+                delay = t0 + arrivals[k] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            tickets.append(sv.submit(kind, s, w, *outs[mode][k])[0])
+        for t in tickets:
+            sv.wait(t)
+        st = sv.stats()
+        sv.close()
+        return st
+
+    cap_one = run("one", None)["jobs_per_s"]
+    cap_batched = run("batched", None)["jobs_per_s"]
+    lam = 0.8 * cap_one
+    arr = np.cumsum(rng.exponential(1.0 / lam, size=jobs))
+    one = run("one", arr)
+    batched = run("batched", arr)
+    arr3 = arr / 3.0
+    batched3 = run("batched", arr3)
+    same = all(bool(torch.equal(outs["one"][k][0], outs["batched"][k][0]) and
+                    torch.equal(outs["one"][k][1], outs["batched"][k][1])) for k in range(jobs))
+    keep = ("jobs_per_s", "p50_latency_ms", "p99_latency_ms", "mean_latency_ms", "batches_launched")
+    return {
+        "jobs": jobs, "mix": "60% list (int64 weights [1,100]) / 40% tree, n log-uniform in [2^10, 2^24], "
+                             "arrays resident in HBM",
+        "capacity_one_job_per_launch_jobs_per_s": cap_one,
+        "capacity_batched_jobs_per_s": cap_batched,
+        "capacity_ratio": cap_batched / cap_one,
+        "offered_load_jobs_per_s": lam,
+        "one_job_per_launch": {k: one[k] for k in keep},
+        "batched": {k: batched[k] for k in keep},
+        "batched_at_3x_load": {"offered_load_jobs_per_s": 3 * lam, **{k: batched3[k] for k in keep}},
+        "bit_exact_alone_vs_mix": same,
+    }
 
 
 def main():
@@ -129,6 +203,10 @@ def main():
     }
     if check is not None:
         out["bit_exact_vs_closed_form"] = check
+    del succ, w, root, rank
+    torch.cuda.empty_cache()
+    if not args.no_serve:
+        out["serve"] = serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(min(args.cpu_sample, n), args.seed)
     ctx.close()

@@ -59,7 +59,9 @@ typedef struct kc_rank_cfg {
   uint32_t ruler_spacing_deep; /* elements per ruler below the top level, 0 = auto */
   uint32_t max_levels;      /* recursion depth cap (sparse_ruling_set_rounds), 0 = auto */
   uint32_t validate;        /* 1: reject in-degree > 1 before ranking (is_list, successor_utils.cpp:31-63) */
-  uint32_t reserved[3];
+  uint32_t chase_blocks_per_cu; /* resident 256-lane chase workgroups per CU, 0 = auto */
+  uint32_t chase_chunk;     /* rulers a wavefront takes per work-counter atomic, 0 = auto */
+  uint32_t reserved;
 } kc_rank_cfg;
 
 /* ---- context ------------------------------------------------------------ */
@@ -181,6 +183,19 @@ kc_status kc_gen_tree(void *ctx, uint64_t n, uint64_t seed, uint64_t *parent_out
 kc_status kc_gen_lists_segmented(void *ctx, const uint64_t *seg_off, uint64_t nseg,
                                  const uint64_t *seeds, const uint8_t *local_flags, int weighted,
                                  uint32_t *succ_out, int64_t *weight_out, void *stream);
+
+/* ---- diagnostics ------------------------------------------------------- */
+/* Random-gather ceiling: dst[i] = src[idx[i]] over n uint64 with idx a seeded
+ * random permutation (n = 2^27 exceeds every cache).  Reports the effective
+ * rate credited 24 bytes per element (idx 8 + src 8 + dst 8), in GB/s, and the
+ * kernel time.  The reference point for the list-ranking kernels' gathers. */
+kc_status kc_probe_random_gather(void *ctx, uint64_t n, uint32_t reps, double *gbps_out,
+                                 double *ms_out);
+/* The same with a memory mode: 0 plain, 1 non-temporal loads, 2 an uncached
+ * source allocation, 3 a fine-grained source allocation (2 and 3 are kept
+ * allocated until the process ends). */
+kc_status kc_probe_random_gather_mode(void *ctx, uint64_t n, uint32_t reps, int mode,
+                                      double *gbps_out, double *ms_out);
 
 /* ---- serving ------------------------------------------------------------ */
 typedef enum kc_job_kind { KC_JOB_LIST = 0, KC_JOB_TREE = 1 } kc_job_kind;

@@ -31,7 +31,8 @@ class KascadeError(RuntimeError):
 class RankCfg(ctypes.Structure):
     _fields_ = [("algorithm", ctypes.c_int32), ("ruler_spacing", ctypes.c_uint32),
                 ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
-                ("validate", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("validate", ctypes.c_uint32), ("chase_blocks_per_cu", ctypes.c_uint32),
+                ("chase_chunk", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class TreeCfg(ctypes.Structure):
@@ -89,6 +90,10 @@ _SIGS = {
     "kc_gen_list": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
     "kc_gen_tree": (ctypes.c_int, [_P, _U64, _U64, _P, _P]),
     "kc_gen_lists_segmented": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.c_int, _P, _P, _P]),
+    "kc_probe_random_gather": (ctypes.c_int, [_P, _U64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                                               ctypes.POINTER(ctypes.c_double)]),
+    "kc_probe_random_gather_mode": (ctypes.c_int, [_P, _U64, ctypes.c_uint32, ctypes.c_int,
+                                                    ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -138,8 +143,11 @@ def _numel(a):
     return a.size if isinstance(a, np.ndarray) else a.numel()
 
 
-def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False):
+def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
+         chase_blocks_per_cu=0, chase_chunk=0):
     c = RankCfg()
+    c.chase_blocks_per_cu = chase_blocks_per_cu
+    c.chase_chunk = chase_chunk
     c.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
     c.ruler_spacing = ruler_spacing
     c.ruler_spacing_deep = ruler_spacing_deep
@@ -328,6 +336,13 @@ class Context:
         k = self._lib.kc_profile_read(self.handle, k, names, ms, cnt)
         return {names[i].decode(): (ms[i], cnt[i]) for i in range(k)}
 
+    def probe_random_gather(self, n=1 << 27, reps=5, mode=0):
+        g = ctypes.c_double()
+        ms = ctypes.c_double()
+        self._check(self._lib.kc_probe_random_gather_mode(self.handle, n, reps, mode, ctypes.byref(g),
+                                                          ctypes.byref(ms)))
+        return g.value, ms.value
+
     def scratch_bytes(self):
         return self._lib.kc_context_scratch_bytes(self.handle)
 
@@ -362,7 +377,7 @@ class Server:
         self._keep = {}
 
     def submit(self, kind, succ, weight=None, root_out=None, rank_out=None):
-        n = succ.size
+        n = _numel(succ)
         root_out = np.empty(n, np.uint64) if root_out is None else root_out
         rank_out = np.empty(n, np.int64) if rank_out is None else rank_out
         j = Job()

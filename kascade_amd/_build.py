@@ -22,7 +22,8 @@ CFLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["api.cpp", "rank.hip", "gen.hip", "util.hip", "euler.hip", "segmented.hip", "server.cpp"]
+SOURCES = ["api.cpp", "rank.hip", "gen.hip", "util.hip", "euler.hip", "segmented.hip", "batch.hip",
+           "server.cpp", "probe.hip"]
 
 
 def _compile(src):

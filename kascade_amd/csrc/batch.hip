@@ -1,0 +1,205 @@
+// Mixed batches: list-ranking and tree-rooting jobs coalesced into one
+// ranking pass (the serving path).
+//
+// Record layout of a batch with N_l list elements and N_t tree vertices:
+//   [0, N_l)                 list elements, ids = list base + local id
+//   [N_l, N_l + N_t)         upward tour edges / per-root dummies of the trees
+//   [N_l + N_t, N_l + 2N_t)  downward tour edges
+// (the same tour numbering as the single-tree path, shifted by N_l).  One
+// ranking of all N_l + 2N_t records serves every job; outputs are copied back
+// to each job's arrays with local root ids.
+#include <algorithm>
+
+#include "batch.hpp"
+#include "device.hpp"
+
+namespace kc {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+// job containing element e: largest j with jobs[j].base <= e (bases ascending)
+__device__ __forceinline__ uint32_t job_of(const BatchJob* __restrict__ jobs, uint32_t lo, uint32_t hi,
+                                           uint64_t e) {
+  while (lo < hi) {
+    uint32_t mid = (lo + hi + 1) / 2;
+    if (jobs[mid].base <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void block_job_range(const BatchJob* __restrict__ jobs, uint32_t njobs,
+                                                uint64_t first, uint64_t last, uint32_t* s_range) {
+  if (threadIdx.x == 0) s_range[0] = job_of(jobs, 0, njobs - 1, first);
+  if (threadIdx.x == 1) s_range[1] = job_of(jobs, 0, njobs - 1, last);
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kThreads)
+batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nl,
+                       Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
+  __shared__ uint32_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  block_job_range(jobs, njobs, first, std::min(first + kThreads, nl) - 1, s_range);
+  const uint64_t e = first + threadIdx.x;
+  bool tail = false;
+  if (e < nl) {
+    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], e)];
+    const uint64_t i = e - j.base;
+    uint64_t s = j.succ[i];
+    if (s >= j.n) {
+      raise_err(err, kErrRange);
+      s = i;
+    }
+    tail = (s == i);
+    int64_t w;
+    if (j.w) {
+      w = j.w[i];
+      if (tail && w != 0) raise_err(err, kErrRootWeight);
+    } else {
+      w = tail ? 0 : 1;
+    }
+    rec[e] = Rec{j.base + s, w};
+  }
+  wave_append(tails, tail);
+}
+
+__global__ void __launch_bounds__(kThreads)
+batch_tree_keys_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
+                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                       uint8_t* __restrict__ is_root, Rec* __restrict__ rec, uint32_t* err,
+                       unsigned long long* roots) {
+  __shared__ uint32_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
+  const uint64_t t = first + threadIdx.x;
+  bool root = false;
+  if (t < nt) {
+    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
+    const uint64_t i = t - j.base;
+    uint64_t p = j.succ[i];
+    if (p >= j.n) {
+      raise_err(err, kErrRange);
+      p = i;
+    }
+    root = (p == i);
+    keys[t] = root ? static_cast<uint32_t>(nt) : static_cast<uint32_t>(j.base + p);
+    vals[t] = static_cast<uint32_t>(t);
+    is_root[t] = root;
+    if (root) {
+      rec[nl + t] = Rec{nl + t, 0};
+      rec[nl + nt + t] = Rec{nl + nt + t, 0};
+    } else {
+      rec[nl + nt + t] = Rec{nl + t, -1};
+    }
+  }
+  wave_append(roots, root);
+}
+
+__global__ void __launch_bounds__(kThreads)
+batch_tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                       const uint8_t* __restrict__ is_root, uint64_t nt, uint64_t nl,
+                       Rec* __restrict__ rec) {
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k >= nt) return;
+  const uint32_t p = keys[k];
+  if (p == nt) return;
+  const uint32_t c = vals[k];
+  const bool first = (k == 0) || keys[k - 1] != p;
+  const bool last = (k + 1 == nt) || keys[k + 1] != p;
+  rec[nl + c] = Rec{last ? nl + p : nl + nt + vals[k + 1], 1};
+  if (first && !is_root[p]) rec[nl + nt + p] = Rec{nl + nt + c, -1};
+}
+
+// Copies outputs back: list element e -> its job, tree element nl + t -> its job.
+__global__ void __launch_bounds__(kThreads)
+batch_scatter_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t count,
+                     uint64_t shift, const uint64_t* __restrict__ root, const int64_t* __restrict__ rank) {
+  __shared__ uint32_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  block_job_range(jobs, njobs, first, std::min(first + kThreads, count) - 1, s_range);
+  const uint64_t e = first + threadIdx.x;
+  if (e >= count) return;
+  const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], e)];
+  const uint64_t i = e - j.base;
+  j.root_out[i] = root[shift + e] - shift - j.base;
+  j.rank_out[i] = rank[shift + e];
+}
+
+unsigned bits_for(uint64_t x) {
+  unsigned b = 1;
+  while (b < 64 && (uint64_t{1} << b) <= x) b++;
+  return b;
+}
+
+}  // namespace
+
+void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const kc_rank_cfg* cfg,
+                       hipStream_t st) {
+  std::vector<BatchJob> lists, trees;
+  uint64_t nl = 0, nt = 0;
+  for (const BatchJob& j : jobs_in) {
+    if (j.n == 0) continue;
+    BatchJob c = j;
+    if (j.kind == KC_JOB_LIST) {
+      c.base = nl;
+      nl += j.n;
+      lists.push_back(c);
+    } else {
+      c.base = nt;
+      nt += j.n;
+      trees.push_back(c);
+    }
+  }
+  if (nl + nt == 0) return;
+  if (nt >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree vertices per batch must be < 2^32 - 1"};
+  Scratch scratch(ctx);
+  const uint64_t total = nl + 2 * nt;
+  Rec* rec = scratch.get<Rec>(total);
+  uint64_t* root = scratch.get<uint64_t>(nl + nt);
+  int64_t* rank = scratch.get<int64_t>(nl + nt);
+  BatchJob* d_lists = lists.empty() ? nullptr : scratch.get<BatchJob>(lists.size());
+  BatchJob* d_trees = trees.empty() ? nullptr : scratch.get<BatchJob>(trees.size());
+  if (d_lists)
+    KC_HIP(hipMemcpyAsync(d_lists, lists.data(), lists.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st));
+  if (d_trees)
+    KC_HIP(hipMemcpyAsync(d_trees, trees.data(), trees.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st));
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
+  if (nl) {
+    Phase ph(ctx, "batch_pack", st);
+    hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st,
+                       d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
+  }
+  uint32_t* keys = nullptr;
+  if (nt) {
+    Phase ph(ctx, "batch_tour", st);
+    keys = scratch.get<uint32_t>(nt);
+    uint32_t* vals = scratch.get<uint32_t>(nt);
+    uint8_t* is_root = scratch.get<uint8_t>(nt);
+    hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, keys, vals, is_root, rec,
+                       ctx->d_err, ctr + 1);
+    radix_sort_pairs_u32(ctx, keys, vals, nt, bits_for(nt), st);
+    hipLaunchKernelGGL(batch_tour_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, keys,
+                       vals, is_root, nt, nl, rec);
+  }
+  uint64_t counts[2] = {0, 0};
+  read_counters(ctx, st, 0, 2, counts);
+  check_err(ctx, st);
+  RankTuning tune = tuning_for(std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
+  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
+  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  rank_records(ctx, rec, total, nl + nt, root, rank, counts[0] + 2 * counts[1], tune, algo, st);
+  check_err(ctx, st);
+  Phase ph(ctx, "batch_scatter", st);
+  if (nl)
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st, d_lists,
+                       static_cast<uint32_t>(lists.size()), nl, uint64_t{0}, root, rank);
+  if (nt)
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, d_trees,
+                       static_cast<uint32_t>(trees.size()), nt, nl, root, rank);
+  KC_HIP(hipGetLastError());
+}
+
+}  // namespace kc

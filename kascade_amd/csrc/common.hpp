@@ -159,6 +159,8 @@ struct RankTuning {
   uint32_t spacing_deep;  // elements per ruler below
   uint32_t max_levels;
   uint32_t small_n;       // sizes at or below this go to the one-workgroup solver
+  uint32_t chase_blocks_per_cu;
+  uint32_t chase_chunk;  // rulers a wave takes per work-counter atomic
 };
 RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg);
 

@@ -1,0 +1,108 @@
+// Random-gather ceiling probe: dst[i] = src[idx[i]], idx a seeded permutation.
+#include "common.hpp"
+#include "device.hpp"
+
+namespace kc {
+namespace {
+constexpr unsigned kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads)
+perm_idx_kernel(uint64_t* __restrict__ idx, uint64_t n, uint64_t seed) {
+  // multiplicative permutation of [0, n) for n a power of two: odd multiplier + xor-shift free
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mask = n - 1;
+  uint64_t x = (i * (splitmix64(seed) | 1)) & mask;
+  x = (x ^ (x >> 7)) & mask;               // invertible on [0, 2^k)
+  x = (x * 0x9E3779B97F4A7C15ull + (seed & mask)) & mask;
+  idx[i] = x;
+}
+
+__global__ void __launch_bounds__(kThreads)
+fill_kernel(uint64_t* __restrict__ a, uint64_t n) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) a[i] = i * 3 + 1;
+}
+
+__global__ void __launch_bounds__(kThreads)
+gather_kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx,
+              uint64_t* __restrict__ dst, uint64_t n) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ void __launch_bounds__(kThreads)
+gather_nt_kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ idx,
+                 uint64_t* __restrict__ dst, uint64_t n) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) dst[i] = __builtin_nontemporal_load(&src[idx[i]]);
+}
+}  // namespace
+}  // namespace kc
+
+// mode 0: plain loads from ordinary device memory; 1: non-temporal loads;
+// 2: plain loads from an uncached allocation (kept until the process ends);
+// 3: plain loads from a fine-grained allocation (kept until the process ends).
+extern "C" kc_status kc_probe_random_gather_mode(void* vctx, uint64_t n, uint32_t reps, int mode,
+                                                 double* gbps_out, double* ms_out) {
+  auto* ctx = static_cast<kc::Context*>(vctx);
+  if (!ctx || n == 0 || (n & (n - 1)) != 0 || reps == 0 || mode < 0 || mode > 3) return KC_EINVAL;
+  try {
+    KC_HIP(hipSetDevice(ctx->device));
+    kc::Scratch scratch(ctx);
+    uint64_t* src = nullptr;
+    if (mode == 2 || mode == 3) {
+      // special allocations are never freed (an uncached buffer that a kernel has read must
+      // stay allocated for the life of the process)
+      static uint64_t* keep[4] = {nullptr, nullptr, nullptr, nullptr};
+      static uint64_t keep_n[4] = {0, 0, 0, 0};
+      if (keep_n[mode] < n) {
+        void* p = nullptr;
+        KC_HIP(hipExtMallocWithFlags(&p, n * sizeof(uint64_t),
+                                     mode == 2 ? hipDeviceMallocUncached : hipDeviceMallocFinegrained));
+        keep[mode] = static_cast<uint64_t*>(p);
+        keep_n[mode] = n;
+      }
+      src = keep[mode];
+    } else {
+      src = scratch.get<uint64_t>(n);
+    }
+    uint64_t* idx = scratch.get<uint64_t>(n);
+    uint64_t* dst = scratch.get<uint64_t>(n);
+    hipStream_t st = ctx->stream;
+    const unsigned blocks = kc::blocks_for(n, 256);
+    hipLaunchKernelGGL(kc::perm_idx_kernel, dim3(blocks), dim3(256), 0, st, idx, n, 12345ull);
+    hipLaunchKernelGGL(kc::fill_kernel, dim3(blocks), dim3(256), 0, st, src, n);
+    hipLaunchKernelGGL(kc::gather_kernel, dim3(blocks), dim3(256), 0, st, src, idx, dst, n);
+    hipEvent_t a, b;
+    KC_HIP(hipEventCreate(&a));
+    KC_HIP(hipEventCreate(&b));
+    KC_HIP(hipEventRecord(a, st));
+    for (uint32_t r = 0; r < reps; r++) {
+      if (mode == 1)
+        hipLaunchKernelGGL(kc::gather_nt_kernel, dim3(blocks), dim3(256), 0, st, src, idx, dst, n);
+      else
+        hipLaunchKernelGGL(kc::gather_kernel, dim3(blocks), dim3(256), 0, st, src, idx, dst, n);
+    }
+    KC_HIP(hipEventRecord(b, st));
+    KC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    KC_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double per = ms / reps;
+    if (ms_out) *ms_out = per;
+    if (gbps_out) *gbps_out = 24.0 * static_cast<double>(n) / (per * 1e-3) / 1e9;
+    return KC_OK;
+  } catch (const kc::KcError& e) {
+    ctx->last_error = e.what;
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return KC_ENOMEM;
+  }
+}
+
+extern "C" kc_status kc_probe_random_gather(void* vctx, uint64_t n, uint32_t reps, double* gbps_out,
+                                            double* ms_out) {
+  return kc_probe_random_gather_mode(vctx, n, reps, 0, gbps_out, ms_out);
+}

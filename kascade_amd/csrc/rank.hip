@@ -75,71 +75,89 @@ __global__ void __launch_bounds__(kThreads)
 chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
-             uint32_t refill_min) {
-  const int lane = static_cast<int>(__lane_id());
+             uint32_t chunk) {
   const uint64_t lt = lanemask_lt();
+  // Wave-uniform chunk of ruler ids [cbase, cend): one atomic per `chunk`
+  // rulers, so idle lanes refill every step without touching the counter.
+  uint64_t cbase = 0, cend = 0;
+  bool wave_done = false;
   bool active = false;
-  bool exhausted = false;
+  bool first = false;
   uint64_t ruler = 0, cur = 0;
   int64_t dist = 0;
   for (;;) {
-    const uint64_t need = __ballot(!active && !exhausted);
-    const uint64_t act = __ballot(active);
-    if (need != 0 && (static_cast<uint32_t>(__popcll(need)) >= refill_min || act == 0)) {
-      const int leader = __ffsll(static_cast<long long>(need)) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(work_ctr, static_cast<unsigned long long>(__popcll(need)));
-      base = __shfl(base, leader);
-      if (!active && !exhausted) {
-        const uint64_t item = base + __popcll(need & lt);
-        if (item < nR) {
-          const uint64_t v0 = ruler_of_block(item, n, logK, seed);
-          const Rec r = rec[v0];
-          sub_orig[item] = v0;
-          rec[v0] = Rec{item | kRootFlag, 0};
-          if (r.s == v0) {
-            sub[item] = Rec{item, 0};  // the ruler is a tail
-          } else {
-            ruler = item;
-            cur = r.s;
-            dist = r.w;
-            active = true;
-          }
+    uint64_t need = __ballot(!active);
+    if (need != 0 && !wave_done) {
+      if (cbase == cend) {
+        unsigned long long b = 0;
+        if (__lane_id() == 0) b = atomicAdd(work_ctr, static_cast<unsigned long long>(chunk));
+        b = __shfl(b, 0);
+        if (b >= nR) {
+          wave_done = true;
         } else {
-          exhausted = true;
+          cbase = b;
+          cend = b + chunk < nR ? b + chunk : nR;
         }
+      }
+      if (!wave_done) {
+        const uint64_t avail = cend - cbase;
+        const uint64_t r = __popcll(need & lt);
+        if (!active && r < avail) {
+          ruler = cbase + r;
+          cur = ruler_of_block(ruler, n, logK, seed);
+          first = true;
+          active = true;
+        }
+        const uint64_t cnt = __popcll(need);
+        cbase += cnt < avail ? cnt : avail;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (wave_done) break;
+      continue;
+    }
+    if (!active) continue;
+    if (first) {
+      // the ruler's own record: claim it with distance 0
+      const Rec rr = rec[cur];
+      sub_orig[ruler] = cur;
+      rec[cur] = Rec{ruler | kRootFlag, 0};
+      first = false;
+      if (rr.s == cur) {
+        sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
+        active = false;
+      } else {
+        dist = rr.w;
+        cur = rr.s;
       }
       continue;
     }
-    if (act == 0) break;  // need == 0 here: every lane is exhausted
-    if (active) {
-      if (is_ruler(cur, n, logK, seed)) {
-        const uint64_t id = cur >> logK;
-        if (id == ruler) raise_err(err, kErrCycle);  // came back to its own ruler
-        sub[ruler] = Rec{id, dist};
-        active = false;
+    if (is_ruler(cur, n, logK, seed)) {
+      const uint64_t id = cur >> logK;
+      if (id == ruler) raise_err(err, kErrCycle);  // came back to its own ruler
+      sub[ruler] = Rec{id, dist};
+      active = false;
+      continue;
+    }
+    const Rec r = rec[cur];
+    if (r.s & (kRootFlag | kAuxFlag)) {
+      raise_err(err, kErrNotList);  // reached twice: in-degree > 1
+      active = false;
+    } else if (r.s == cur) {
+      const uint64_t j = nR + atomicAdd(extra_ctr, 1ull);
+      if (j < cap) {
+        sub[j] = Rec{j, 0};
+        sub_orig[j] = cur;
+        rec[cur] = Rec{j | kRootFlag, 0};
+        sub[ruler] = Rec{j, dist};
       } else {
-        const Rec r = rec[cur];
-        if (r.s & (kRootFlag | kAuxFlag)) {
-          raise_err(err, kErrNotList);  // reached twice: in-degree > 1
-          active = false;
-        } else if (r.s == cur) {
-          const uint64_t j = nR + atomicAdd(extra_ctr, 1ull);
-          if (j < cap) {
-            sub[j] = Rec{j, 0};
-            sub_orig[j] = cur;
-            rec[cur] = Rec{j | kRootFlag, 0};
-            sub[ruler] = Rec{j, dist};
-          } else {
-            raise_err(err, kErrNotList);
-          }
-          active = false;
-        } else {
-          rec[cur] = Rec{ruler | kRootFlag, dist};
-          dist += r.w;
-          cur = r.s;
-        }
+        raise_err(err, kErrNotList);
       }
+      active = false;
+    } else {
+      rec[cur] = Rec{ruler | kRootFlag, dist};
+      dist += r.w;
+      cur = r.s;
     }
   }
 }
@@ -319,6 +337,8 @@ pd_flag_roots_kernel(Rec* __restrict__ a, uint64_t n, uint32_t* err) {
   }
 }
 
+// `active` is a flag (any element still unfinished), written by one lane per
+// wave that has one; no counting, so no same-address atomics.
 __global__ void __launch_bounds__(kThreads)
 pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
                 unsigned long long* active) {
@@ -334,7 +354,11 @@ pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
       still = !(t.s & kRootFlag);
     }
   }
-  wave_append(active, still);
+  const uint64_t m = __ballot(still);
+  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) {
+    if (__hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __hip_atomic_store(active, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -420,13 +444,14 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   zero_counters(ctx, st);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
   const uint64_t lanes_wanted = nR;
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * 8;
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
   const unsigned chase_blocks = static_cast<unsigned>(
       std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
   {
     Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(chase_kernel, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed,
-                       nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err, 8u);
+                       nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
+                       tune.chase_chunk);
   }
   check_err(ctx, st);
   uint64_t extra = 0;
@@ -491,7 +516,11 @@ RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg) {
   t.spacing_deep = 16;
   t.max_levels = 8;
   t.small_n = kSmallMax;
+  t.chase_blocks_per_cu = 4;
+  t.chase_chunk = 256;
   if (cfg) {
+    if (cfg->chase_blocks_per_cu) t.chase_blocks_per_cu = std::min(cfg->chase_blocks_per_cu, 64u);
+    if (cfg->chase_chunk) t.chase_chunk = std::min(cfg->chase_chunk, 1u << 16);
     auto pow2 = [](uint32_t x) { return x >= 2 && (x & (x - 1)) == 0; };
     if (cfg->ruler_spacing) {
       if (!pow2(cfg->ruler_spacing) || cfg->ruler_spacing > (1u << 20))

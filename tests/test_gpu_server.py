@@ -1,0 +1,117 @@
+"""Continuous-batching server: every job's result is identical alone and in the
+mix (bit-exact), in both coalesced and one-job-per-launch modes, with device
+or host arrays; budget admission and per-job error isolation."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def make_jobs(count, seed, lo=6, hi=16):
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(count):
+        n = int(2 ** rng.uniform(lo, hi))
+        if rng.random() < 0.6:
+            succ, w = o.gen_list(n, 1000 + k, True)
+            jobs.append(("list", succ, w))
+        else:
+            jobs.append(("tree", o.gen_tree(n, 2000 + k), None))
+    return jobs
+
+
+def alone(ctx, kind, succ, w):
+    if kind == "list":
+        r, k = ctx.rank_list(dev(succ), dev(w))
+    else:
+        r, k = ctx.root_tree(dev(succ))
+    return host(r), host(k)
+
+
+@pytest.mark.parametrize("mode", ["batched", "one_job_per_launch"])
+def test_server_mix_matches_alone(ctx, mode):
+    import torch
+    import kascade_amd
+    say("server", mode)
+    jobs = make_jobs(120, 5)
+    sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one_job_per_launch"), num_slots=3)
+    tickets = []
+    for kind, succ, w in jobs:
+        s = dev(succ)
+        ww = dev(w) if w is not None else None
+        ro = torch.empty(len(succ), dtype=torch.uint64, device="cuda")
+        ra = torch.empty(len(succ), dtype=torch.int64, device="cuda")
+        t, _, _ = sv.submit(kind, s, ww, ro, ra)
+        tickets.append((t, s, ww, ro, ra))
+    for t, *_ in tickets:
+        sv.wait(t)
+    st = sv.stats()
+    assert st["jobs_completed"] == len(jobs)
+    if mode == "batched":
+        assert st["batches_launched"] < len(jobs)
+    for (kind, succ, w), (t, s, ww, ro, ra) in zip(jobs, tickets):
+        er, ek = alone(ctx, kind, succ, w)
+        assert (host(ro) == er).all() and (host(ra) == ek).all()
+        if kind == "list":
+            cr, ck = o.sequential_walk(succ, w)
+        else:
+            cr, ck = o.euler_tour_rooting(succ)
+        assert (er == cr).all() and (ek == ck).all()
+    sv.close()
+
+
+def test_server_host_arrays_and_budget(ctx):
+    import kascade_amd
+    say("server host/budget")
+    jobs = make_jobs(40, 8, 4, 13)
+    # a small budget forces admission to wait for memory to free up
+    sv = kascade_amd.Server(ctx, mem_budget_bytes=8 << 20, num_slots=2)
+    outs = []
+    for kind, succ, w in jobs:
+        t, ro, ra = sv.submit(kind, succ.copy(), w.copy() if w is not None else None)
+        outs.append((t, ro, ra))
+    for (kind, succ, w), (t, ro, ra) in zip(jobs, outs):
+        sv.wait(t)
+        cr, ck = (o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ))
+        assert (ro == cr).all() and (ra == ck).all()
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        sv.submit("list", np.zeros(1 << 20, np.uint64))
+    assert e.value.status == kascade_amd.KC_EBUDGET
+    sv.close()
+
+
+def test_server_error_isolation(ctx):
+    import kascade_amd
+    say("server errors")
+    sv = kascade_amd.Server(ctx, num_slots=1, max_batch_jobs=64)
+    good = make_jobs(10, 3, 4, 10)
+    subs = []
+    for i, (kind, succ, w) in enumerate(good):
+        subs.append(sv.submit(kind, succ.copy(), w.copy() if w is not None else None))
+        if i == 4:
+            bad = sv.submit("list", np.array([1, 9, 2], np.uint64))
+    for (kind, succ, w), (t, ro, ra) in zip(good, subs):
+        sv.wait(t)
+        cr, ck = (o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ))
+        assert (ro == cr).all() and (ra == ck).all()
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        sv.wait(bad[0])
+    assert e.value.status == kascade_amd.KC_EINVAL
+    sv.close()

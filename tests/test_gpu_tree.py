@@ -1,0 +1,121 @@
+"""GPU parity of Euler-tour tree rooting through the C ABI: (root, depth) per
+vertex bit-exact against the generator's closed form, BFS and the oracle's
+restatement of rank_via_euler_tour."""
+import numpy as np
+import pytest
+
+import oracle_lib as o
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def relabel_forest(seed):
+    rng = np.random.default_rng(seed)
+    n = 5000
+    parent = np.arange(n, dtype=np.uint64)
+    for v in range(1, 1500):
+        parent[v] = rng.integers(0, v)            # random recursive tree at 0
+    for v in range(1501, 2500):
+        parent[v] = 1500                          # star at 1500
+    for v in range(2501, 4000):
+        parent[v] = v - 1                         # path at 2500 (depth 1499)
+    for v in range(4001, 4500):
+        parent[v] = 4000 + (v - 4001) // 2        # binary-ish tree at 4000
+    # 4500..4999 isolated roots
+    sigma = rng.permutation(n)
+    out = np.empty(n, np.uint64)
+    out[sigma] = sigma[parent]
+    return out
+
+
+@pytest.mark.parametrize("algo", ["srs", "pd"])
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 2049, 4097, 100003, 1 << 20])
+def test_root_random_tree(ctx, algo, n):
+    say("tree", algo, n)
+    parent = o.gen_tree(n, 31 + n)
+    root, depth = ctx.root_tree(dev(parent), algorithm=algo)
+    er, ed = o.tree_closed_form(n, 31 + n)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+@pytest.mark.parametrize("algo", ["srs", "pd"])
+def test_root_forest_shapes(ctx, algo):
+    say("forest shapes", algo)
+    parent = relabel_forest(3)
+    root, depth = ctx.root_tree(dev(parent), algorithm=algo)
+    er, ed = o.euler_tour_rooting(parent)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+def test_in_place_and_host(ctx):
+    say("tree in place")
+    n = 54321
+    parent = o.gen_tree(n, 5)
+    er, ed = o.tree_closed_form(n, 5)
+    s = dev(parent)
+    r = dev(np.ones(n, np.int64))
+    ctx.rank_via_euler_tour(s, r)
+    assert (host(s) == er).all() and (host(r) == ed).all()
+    hs, hr = parent.copy(), np.zeros(n, np.int64)
+    ctx.rank_via_euler_tour(hs, hr)
+    assert (hs == er).all() and (hr == ed).all()
+
+
+def test_tour_structure(ctx):
+    """compute_euler_tour: one path per tree ending at the root's dummy, 2(n-1)
+    edges, weights summing to 0 over each tour, unused slots isolated."""
+    say("tour structure")
+    parent = relabel_forest(9)
+    n = len(parent)
+    ts, tw = ctx.compute_euler_tour(dev(parent))
+    ts, tw = host(ts), host(tw)
+    roots = np.nonzero(parent == np.arange(n))[0]
+    # ranking the tour with the oracle gives the same depths as the tree
+    r, k = o.sequential_walk(ts, tw)
+    er, ed = o.euler_tour_rooting(parent)
+    assert (r[:n] == er).all() and (k[:n] == ed).all()
+    # tour heads: the first down edge of each root with children has rank 0
+    indeg = np.bincount(ts[ts != np.arange(2 * n)].astype(np.int64), minlength=2 * n)
+    assert indeg.max() <= 1                                  # a list
+    for rt in roots:
+        assert ts[rt] == rt and tw[rt] == 0                  # dummy self loop
+        assert ts[n + rt] == n + rt                          # unused slot
+    heads = np.nonzero((indeg == 0) & (ts != np.arange(2 * n)))[0]
+    assert (k[heads] == 0).all()                             # sum of weights per tour is 0
+    assert np.count_nonzero(tw == 1) == n - len(roots)
+    assert np.count_nonzero(tw == -1) == n - len(roots)
+
+
+def test_tree_errors(ctx):
+    import kascade_amd
+    say("tree errors")
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        ctx.root_tree(dev(np.array([0, 7, 0], np.uint64)))
+    assert e.value.status == kascade_amd.KC_EINVAL
+    for n in [2, 3, 9000]:                                   # parent cycle, no root
+        cyc = np.roll(np.arange(n, dtype=np.uint64), 1)
+        with pytest.raises(kascade_amd.KascadeError) as e:
+            ctx.root_tree(dev(cyc))
+        assert e.value.status == kascade_amd.KC_ECYCLE, n
+
+
+@pytest.mark.parametrize("n", [1 << 22])
+def test_root_large_device_generated(ctx, n):
+    say("tree large", n)
+    parent = ctx.gen_tree(n, 1234)
+    root, depth = ctx.root_tree(parent)
+    er, ed = o.tree_closed_form(n, 1234)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
