@@ -30,7 +30,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=1 << 27)
+    ap.add_argument("--workload", choices=["list", "tree", "segmented"], default="list",
+                    help="list: one random weighted list (the headline); tree: random rooted tree "
+                         "(Euler tour); segmented: 20,000 lists of log-uniform size in one batch")
+    ap.add_argument("--n", type=int, default=0, help="elements (list: 2^27, tree: 2^26 by default)")
+    ap.add_argument("--segments", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=2027)
     ap.add_argument("--algorithm", default="srs")
     ap.add_argument("--spacing", type=int, default=0)
@@ -130,6 +134,87 @@ def serve_bench(ctx, jobs, seed, slots):
     }
 
 
+def timed(ctx, fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    phases = ctx.profile_read()
+    ctx.profile(False)
+    return dt / steps, phases
+
+
+def tree_main(args, ctx):
+    """Random rooted tree (C3): parent of generation index i uniform in [0, i),
+    relabelled by a seeded permutation; Euler-tour rooting."""
+    import torch
+    n = args.n or (1 << 26)
+    parent = ctx.gen_tree(n, args.seed)
+    root = torch.empty(n, dtype=torch.uint64, device="cuda")
+    depth = torch.empty(n, dtype=torch.int64, device="cuda")
+    t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth), args.steps, args.warmup)
+    out = {"metric": "vertices rooted/s", "value": n / t, "unit": "vertices/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+           "data": "synthetic: random recursive tree, seeded relabelling",
+           "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting (C3)", "n": n, "seed": args.seed},
+           "effective_gbps": 24 * n / t / 1e9,
+           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
+    if args.check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as o
+        er, ed = o.tree_closed_form(n, args.seed)
+        out["bit_exact_vs_closed_form"] = bool((root.cpu().numpy() == er).all() and (depth.cpu().numpy() == ed).all())
+    print(json.dumps(out), flush=True)
+
+
+def segmented_main(args, ctx):
+    """20,000 lists, sizes log-uniform in [2^8, 2^20], 30 % locality-biased,
+    u32 local ids, unit weights, ranked in one segmented call (C4)."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(args.seed)
+    sizes = np.floor(2.0 ** rng.uniform(8, 20, size=args.segments)).astype(np.uint64)
+    off = np.zeros(args.segments + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    seeds = rng.integers(1, 1 << 62, size=args.segments, dtype=np.uint64)
+    local = (rng.random(args.segments) < 0.3).astype(np.uint8)
+    total = int(off[-1])
+    d_off = torch.from_numpy(off).cuda()
+    succ, _ = ctx.gen_lists_segmented(d_off, torch.from_numpy(seeds).cuda(), torch.from_numpy(local).cuda(),
+                                      weighted=False)
+    root = torch.empty(total, dtype=torch.uint32, device="cuda")
+    rank = torch.empty(total, dtype=torch.int64, device="cuda")
+    t, phases = timed(ctx, lambda: ctx.rank_lists_segmented(succ, d_off, None, root, rank), args.steps,
+                      args.warmup)
+    out = {"metric": "elements ranked/s (segmented batch)", "value": total / t, "unit": "elements/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+           "data": "synthetic: seeded random permutations, 30% locality-biased (blocks of 10)",
+           "config": {"workload": f"{args.segments} lists log-uniform in [2^8, 2^20] (C4)",
+                      "segments": args.segments, "total_elements": total, "seed": args.seed},
+           "effective_gbps": (16 * total + 8 * args.segments) / t / 1e9,
+           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
+    if args.check:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as o
+        ok = True
+        for s in rng.choice(args.segments, size=min(40, args.segments), replace=False):
+            a, b = int(off[s]), int(off[s + 1])
+            es, ew = o.gen_list(b - a, int(seeds[s]), False, local=bool(local[s]))
+            er, ek = o.sequential_walk(es, ew)
+            ok &= bool((root[a:b].cpu().numpy() == er).all() and (rank[a:b].cpu().numpy() == ek).all())
+        out["bit_exact_sampled_segments"] = ok
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -138,8 +223,12 @@ def main():
     import torch
     import kascade_amd
 
-    n = args.n
     ctx = kascade_amd.Context(0)
+    if args.workload == "tree":
+        return tree_main(args, ctx)
+    if args.workload == "segmented":
+        return segmented_main(args, ctx)
+    n = args.n or (1 << 27)
     succ, w = ctx.gen_list(n, args.seed, weighted=True)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")

@@ -113,10 +113,7 @@ Profiler::~Profiler() {
   for (auto& o : open_) (void)hipEventDestroy(o.second);
 }
 
-void check_err(Context* ctx, hipStream_t st) {
-  uint32_t e = 0;
-  KC_HIP(hipMemcpyAsync(&e, ctx->d_err, sizeof(e), hipMemcpyDeviceToHost, st));
-  KC_HIP(hipStreamSynchronize(st));
+static void raise_from_bits(Context* ctx, hipStream_t st, uint32_t e) {
   if (e == 0) return;
   KC_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), st));
   KC_HIP(hipStreamSynchronize(st));
@@ -127,11 +124,21 @@ void check_err(Context* ctx, hipStream_t st) {
   throw KcError{KC_EDEVICE, "unknown device error"};
 }
 
-void read_counters(Context* ctx, hipStream_t st, int first, int count, uint64_t* out) {
-  KC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_counters + first, count * sizeof(uint64_t),
+// The error word lives in the last counter slot, so one small copy and one
+// stream synchronisation return both.
+void check_err(Context* ctx, hipStream_t st) {
+  KC_HIP(hipMemcpyAsync(ctx->h_pinned + kErrSlot, ctx->d_counters + kErrSlot, sizeof(uint64_t),
                         hipMemcpyDeviceToHost, st));
   KC_HIP(hipStreamSynchronize(st));
-  std::memcpy(out, ctx->h_pinned, count * sizeof(uint64_t));
+  raise_from_bits(ctx, st, static_cast<uint32_t>(ctx->h_pinned[kErrSlot]));
+}
+
+void read_counters(Context* ctx, hipStream_t st, int first, int count, uint64_t* out) {
+  KC_HIP(hipMemcpyAsync(ctx->h_pinned + first, ctx->d_counters + first,
+                        (kErrSlot + 1 - first) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  KC_HIP(hipStreamSynchronize(st));
+  std::memcpy(out, ctx->h_pinned + first, count * sizeof(uint64_t));
+  raise_from_bits(ctx, st, static_cast<uint32_t>(ctx->h_pinned[kErrSlot]));
 }
 
 // Host arrays are staged through device buffers; device arrays pass through.
@@ -231,12 +238,13 @@ kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void** ctx_ou
   ctx->device = device;
   ctx->budget = mem_budget_bytes;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&ctx->d_err, 256) != hipSuccess || hipMalloc(&ctx->d_counters, 64 * sizeof(uint64_t)) != hipSuccess ||
-      hipHostMalloc(&ctx->h_pinned, 64 * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
-      hipMemset(ctx->d_err, 0, 256) != hipSuccess || hipMemset(ctx->d_counters, 0, 64 * sizeof(uint64_t)) != hipSuccess) {
+      hipMalloc(&ctx->d_counters, kc::kCounterSlots * sizeof(uint64_t)) != hipSuccess ||
+      hipHostMalloc(&ctx->h_pinned, kc::kCounterSlots * sizeof(uint64_t), hipHostMallocDefault) != hipSuccess ||
+      hipMemset(ctx->d_counters, 0, kc::kCounterSlots * sizeof(uint64_t)) != hipSuccess) {
     kc_context_destroy(ctx);
     return KC_EDEVICE;
   }
+  ctx->d_err = reinterpret_cast<uint32_t*>(ctx->d_counters + kc::kErrSlot);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   *ctx_out = ctx;
@@ -250,7 +258,6 @@ void kc_context_destroy(void* vctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->arena.release(0);
   ctx->arena.trim();
-  if (ctx->d_err) (void)hipFree(ctx->d_err);
   if (ctx->d_counters) (void)hipFree(ctx->d_counters);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -186,7 +186,6 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   }
   uint64_t counts[2] = {0, 0};
   read_counters(ctx, st, 0, 2, counts);
-  check_err(ctx, st);
   RankTuning tune = tuning_for(std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;

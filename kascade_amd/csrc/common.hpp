@@ -86,14 +86,19 @@ class Profiler {
   std::vector<std::pair<std::string, hipEvent_t>> open_;
 };
 
+// Device counter block: slots [0, kErrSlot) are scratch counters of the
+// current operation, slot kErrSlot holds the error word.
+constexpr int kCounterSlots = 64;
+constexpr int kErrSlot = kCounterSlots - 1;
+
 struct Context {
   Profiler prof;
   int device = 0;
   uint64_t budget = 0;
   hipStream_t stream = nullptr;
   Arena arena;
-  uint32_t* d_err = nullptr;       // error word
-  uint64_t* d_counters = nullptr;  // small device counters (64 slots)
+  uint32_t* d_err = nullptr;       // error word (inside the counter block)
+  uint64_t* d_counters = nullptr;  // kCounterSlots device counters
   uint64_t* h_pinned = nullptr;    // pinned host mirror of counters
   std::string last_error;
   int num_cus = 256;

@@ -89,7 +89,6 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
                      reinterpret_cast<unsigned long long*>(ctx->d_counters));
   uint64_t roots = 0;
   read_counters(ctx, st, 0, 1, &roots);
-  check_err(ctx, st);
   radix_sort_pairs_u32(ctx, keys, vals, n, bits_for(n), st);
   hipLaunchKernelGGL(tour_link_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, keys,
                      vals, parent, n, tour);

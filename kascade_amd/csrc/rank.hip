@@ -177,7 +177,7 @@ map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uin
 __global__ void __launch_bounds__(kThreads)
 propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride,
-                 uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig,
+                 uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
                  unsigned long long* u_ctr) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   Rec r{0, 0};
@@ -195,7 +195,28 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
     }
   }
   const uint64_t k = wave_append(u_ctr, unclaimed);
-  if (unclaimed) {
+  if (unclaimed && k < ucap) {  // beyond the capacity: collected by a second pass
+    u_ids[k] = v;
+    u_orig[k] = r;
+    rec[v].s = k | kAuxFlag;
+  }
+}
+
+// Second collection pass after the unclaimed list outgrew its first capacity:
+// appends the still unmarked unclaimed elements after slot `first`.
+__global__ void __launch_bounds__(kThreads)
+collect_unclaimed_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t* __restrict__ u_ids,
+                         Rec* __restrict__ u_orig, uint64_t first, uint64_t cap,
+                         unsigned long long* ctr) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  Rec r{0, 0};
+  bool mine = false;
+  if (v < n) {
+    r = rec[v];
+    mine = !(r.s & (kRootFlag | kAuxFlag));
+  }
+  const uint64_t k = first + wave_append(ctr, mine);
+  if (mine && k < cap) {
     u_ids[k] = v;
     u_orig[k] = r;
     rec[v].s = k | kAuxFlag;
@@ -453,9 +474,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                        nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
                        tune.chase_chunk);
   }
-  check_err(ctx, st);
   uint64_t extra = 0;
-  read_counters(ctx, st, kSlotExtra, 1, &extra);
+  read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
   const uint64_t m = nR + extra;
 
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
@@ -466,16 +486,29 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
 
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
-  uint64_t* u_ids = scratch.get<uint64_t>(n);
-  Rec* u_orig = scratch.get<Rec>(n);
+  // unclaimed elements are list prefixes: few unless the lists are short
+  const uint64_t ucap = std::min<uint64_t>(n, n / 8 + 65536);
+  uint64_t* u_ids = scratch.get<uint64_t>(ucap);
+  Rec* u_orig = scratch.get<Rec>(ucap);
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, T, root, rank, stride, u_ids, u_orig, ctr + kSlotUnclaimed);
+                       n, n_out, T, root, rank, stride, u_ids, u_orig, ucap, ctr + kSlotUnclaimed);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
   if (nu == 0) return;
+  if (nu > ucap) {
+    uint64_t* ids2 = scratch.get<uint64_t>(nu);
+    Rec* orig2 = scratch.get<Rec>(nu);
+    KC_HIP(hipMemcpyAsync(ids2, u_ids, ucap * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+    KC_HIP(hipMemcpyAsync(orig2, u_orig, ucap * sizeof(Rec), hipMemcpyDeviceToDevice, st));
+    zero_counters(ctx, st);
+    hipLaunchKernelGGL(collect_unclaimed_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
+                       rec, n, ids2, orig2, ucap, nu, ctr + kSlotUnclaimed);
+    u_ids = ids2;
+    u_orig = orig2;
+  }
   Phase ph(ctx, "unclaimed", st);
   Rec* urec = scratch.get<Rec>(nu);
   Rec* ext = scratch.get<Rec>(nu);
@@ -567,7 +600,6 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
   pack_records(ctx, succ, w, n, rec, st);
   uint64_t tails = 0;
   read_counters(ctx, st, kSlotTails, 1, &tails);
-  check_err(ctx, st);
   rank_records_impl(ctx, rec, n, n, root, rank, 1, tails, tune, algo, 0, st);
   check_err(ctx, st);
 }

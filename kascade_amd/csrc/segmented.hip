@@ -118,7 +118,6 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
                      reinterpret_cast<unsigned long long*>(ctx->d_counters));
   uint64_t tails = 0;
   read_counters(ctx, st, 0, 1, &tails);
-  check_err(ctx, st);
   rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st);
   check_err(ctx, st);
   hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st,

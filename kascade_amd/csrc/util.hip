@@ -111,61 +111,95 @@ scan_add_kernel(uint64_t* __restrict__ out, uint64_t n, const uint64_t* __restri
 }
 
 // ------------------------------------------------------------ radix sort ---
-constexpr unsigned kRadixBits = 8;
-constexpr unsigned kBins = 1u << kRadixBits;
+// LSD radix sort, up to 8 bits per pass.  Each pass: a per-tile histogram, a
+// scan of the (digit-major) histogram table, then a scatter that first sorts
+// the tile by digit in LDS (stable) and writes each digit's run contiguously,
+// so global writes are coalesced runs instead of scattered words.
+constexpr unsigned kMaxRadixBits = 8;
+constexpr unsigned kMaxBins = 1u << kMaxRadixBits;
 constexpr unsigned kSortItems = 16;
 constexpr unsigned kSortTile = kThreads * kSortItems;
 constexpr unsigned kWaves = kThreads / 64;
 
 template <typename K>
 __global__ void __launch_bounds__(kThreads)
-radix_hist_kernel(const K* __restrict__ keys, uint64_t n, unsigned shift, uint64_t nblocks,
-                  uint64_t* __restrict__ counts) {
-  __shared__ uint32_t s_hist[kBins];
-  for (unsigned d = threadIdx.x; d < kBins; d += kThreads) s_hist[d] = 0;
+radix_hist_kernel(const K* __restrict__ keys, uint64_t n, unsigned shift, unsigned bits,
+                  uint64_t nblocks, uint64_t* __restrict__ counts) {
+  __shared__ uint32_t s_hist[kMaxBins];
+  const unsigned bins = 1u << bits;
+  for (unsigned d = threadIdx.x; d < bins; d += kThreads) s_hist[d] = 0;
   __syncthreads();
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
   for (unsigned r = 0; r < kSortItems; r++) {
     uint64_t i = base + r * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&s_hist[static_cast<unsigned>(keys[i] >> shift) & (kBins - 1)], 1u);
+    if (i < n) atomicAdd(&s_hist[static_cast<unsigned>(keys[i] >> shift) & (bins - 1)], 1u);
   }
   __syncthreads();
-  for (unsigned d = threadIdx.x; d < kBins; d += kThreads)
+  for (unsigned d = threadIdx.x; d < bins; d += kThreads)
     counts[static_cast<uint64_t>(d) * nblocks + blockIdx.x] = s_hist[d];
 }
 
-// Stable scatter: items are ranked in (round, wave, lane) order, which is the
-// input order inside the tile; a lane finds the peers with its digit by eight
-// ballots.
 template <typename K, typename V>
 __global__ void __launch_bounds__(kThreads)
 radix_scatter_kernel(const K* __restrict__ keys, const V* __restrict__ vals, uint64_t n,
-                     unsigned shift, uint64_t nblocks, const uint64_t* __restrict__ offsets,
-                     K* __restrict__ keys_out, V* __restrict__ vals_out) {
-  __shared__ uint64_t s_run[kBins];
-  __shared__ uint32_t s_wcnt[kWaves][kBins];
+                     unsigned shift, unsigned bits, uint64_t nblocks,
+                     const uint64_t* __restrict__ offsets, K* __restrict__ keys_out,
+                     V* __restrict__ vals_out) {
+  __shared__ uint32_t s_run[kMaxBins];      // next local slot of each digit
+  __shared__ uint32_t s_start[kMaxBins];    // local start of each digit's run
+  __shared__ uint32_t s_wcnt[kWaves][kMaxBins];
+  __shared__ K s_keys[kSortTile];
+  __shared__ V s_vals[kSortTile];
+  __shared__ uint32_t s_scan[kThreads / 64];
+  const unsigned bins = 1u << bits;
   const unsigned wave = threadIdx.x / 64;
   const uint64_t lt = lanemask_lt();
-  for (unsigned d = threadIdx.x; d < kBins; d += kThreads) {
-    s_run[d] = offsets[static_cast<uint64_t>(d) * nblocks + blockIdx.x];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  const unsigned count = static_cast<unsigned>(n - base < kSortTile ? n - base : kSortTile);
+
+  // local histogram -> local run starts (exclusive scan over digits)
+  for (unsigned d = threadIdx.x; d < kMaxBins; d += kThreads) {
+    s_run[d] = 0;
     for (unsigned w = 0; w < kWaves; w++) s_wcnt[w][d] = 0;
   }
   __syncthreads();
-  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSortTile;
   for (unsigned r = 0; r < kSortItems; r++) {
-    const uint64_t i = base + r * kThreads + threadIdx.x;
-    const bool valid = i < n;
+    const unsigned li = r * kThreads + threadIdx.x;
+    if (li < count) atomicAdd(&s_run[static_cast<unsigned>(keys[base + li] >> shift) & (bins - 1)], 1u);
+  }
+  __syncthreads();
+  {
+    // kThreads == kMaxBins: thread t owns digit t
+    const unsigned c = s_run[threadIdx.x];
+    unsigned incl = c;
+    const unsigned lane = __lane_id();
+    for (int d = 1; d < 64; d <<= 1) {
+      unsigned y = __shfl_up(incl, d);
+      if (lane >= static_cast<unsigned>(d)) incl += y;
+    }
+    if (lane == 63) s_scan[wave] = incl;
+    __syncthreads();
+    unsigned off = 0;
+    for (unsigned w = 0; w < wave; w++) off += s_scan[w];
+    s_start[threadIdx.x] = off + incl - c;
+    s_run[threadIdx.x] = off + incl - c;
+  }
+  __syncthreads();
+
+  // stable local ranking in (round, wave, lane) order; place into LDS
+  for (unsigned r = 0; r < kSortItems; r++) {
+    const unsigned li = r * kThreads + threadIdx.x;
+    const bool valid = li < count;
     K key = 0;
     V val = 0;
     unsigned d = 0;
     if (valid) {
-      key = keys[i];
-      val = vals[i];
-      d = static_cast<unsigned>(key >> shift) & (kBins - 1);
+      key = keys[base + li];
+      val = vals[base + li];
+      d = static_cast<unsigned>(key >> shift) & (bins - 1);
     }
     uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (unsigned b = 0; b < kRadixBits; b++) {
+    for (unsigned b = 0; b < bits; b++) {
       const uint64_t bb = __ballot(valid && ((d >> b) & 1u));
       peers &= ((d >> b) & 1u) ? bb : ~bb;
     }
@@ -173,13 +207,14 @@ radix_scatter_kernel(const K* __restrict__ keys, const V* __restrict__ vals, uin
     if (valid && rank_in_wave == 0) s_wcnt[wave][d] = __popcll(peers);
     __syncthreads();
     if (valid) {
-      uint64_t off = s_run[d] + rank_in_wave;
-      for (unsigned w = 0; w < wave; w++) off += s_wcnt[w][d];
-      keys_out[off] = key;
-      vals_out[off] = val;
+      unsigned pos = s_run[d] + rank_in_wave;
+      for (unsigned w = 0; w < wave; w++) pos += s_wcnt[w][d];
+      s_keys[pos] = key;
+      s_vals[pos] = val;
     }
     __syncthreads();
-    for (unsigned dd = threadIdx.x; dd < kBins; dd += kThreads) {
+    {
+      const unsigned dd = threadIdx.x;
       uint32_t t = 0;
       for (unsigned w = 0; w < kWaves; w++) {
         t += s_wcnt[w][dd];
@@ -189,31 +224,44 @@ radix_scatter_kernel(const K* __restrict__ keys, const V* __restrict__ vals, uin
     }
     __syncthreads();
   }
+
+  // coalesced write-out: consecutive slots of a digit go to consecutive addresses
+  for (unsigned li = threadIdx.x; li < count; li += kThreads) {
+    const K key = s_keys[li];
+    const unsigned d = static_cast<unsigned>(key >> shift) & (bins - 1);
+    const uint64_t dst = offsets[static_cast<uint64_t>(d) * nblocks + blockIdx.x] + (li - s_start[d]);
+    keys_out[dst] = key;
+    vals_out[dst] = s_vals[li];
+  }
 }
 
 template <typename K, typename V>
 void radix_sort_pairs(Context* ctx, K* keys, V* vals, uint64_t n, unsigned key_bits,
                       hipStream_t st) {
   if (n <= 1) return;
+  Phase ph(ctx, "radix_sort", st);
   Scratch scratch(ctx);
   K* k2 = scratch.get<K>(n);
   V* v2 = scratch.get<V>(n);
   const uint64_t nblocks = (n + kSortTile - 1) / kSortTile;
-  uint64_t* counts = scratch.get<uint64_t>(nblocks * kBins);
-  uint64_t* offs = scratch.get<uint64_t>(nblocks * kBins);
+  uint64_t* counts = scratch.get<uint64_t>(nblocks * kMaxBins);
+  uint64_t* offs = scratch.get<uint64_t>(nblocks * kMaxBins);
   K* src_k = keys;
   V* src_v = vals;
   K* dst_k = k2;
   V* dst_v = v2;
-  unsigned passes = (key_bits + kRadixBits - 1) / kRadixBits;
-  if (passes == 0) passes = 1;
+  if (key_bits == 0) key_bits = 1;
+  const unsigned passes = (key_bits + kMaxRadixBits - 1) / kMaxRadixBits;
+  const unsigned bits = (key_bits + passes - 1) / passes;  // even split, <= 8
   for (unsigned p = 0; p < passes; p++) {
-    const unsigned shift = p * kRadixBits;
+    const unsigned shift = p * bits;
+    const unsigned pbits = std::min(bits, key_bits - shift);
+    const uint64_t nbins = uint64_t{1} << pbits;
     hipLaunchKernelGGL(HIP_KERNEL_NAME(radix_hist_kernel<K>), dim3(static_cast<unsigned>(nblocks)),
-                       dim3(kThreads), 0, st, src_k, n, shift, nblocks, counts);
-    exclusive_scan_u64(ctx, counts, offs, nblocks * kBins, st);
+                       dim3(kThreads), 0, st, src_k, n, shift, pbits, nblocks, counts);
+    exclusive_scan_u64(ctx, counts, offs, nblocks * nbins, st);
     hipLaunchKernelGGL(HIP_KERNEL_NAME(radix_scatter_kernel<K, V>), dim3(static_cast<unsigned>(nblocks)),
-                       dim3(kThreads), 0, st, src_k, src_v, n, shift, nblocks, offs, dst_k, dst_v);
+                       dim3(kThreads), 0, st, src_k, src_v, n, shift, pbits, nblocks, offs, dst_k, dst_v);
     std::swap(src_k, dst_k);
     std::swap(src_v, dst_v);
   }

@@ -217,3 +217,15 @@ def test_rank_large_device_generated(ctx, n):
     root, rank = ctx.rank_list(s, w, algorithm="srs")
     er, ek = o.list_closed_form(n, 42, host(w))
     assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
+def test_cpp_dropin_header():
+    """The C++ drop-in layer (include/kascade_amd.hpp): pack KAT, identity
+    sanity check, AbstractAlgorithm adapters against GatherRank, errors."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "dropin_test")
+    if not os.path.exists(exe):
+        subprocess.run(["sh", os.path.join(os.path.dirname(__file__), "cpp", "build.sh")], check=True)
+    say("cpp dropin")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
