@@ -35,6 +35,7 @@ def parse():
                          "(Euler tour); segmented: 20,000 lists of log-uniform size in one batch")
     ap.add_argument("--n", type=int, default=0, help="elements (list: 2^27, tree: 2^26 by default)")
     ap.add_argument("--segments", type=int, default=20000)
+    ap.add_argument("--tour-build", choices=["auto", "linked", "sorted"], default="auto")
     ap.add_argument("--seed", type=int, default=2027)
     ap.add_argument("--algorithm", default="srs")
     ap.add_argument("--spacing", type=int, default=0)
@@ -159,12 +160,14 @@ def tree_main(args, ctx):
     parent = ctx.gen_tree(n, args.seed)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     depth = torch.empty(n, dtype=torch.int64, device="cuda")
-    t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth), args.steps, args.warmup)
+    t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth, tour_build=args.tour_build), args.steps,
+                      args.warmup)
     out = {"metric": "vertices rooted/s", "value": n / t, "unit": "vertices/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int64",
            "data": "synthetic: random recursive tree, seeded relabelling",
-           "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting (C3)", "n": n, "seed": args.seed},
+           "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting (C3)", "n": n, "seed": args.seed,
+                      "tour_build": args.tour_build},
            "effective_gbps": 24 * n / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
     if args.check:

@@ -84,6 +84,16 @@ kc_status kc_profile_reset(void *ctx);
 uint32_t kc_profile_read(void *ctx, uint32_t max_entries, const char **names, double *ms,
                          uint64_t *counts);
 
+/* Per-size-class tuning (the autotuner's table): rows of 5 uint32
+ * {log2(n) class, ruler spacing top, ruler spacing deep, chase workgroups per
+ * CU, rulers per work-counter atomic}; a size uses the row of the largest
+ * class not above log2(n).  Contexts load tuning_table.txt next to the library
+ * (or $KASCADE_AMD_TUNING) at creation.  kc_tuning_get reports the knobs a
+ * size n would use (out5: spacing top, spacing deep, workgroups per CU,
+ * chunk, one-workgroup threshold) and returns the number of loaded rows. */
+kc_status kc_tuning_set(void *ctx, const uint32_t *rows, uint32_t nrows);
+uint32_t kc_tuning_get(void *ctx, uint64_t n, uint32_t *out5);
+
 /* ---- list ranking ------------------------------------------------------- */
 
 /* Replaces kascade::set_initial_ranking_state (list_ranking.hpp:47-50,
@@ -145,9 +155,17 @@ kc_status kc_rank_lists_segmented(void *ctx, const uint32_t *succ_local, const u
                                   int64_t *rank_out, const kc_rank_cfg *cfg, void *stream);
 
 /* ---- tree rooting ------------------------------------------------------- */
+/* How the children of every vertex are grouped before tour linking. */
+typedef enum kc_tour_build {
+  KC_TOUR_AUTO = 0,   /* = KC_TOUR_LINKED */
+  KC_TOUR_SORTED = 1, /* stable device radix sort of (parent, child): the tour is reproducible */
+  KC_TOUR_LINKED = 2  /* child lists by atomic exchange: one pass, child order may vary per run */
+} kc_tour_build;
+
 typedef struct kc_tree_cfg {
-  kc_rank_cfg rank; /* how the Euler tour is ranked (eulertour.cpp:282-321) */
-  uint32_t reserved[4];
+  kc_rank_cfg rank;   /* how the Euler tour is ranked (eulertour.cpp:282-321) */
+  int32_t tour_build; /* kc_tour_build; (root, depth) are identical either way */
+  uint32_t reserved[3];
 } kc_tree_cfg;
 
 /* parent[v] = parent of v, roots are their own parent.  Outputs root_out[v]
@@ -196,6 +214,10 @@ kc_status kc_probe_random_gather(void *ctx, uint64_t n, uint32_t reps, double *g
  * allocated until the process ends). */
 kc_status kc_probe_random_gather_mode(void *ctx, uint64_t n, uint32_t reps, int mode,
                                       double *gbps_out, double *ms_out);
+/* n random gathers from a table of m uint64 (m <= n): gathers per second as a
+ * function of the table size (which cache level serves a random lookup). */
+kc_status kc_probe_table_gather(void *ctx, uint64_t n, uint64_t m, uint32_t reps,
+                                double *gathers_per_s, double *ms_out);
 
 /* ---- serving ------------------------------------------------------------ */
 typedef enum kc_job_kind { KC_JOB_LIST = 0, KC_JOB_TREE = 1 } kc_job_kind;

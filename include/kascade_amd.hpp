@@ -81,6 +81,7 @@ struct EulerTourConfig {
   Algorithm algorithm = Algorithm::SparseRulingSet;
   bool use_high_degree_handling = false;  // a communication-volume optimisation; ignored
   std::any algo_config;
+  bool reproducible_tour = false;  // group children by radix sort instead of atomic linking
 };
 
 class Error : public std::runtime_error {
@@ -182,6 +183,7 @@ inline void rank_via_euler_tour(EulerTourConfig const& config, std::span<idx_t> 
   if (config.algorithm == Algorithm::EulerTour) throw Error(KC_EINVAL, "Cannot apply Euler Tour recursively.");
   kc_tree_cfg t{};
   t.rank = detail::rank_cfg(detail::algorithm_code(config.algorithm));
+  t.tour_build = config.reproducible_tour ? KC_TOUR_SORTED : KC_TOUR_LINKED;
   ctx.check(kc_rank_via_euler_tour(ctx.handle(), &t, succ_array.data(), rank_array.data(), succ_array.size(),
                                    nullptr));
 }

@@ -36,7 +36,10 @@ class RankCfg(ctypes.Structure):
 
 
 class TreeCfg(ctypes.Structure):
-    _fields_ = [("rank", RankCfg), ("reserved", ctypes.c_uint32 * 4)]
+    _fields_ = [("rank", RankCfg), ("tour_build", ctypes.c_int32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+TOUR_BUILDS = {"auto": 0, "sorted": 1, "linked": 2}
 
 
 class Job(ctypes.Structure):
@@ -76,6 +79,8 @@ _SIGS = {
     "kc_profile_reset": (ctypes.c_int, [_P]),
     "kc_profile_read": (ctypes.c_uint32, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
+    "kc_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]),
+    "kc_tuning_get": (ctypes.c_uint32, [_P, _U64, ctypes.POINTER(ctypes.c_uint32)]),
     "kc_set_initial_ranking_state": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
     "kc_rank_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, ctypes.POINTER(RankCfg), _P]),
     "kc_pointer_doubling": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
@@ -94,6 +99,8 @@ _SIGS = {
                                                ctypes.POINTER(ctypes.c_double)]),
     "kc_probe_random_gather_mode": (ctypes.c_int, [_P, _U64, ctypes.c_uint32, ctypes.c_int,
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "kc_probe_table_gather": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
+                                              ctypes.POINTER(ctypes.c_double)]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -105,11 +112,19 @@ _SIGS = {
 
 
 def load():
-    """Loads the HIP library; raises ImportError if it has not been built."""
+    """Loads the HIP library; raises ImportError if it has not been built.
+
+    torch is imported first when it is installed: the library then binds to
+    the HIP runtime torch already loaded, so the process has one runtime and
+    device tensors can be passed straight through."""
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             raise ImportError(f"{_LIB_PATH} is missing: run `python -m kascade_amd._build`")
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(_LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
@@ -266,20 +281,22 @@ class Context:
         return root_out, rank_out
 
     # ---- trees --------------------------------------------------------------
-    def root_tree(self, parent, root_out=None, depth_out=None, stream=None, **cfg):
+    def root_tree(self, parent, root_out=None, depth_out=None, stream=None, tour_build="auto", **cfg):
         root_out = _like(parent, np.uint64) if root_out is None else root_out
         depth_out = _like(parent, np.int64) if depth_out is None else depth_out
         t = TreeCfg()
         t.rank = _cfg(**cfg)
+        t.tour_build = TOUR_BUILDS[tour_build]
         self._check(self._lib.kc_root_tree(self.handle, _ptr(parent, np.uint64), _numel(parent),
                                            _ptr(root_out, np.uint64), _ptr(depth_out, np.int64),
                                            ctypes.byref(t), stream))
         return root_out, depth_out
 
-    def rank_via_euler_tour(self, succ_array, rank_array, stream=None, **cfg):
+    def rank_via_euler_tour(self, succ_array, rank_array, stream=None, tour_build="auto", **cfg):
         """In place: parents -> roots, rank -> depth (eulertour.hpp:22-27)."""
         t = TreeCfg()
         t.rank = _cfg(**cfg)
+        t.tour_build = TOUR_BUILDS[tour_build]
         self._check(self._lib.kc_rank_via_euler_tour(self.handle, ctypes.byref(t), _ptr(succ_array, np.uint64),
                                                      _ptr(rank_array, np.int64), _numel(succ_array), stream))
 
@@ -342,6 +359,25 @@ class Context:
         self._check(self._lib.kc_probe_random_gather_mode(self.handle, n, reps, mode, ctypes.byref(g),
                                                           ctypes.byref(ms)))
         return g.value, ms.value
+
+    def probe_table_gather(self, n, m, reps=5):
+        g = ctypes.c_double()
+        ms = ctypes.c_double()
+        self._check(self._lib.kc_probe_table_gather(self.handle, n, m, reps, ctypes.byref(g), ctypes.byref(ms)))
+        return g.value, ms.value
+
+    # ---- tuning ---------------------------------------------------------------
+    def tuning_set(self, rows):
+        """rows: iterable of (log2n, spacing_top, spacing_deep, chase_blocks_per_cu, chase_chunk)."""
+        flat = [int(x) for r in rows for x in r]
+        arr = (ctypes.c_uint32 * max(1, len(flat)))(*flat)
+        self._check(self._lib.kc_tuning_set(self.handle, arr, len(flat) // 5))
+
+    def tuning_get(self, n):
+        out = (ctypes.c_uint32 * 5)()
+        rows = self._lib.kc_tuning_get(self.handle, n, out)
+        return {"spacing_top": out[0], "spacing_deep": out[1], "chase_blocks_per_cu": out[2],
+                "chase_chunk": out[3], "small_n": out[4], "table_rows": rows}
 
     def scratch_bytes(self):
         return self._lib.kc_context_scratch_bytes(self.handle)

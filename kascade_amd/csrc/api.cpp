@@ -2,8 +2,12 @@
 // mapping.  The kernels live in the .hip sources.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -187,6 +191,43 @@ class Staged {
   hipStream_t st_;
 };
 
+bool load_tuning_file(Context* ctx, const std::string& path) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::vector<TuneRow> rows;
+  std::string line;
+  while (std::getline(in, line)) {
+    const size_t hash = line.find('#');
+    if (hash != std::string::npos) line.resize(hash);
+    std::istringstream ls(line);
+    TuneRow r{};
+    if (!(ls >> r.log2n >> r.spacing_top >> r.spacing_deep >> r.chase_blocks_per_cu >> r.chase_chunk)) continue;
+    auto pow2 = [](uint32_t x) { return x >= 2 && (x & (x - 1)) == 0; };
+    if (!pow2(r.spacing_top) || !pow2(r.spacing_deep) || r.chase_blocks_per_cu == 0 || r.chase_chunk == 0)
+      continue;
+    rows.push_back(r);
+  }
+  if (rows.empty()) return false;
+  std::sort(rows.begin(), rows.end(), [](const TuneRow& a, const TuneRow& b) { return a.log2n < b.log2n; });
+  ctx->tuning = rows;
+  return true;
+}
+
+// tuning_table.txt next to the shared library, or $KASCADE_AMD_TUNING.
+static void load_default_tuning(Context* ctx) {
+  if (const char* env = std::getenv("KASCADE_AMD_TUNING")) {
+    load_tuning_file(ctx, env);
+    return;
+  }
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void*>(&load_tuning_file), &info) && info.dli_fname) {
+    std::string so(info.dli_fname);
+    const size_t slash = so.rfind('/');
+    const std::string dir = slash == std::string::npos ? std::string(".") : so.substr(0, slash);
+    load_tuning_file(ctx, dir + "/tuning_table.txt");
+  }
+}
+
 }  // namespace kc
 
 using kc::Context;
@@ -247,6 +288,7 @@ kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void** ctx_ou
   ctx->d_err = reinterpret_cast<uint32_t*>(ctx->d_counters + kc::kErrSlot);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+  kc::load_default_tuning(ctx);
   *ctx_out = ctx;
   return KC_OK;
 }
@@ -553,6 +595,38 @@ uint32_t kc_profile_read(void* vctx, uint32_t max_entries, const char** names, d
     k++;
   }
   return static_cast<uint32_t>(ctx->prof.totals.size());
+}
+
+}  // extern "C"
+
+extern "C" {
+
+kc_status kc_tuning_set(void* vctx, const uint32_t* rows, uint32_t nrows) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx || (nrows && !rows)) return KC_EINVAL;
+  std::vector<kc::TuneRow> t;
+  for (uint32_t i = 0; i < nrows; i++) {
+    kc::TuneRow r{rows[5 * i], rows[5 * i + 1], rows[5 * i + 2], rows[5 * i + 3], rows[5 * i + 4]};
+    auto pow2 = [](uint32_t x) { return x >= 2 && (x & (x - 1)) == 0; };
+    if (!pow2(r.spacing_top) || !pow2(r.spacing_deep) || r.chase_blocks_per_cu == 0 || r.chase_chunk == 0)
+      return KC_EINVAL;
+    t.push_back(r);
+  }
+  std::sort(t.begin(), t.end(), [](const kc::TuneRow& a, const kc::TuneRow& b) { return a.log2n < b.log2n; });
+  ctx->tuning = t;
+  return KC_OK;
+}
+
+uint32_t kc_tuning_get(void* vctx, uint64_t n, uint32_t* out5) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx || !out5) return 0;
+  kc::RankTuning t = kc::tuning_for(ctx, n, nullptr);
+  out5[0] = t.spacing_top;
+  out5[1] = t.spacing_deep;
+  out5[2] = t.chase_blocks_per_cu;
+  out5[3] = t.chase_chunk;
+  out5[4] = t.small_n;
+  return static_cast<uint32_t>(ctx->tuning.size());
 }
 
 }  // extern "C"

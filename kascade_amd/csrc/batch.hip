@@ -111,6 +111,59 @@ batch_tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __rest
   if (first && !is_root[p]) rec[nl + nt + p] = Rec{nl + nt + c, -1};
 }
 
+// Linked tour build for a batch of trees (child lists by atomic exchange).
+__global__ void __launch_bounds__(kThreads)
+batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
+                       uint32_t* __restrict__ head, Rec* __restrict__ rec, uint32_t* err,
+                       unsigned long long* roots) {
+  __shared__ uint32_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
+  const uint64_t t = first + threadIdx.x;
+  bool root = false;
+  if (t < nt) {
+    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
+    const uint64_t i = t - j.base;
+    uint64_t p = j.succ[i];
+    if (p >= j.n) {
+      raise_err(err, kErrRange);
+      p = i;
+    }
+    root = (p == i);
+    if (root) {
+      rec[nl + t] = Rec{nl + t, 0};
+    } else {
+      const uint32_t prev = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
+      rec[nl + t] = Rec{prev != 0xFFFFFFFFu ? nl + nt + prev : nl + j.base + p, 1};
+    }
+  }
+  wave_append(roots, root);
+}
+
+__global__ void __launch_bounds__(kThreads)
+batch_tree_down_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
+                       const uint32_t* __restrict__ head, Rec* __restrict__ rec) {
+  __shared__ uint32_t s_range[2];
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
+  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
+  const uint64_t t = first + threadIdx.x;
+  if (t >= nt) return;
+  const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
+  const uint64_t i = t - j.base;
+  const uint64_t p = j.succ[i];
+  if (p == i || p >= j.n) {
+    rec[nl + nt + t] = Rec{nl + nt + t, 0};
+    return;
+  }
+  const uint32_t h = head[t];
+  rec[nl + nt + t] = Rec{h != 0xFFFFFFFFu ? nl + nt + h : nl + t, -1};
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
 // Copies outputs back: list element e -> its job, tree element nl + t -> its job.
 __global__ void __launch_bounds__(kThreads)
 batch_scatter_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t count,
@@ -135,7 +188,7 @@ unsigned bits_for(uint64_t x) {
 }  // namespace
 
 void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const kc_rank_cfg* cfg,
-                       hipStream_t st) {
+                       int tour_build, hipStream_t st) {
   std::vector<BatchJob> lists, trees;
   uint64_t nl = 0, nt = 0;
   for (const BatchJob& j : jobs_in) {
@@ -172,7 +225,17 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
                        d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
   }
   uint32_t* keys = nullptr;
-  if (nt) {
+  const bool linked = tour_build != KC_TOUR_SORTED;
+  if (nt && linked) {
+    Phase ph(ctx, "batch_tour", st);
+    uint32_t* head = scratch.get<uint32_t>(nt);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
+                       0xFFFFFFFFu);
+    hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, rec, ctx->d_err, ctr + 1);
+    hipLaunchKernelGGL(batch_tree_down_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, rec);
+  } else if (nt) {
     Phase ph(ctx, "batch_tour", st);
     keys = scratch.get<uint32_t>(nt);
     uint32_t* vals = scratch.get<uint32_t>(nt);
@@ -186,7 +249,7 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   }
   uint64_t counts[2] = {0, 0};
   read_counters(ctx, st, 0, 2, counts);
-  RankTuning tune = tuning_for(std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
+  RankTuning tune = tuning_for(ctx, std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
   rank_records(ctx, rec, total, nl + nt, root, rank, counts[0] + 2 * counts[1], tune, algo, st);

@@ -19,7 +19,8 @@ struct BatchJob {
 };
 
 // Ranks every job of the batch in one pass; all pointers are device pointers.
+// tour_build: kc_tour_build for the trees of the batch.
 void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs, const kc_rank_cfg* cfg,
-                       hipStream_t st);
+                       int tour_build, hipStream_t st);
 
 }  // namespace kc

@@ -91,8 +91,10 @@ class Profiler {
 constexpr int kCounterSlots = 64;
 constexpr int kErrSlot = kCounterSlots - 1;
 
+struct TuneRow;
 struct Context {
   Profiler prof;
+  std::vector<TuneRow> tuning;  // sorted by log2n; empty = built-in table
   int device = 0;
   uint64_t budget = 0;
   hipStream_t stream = nullptr;
@@ -167,7 +169,19 @@ struct RankTuning {
   uint32_t chase_blocks_per_cu;
   uint32_t chase_chunk;  // rulers a wave takes per work-counter atomic
 };
-RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg);
+// Per-size-class tuning table row: log2(n) class and the knobs picked for it
+// (written by kascade_amd/autotune.py into tuning_table.txt, which contexts
+// load at creation; a built-in table is used otherwise).
+struct TuneRow {
+  uint32_t log2n;
+  uint32_t spacing_top;
+  uint32_t spacing_deep;
+  uint32_t chase_blocks_per_cu;
+  uint32_t chase_chunk;
+};
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg);
+// Loads a tuning table file (one row per line, '#' comments); false if absent.
+bool load_tuning_file(Context* ctx, const std::string& path);
 
 // ---- internal entry points (device pointers only) -----------------------
 // Level-0 ranking of separate succ / weight arrays into root / rank arrays.

@@ -67,6 +67,53 @@ tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__
   if (first && parent[p] != p) tour[n + p] = Rec{n + c, -1};
 }
 
+// Linked build: children lists by atomic exchange on the parent's head slot
+// (one random atomic per vertex; every record write is coalesced).  The child
+// order is whatever order the exchanges happened in; any cyclic order of a
+// vertex's neighbours gives a valid tour and the same (root, depth).
+__global__ void __launch_bounds__(kThreads)
+tour_link_atomic_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __restrict__ head,
+                        Rec* __restrict__ tour, uint32_t* err, unsigned long long* roots) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool is_root = false;
+  if (v < n) {
+    uint64_t p = parent[v];
+    if (p >= n) {
+      raise_err(err, kErrRange);
+      p = v;
+    }
+    is_root = (p == v);
+    if (is_root) {
+      tour[v] = Rec{v, 0};  // terminal dummy
+    } else {
+      const uint32_t prev = atomicExch(&head[p], static_cast<uint32_t>(v));
+      // up(v) -> down(previously inserted sibling), or up(p) / p's dummy
+      tour[v] = Rec{prev != 0xFFFFFFFFu ? n + prev : p, 1};
+    }
+  }
+  wave_append(roots, is_root);
+}
+
+// down(v) -> down(head child of v), or up(v) for a leaf; unused slot for roots.
+__global__ void __launch_bounds__(kThreads)
+tour_down_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_t* __restrict__ head,
+                 Rec* __restrict__ tour) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t p = parent[v];
+  if (p == v || p >= n) {
+    tour[n + v] = Rec{n + v, 0};
+    return;
+  }
+  const uint32_t h = head[v];
+  tour[n + v] = Rec{h != 0xFFFFFFFFu ? n + h : v, -1};
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) p[i] = value;
+}
+
 unsigned bits_for(uint64_t x) {
   unsigned b = 1;
   while (b < 64 && (uint64_t{1} << b) <= x) b++;
@@ -75,12 +122,30 @@ unsigned bits_for(uint64_t x) {
 
 }  // namespace
 
-// Builds the 2n-element tour into `tour`; returns the number of roots through
-// counter slot 0.
-static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
+// Builds the 2n-element tour into `tour`; returns the number of roots.
+// sorted: children grouped by a stable radix sort of (parent, child) pairs, so
+// the tour is the same on every run; otherwise children are linked by atomic
+// exchange (faster; the tour's child order may differ between runs, the ranks
+// never do).
+static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour, bool sorted,
                            hipStream_t st) {
   if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree rooting supports n < 2^32 - 1"};
+  Phase ph(ctx, "tour_build", st);
   Scratch scratch(ctx);
+  if (!sorted) {
+    uint32_t* head = scratch.get<uint32_t>(n);
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, head, n,
+                       0xFFFFFFFFu);
+    hipLaunchKernelGGL(tour_link_atomic_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
+                       parent, n, head, tour, ctx->d_err,
+                       reinterpret_cast<unsigned long long*>(ctx->d_counters));
+    hipLaunchKernelGGL(tour_down_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
+                       head, tour);
+    uint64_t roots = 0;
+    read_counters(ctx, st, 0, 1, &roots);
+    return roots;
+  }
   uint32_t* keys = scratch.get<uint32_t>(n);
   uint32_t* vals = scratch.get<uint32_t>(n);
   KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
@@ -98,7 +163,7 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
 void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
                        hipStream_t st) {
   if (n == 0) return;
-  build_tour(ctx, parent, n, tour, st);
+  build_tour(ctx, parent, n, tour, true, st);
 }
 
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
@@ -106,9 +171,10 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   if (n == 0) return;
   Scratch scratch(ctx);
   Rec* tour = scratch.get<Rec>(2 * n);
-  const uint64_t roots = build_tour(ctx, parent, n, tour, st);
+  const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
+  const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, st);
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
-  RankTuning tune = tuning_for(2 * n, rc);
+  RankTuning tune = tuning_for(ctx, 2 * n, rc);
   int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};

@@ -19,6 +19,12 @@ perm_idx_kernel(uint64_t* __restrict__ idx, uint64_t n, uint64_t seed) {
 }
 
 __global__ void __launch_bounds__(kThreads)
+table_idx_kernel(uint64_t* __restrict__ idx, uint64_t n, uint64_t m, uint64_t seed) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i < n) idx[i] = splitmix64(seed ^ (i * 0x9E3779B97F4A7C15ull)) % m;
+}
+
+__global__ void __launch_bounds__(kThreads)
 fill_kernel(uint64_t* __restrict__ a, uint64_t n) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i < n) a[i] = i * 3 + 1;
@@ -105,4 +111,46 @@ extern "C" kc_status kc_probe_random_gather_mode(void* vctx, uint64_t n, uint32_
 extern "C" kc_status kc_probe_random_gather(void* vctx, uint64_t n, uint32_t reps, double* gbps_out,
                                             double* ms_out) {
   return kc_probe_random_gather_mode(vctx, n, reps, 0, gbps_out, ms_out);
+}
+
+// Gather from a table of m u64 (random indices in [0, m)), n gathers: the rate
+// as a function of the table size shows which level of the memory hierarchy
+// serves a random lookup.
+extern "C" kc_status kc_probe_table_gather(void* vctx, uint64_t n, uint64_t m, uint32_t reps,
+                                           double* gathers_per_s, double* ms_out) {
+  auto* ctx = static_cast<kc::Context*>(vctx);
+  if (!ctx || n == 0 || m == 0 || m > n || reps == 0) return KC_EINVAL;
+  try {
+    KC_HIP(hipSetDevice(ctx->device));
+    kc::Scratch scratch(ctx);
+    uint64_t* src = scratch.get<uint64_t>(m);
+    uint64_t* idx = scratch.get<uint64_t>(n);
+    uint64_t* dst = scratch.get<uint64_t>(n);
+    hipStream_t st = ctx->stream;
+    const unsigned blocks = kc::blocks_for(n, 256);
+    hipLaunchKernelGGL(kc::table_idx_kernel, dim3(blocks), dim3(256), 0, st, idx, n, m, 777ull);
+    hipLaunchKernelGGL(kc::fill_kernel, dim3(kc::blocks_for(m, 256)), dim3(256), 0, st, src, m);
+    hipLaunchKernelGGL(kc::gather_kernel, dim3(blocks), dim3(256), 0, st, src, idx, dst, n);
+    hipEvent_t a, b;
+    KC_HIP(hipEventCreate(&a));
+    KC_HIP(hipEventCreate(&b));
+    KC_HIP(hipEventRecord(a, st));
+    for (uint32_t r = 0; r < reps; r++)
+      hipLaunchKernelGGL(kc::gather_kernel, dim3(blocks), dim3(256), 0, st, src, idx, dst, n);
+    KC_HIP(hipEventRecord(b, st));
+    KC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    KC_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double per = ms / reps;
+    if (ms_out) *ms_out = per;
+    if (gathers_per_s) *gathers_per_s = static_cast<double>(n) / (per * 1e-3);
+    return KC_OK;
+  } catch (const kc::KcError& e) {
+    ctx->last_error = e.what;
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return KC_ENOMEM;
+  }
 }

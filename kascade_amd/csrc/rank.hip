@@ -542,15 +542,35 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 
 }  // namespace
 
-RankTuning tuning_for(uint64_t n, const kc_rank_cfg* cfg) {
+// Built-in table (the autotuner's output for MI355X at the time of the last
+// sweep; a tuning_table.txt next to the library overrides it).
+const TuneRow kBuiltinTuning[] = {
+    {10, 32, 16, 8, 64}, {12, 8, 16, 4, 256}, {14, 8, 8, 4, 64},
+    {16, 8, 8, 8, 64}, {18, 8, 16, 1, 64}, {20, 8, 8, 1, 64},
+    {22, 16, 16, 1, 64}, {24, 32, 16, 2, 64}, {26, 128, 16, 1, 64},
+    {27, 64, 8, 1, 64},
+};
+
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
   RankTuning t{};
-  // Defaults from the spacing sweep recorded in the design notes.
-  t.spacing_top = n >= (uint64_t{1} << 24) ? 32 : 16;
-  t.spacing_deep = 16;
   t.max_levels = 8;
   t.small_n = kSmallMax;
-  t.chase_blocks_per_cu = 4;
-  t.chase_chunk = 256;
+  uint32_t lg = 0;
+  while (lg < 63 && (uint64_t{2} << lg) <= n) lg++;
+  const TuneRow* rows = kBuiltinTuning;
+  size_t nrows = sizeof(kBuiltinTuning) / sizeof(kBuiltinTuning[0]);
+  if (ctx && !ctx->tuning.empty()) {
+    rows = ctx->tuning.data();
+    nrows = ctx->tuning.size();
+  }
+  // the row of the largest size class not above log2(n), else the smallest
+  const TuneRow* pick = &rows[0];
+  for (size_t i = 0; i < nrows; i++)
+    if (rows[i].log2n <= lg) pick = &rows[i];
+  t.spacing_top = pick->spacing_top;
+  t.spacing_deep = pick->spacing_deep;
+  t.chase_blocks_per_cu = pick->chase_blocks_per_cu;
+  t.chase_chunk = pick->chase_chunk;
   if (cfg) {
     if (cfg->chase_blocks_per_cu) t.chase_blocks_per_cu = std::min(cfg->chase_blocks_per_cu, 64u);
     if (cfg->chase_chunk) t.chase_chunk = std::min(cfg->chase_chunk, 1u << 16);
@@ -589,7 +609,7 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
                       uint64_t* root, int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st) {
   if (n == 0) return;
   if (n >= kRootFlag >> 1) throw KcError{KC_EINVAL, "n must be below 2^62"};
-  RankTuning tune = tuning_for(n, cfg);
+  RankTuning tune = tuning_for(ctx, n, cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};

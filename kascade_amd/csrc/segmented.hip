@@ -104,7 +104,7 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
                      seg_off, nseg, ctx->d_err);
   check_err(ctx, st);
   // size class = mean segment size
-  RankTuning tune = tuning_for(std::max<uint64_t>(1, total / nseg), cfg);
+  RankTuning tune = tuning_for(ctx, std::max<uint64_t>(1, total / nseg), cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};

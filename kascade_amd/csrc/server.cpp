@@ -130,7 +130,7 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
         if (j.kind == KC_JOB_TREE) b.w = nullptr;
         jobs.push_back(b);
       }
-      kc::rank_batch_device(ctx, jobs, nullptr, st);
+      kc::rank_batch_device(ctx, jobs, nullptr, KC_TOUR_AUTO, st);
       for (auto& s : staged) {
         KC_HIP(hipMemcpyAsync(s.t->job.root_out, s.root, s.t->job.n * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         KC_HIP(hipMemcpyAsync(s.t->job.rank_out, s.rank, s.t->job.n * sizeof(int64_t), hipMemcpyDeviceToHost, st));

@@ -229,3 +229,22 @@ def test_cpp_dropin_header():
     say("cpp dropin")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_tuning_table(ctx):
+    """A per-size-class tuning table changes the knobs, never the result."""
+    say("tuning")
+    before = ctx.tuning_get(1 << 20)
+    try:
+        ctx.tuning_set([(10, 4, 4, 1, 64), (18, 128, 8, 2, 1024)])
+        assert ctx.tuning_get(1 << 12)["spacing_top"] == 4
+        assert ctx.tuning_get(1 << 20)["spacing_top"] == 128
+        for n in [5000, 300000]:
+            succ, w = o.gen_list(n, 17 + n, True)
+            root, rank = ctx.rank_list(dev(succ), dev(w))
+            er, ek = o.list_closed_form(n, 17 + n, w)
+            assert (host(root) == er).all() and (host(rank) == ek).all()
+    finally:
+        ctx.tuning_set([])
+    assert ctx.tuning_get(1 << 20)["spacing_top"] in (8, 16, 32, 64, 128)
+    assert before["small_n"] == 4096

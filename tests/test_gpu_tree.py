@@ -41,21 +41,23 @@ def relabel_forest(seed):
     return out
 
 
+@pytest.mark.parametrize("build", ["linked", "sorted"])
 @pytest.mark.parametrize("algo", ["srs", "pd"])
 @pytest.mark.parametrize("n", [1, 2, 3, 100, 2049, 4097, 100003, 1 << 20])
-def test_root_random_tree(ctx, algo, n):
-    say("tree", algo, n)
+def test_root_random_tree(ctx, algo, n, build):
+    say("tree", algo, n, build)
     parent = o.gen_tree(n, 31 + n)
-    root, depth = ctx.root_tree(dev(parent), algorithm=algo)
+    root, depth = ctx.root_tree(dev(parent), algorithm=algo, tour_build=build)
     er, ed = o.tree_closed_form(n, 31 + n)
     assert (host(root) == er).all() and (host(depth) == ed).all()
 
 
+@pytest.mark.parametrize("build", ["linked", "sorted"])
 @pytest.mark.parametrize("algo", ["srs", "pd"])
-def test_root_forest_shapes(ctx, algo):
-    say("forest shapes", algo)
+def test_root_forest_shapes(ctx, algo, build):
+    say("forest shapes", algo, build)
     parent = relabel_forest(3)
-    root, depth = ctx.root_tree(dev(parent), algorithm=algo)
+    root, depth = ctx.root_tree(dev(parent), algorithm=algo, tour_build=build)
     er, ed = o.euler_tour_rooting(parent)
     assert (host(root) == er).all() and (host(depth) == ed).all()
 
@@ -105,11 +107,12 @@ def test_tree_errors(ctx):
     with pytest.raises(kascade_amd.KascadeError) as e:
         ctx.root_tree(dev(np.array([0, 7, 0], np.uint64)))
     assert e.value.status == kascade_amd.KC_EINVAL
-    for n in [2, 3, 9000]:                                   # parent cycle, no root
-        cyc = np.roll(np.arange(n, dtype=np.uint64), 1)
-        with pytest.raises(kascade_amd.KascadeError) as e:
-            ctx.root_tree(dev(cyc))
-        assert e.value.status == kascade_amd.KC_ECYCLE, n
+    for build in ["linked", "sorted"]:
+        for n in [2, 3, 9000]:                               # parent cycle, no root
+            cyc = np.roll(np.arange(n, dtype=np.uint64), 1)
+            with pytest.raises(kascade_amd.KascadeError) as e:
+                ctx.root_tree(dev(cyc), tour_build=build)
+            assert e.value.status == kascade_amd.KC_ECYCLE, (n, build)
 
 
 @pytest.mark.parametrize("n", [1 << 22])
