@@ -21,6 +21,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per element moved by the level-0 chase, from rocprofv3 FETCH_SIZE
+# (x2.0, calibrated on the pack kernel's known 2 GiB read) and WRITE_SIZE (x1.0)
+# at n = 2^27: 127.8 B read + 33.6 B written.
+CHASE_TRAFFIC_BYTES_PER_ELEMENT = 127.8 + 33.6
 BYTES_PER_ELEMENT = 32  # succ 8 in + weight 8 in + root 8 out + rank 8 out
 CHASE_BYTES_PER_ELEMENT = 32  # chase: read (succ, w) 16 B + write (ruler, dist) 16 B per element
 
@@ -288,7 +292,8 @@ def main():
                                f"(C2: uint64 successors, weighted ranks bit-exact)",
                    "n": n, "algorithm": args.algorithm, "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": CHASE_TRAFFIC_BYTES_PER_ELEMENT * n if dom == "chase_ruler_L0" else None,
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "whole_call_effective_gbps": whole_gbps,
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},

@@ -162,11 +162,26 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
   }
 }
 
-// T[i].s (sublist root id) -> this level's id.
+// T[i].s (sublist root id) -> this level's id; T8[i] = T[i].w; mixed_roots
+// is set when two rulers have different roots (a forest).
+__global__ void first_root_kernel(const Rec* __restrict__ T, uint64_t* __restrict__ first,
+                                  uint32_t* __restrict__ mixed_roots) {
+  if (threadIdx.x == 0) {
+    first[0] = T[0].s;
+    mixed_roots[0] = 0;
+  }
+}
+
 __global__ void __launch_bounds__(kThreads)
-map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m) {
+map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m,
+                 const uint64_t* __restrict__ first, int64_t* __restrict__ T8,
+                 uint32_t* __restrict__ mixed_roots) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i < m) T[i].s = sub_orig[T[i].s];
+  if (i >= m) return;
+  const Rec t = T[i];
+  if (t.s != first[0] && *mixed_roots == 0) *mixed_roots = 1;
+  T[i].s = sub_orig[t.s];
+  T8[i] = t.w;
 }
 
 // Rank propagation (ruler_propagation.hpp:148-257, pull mode): every claimed
@@ -176,19 +191,29 @@ map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uin
 // their record is re-marked with that compact id.
 __global__ void __launch_bounds__(kThreads)
 propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
+                 const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
                  unsigned long long* u_ctr) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  // All rulers in one list (one root): gather only the 8-byte rank, from a
+  // table half the size, so more of it stays in L2.
+  const bool one_root = *mixed_roots == 0;
+  const uint64_t root0 = T[0].s;
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
     r = rec[v];
     if (r.s & kRootFlag) {
       if (v < n_out) {
-        const Rec t = T[r.s & kIdMask];
-        root[v * stride] = t.s;
-        rank[v * stride] = t.w - r.w;
+        if (one_root) {
+          root[v * stride] = root0;
+          rank[v * stride] = T8[r.s & kIdMask] - r.w;
+        } else {
+          const Rec t = T[r.s & kIdMask];
+          root[v * stride] = t.s;
+          rank[v * stride] = t.w - r.w;
+        }
       }
     } else {
       unclaimed = true;
@@ -481,8 +506,12 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
   rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, 2, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
                     level + 1, st);
+  int64_t* T8 = scratch.get<int64_t>(m);
+  uint64_t* first = scratch.get<uint64_t>(1);
+  uint32_t* mixed_roots = scratch.get<uint32_t>(1);
+  hipLaunchKernelGGL(first_root_kernel, dim3(1), dim3(64), 0, st, T, first, mixed_roots);
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, T,
-                     sub_orig, m);
+                     sub_orig, m, first, T8, mixed_roots);
 
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
@@ -493,7 +522,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, T, root, rank, stride, u_ids, u_orig, ucap, ctr + kSlotUnclaimed);
+                       n, n_out, T, T8, mixed_roots, root, rank, stride, u_ids, u_orig, ucap,
+                       ctr + kSlotUnclaimed);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
@@ -545,10 +575,10 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 // Built-in table (the autotuner's output for MI355X at the time of the last
 // sweep; a tuning_table.txt next to the library overrides it).
 const TuneRow kBuiltinTuning[] = {
-    {10, 32, 16, 8, 64}, {12, 8, 16, 4, 256}, {14, 8, 8, 4, 64},
-    {16, 8, 8, 8, 64}, {18, 8, 16, 1, 64}, {20, 8, 8, 1, 64},
-    {22, 16, 16, 1, 64}, {24, 32, 16, 2, 64}, {26, 128, 16, 1, 64},
-    {27, 64, 8, 1, 64},
+    {10, 128, 16, 2, 1024}, {12, 128, 16, 4, 256}, {14, 8, 16, 1, 64},
+    {16, 8, 8, 1, 64}, {18, 8, 16, 8, 64}, {20, 8, 8, 1, 64},
+    {22, 16, 16, 1, 64}, {24, 32, 16, 2, 64}, {26, 128, 8, 2, 64},
+    {27, 128, 8, 4, 256},
 };
 
 RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
