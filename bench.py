@@ -54,18 +54,59 @@ def parse():
 
 
 def cpu_baseline(n_sample, seed):
-    """Oracle restatement of the reference's sync pointer doubling
-    (pointer_doubling_generic.hpp:608-700) on emulated ranks, one thread each."""
+    """The oracle's restatement of the reference's CPU path on emulated MPI
+    ranks (one thread per rank): the sparse ruling set in the submission
+    configuration (sparse_ruling_set.cpp:335-393) as the value, sync pointer
+    doubling (pointer_doubling_generic.hpp:608-700) beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as o
     ranks = max(1, min(16, os.cpu_count() or 1))
     succ, w = o.gen_list(n_sample, seed, True)
     t0 = time.perf_counter()
-    r, k, rounds = o.pointer_doubling(succ, w, ranks)
+    r, k = o.sparse_ruling_set(succ, w, ranks)
     dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    r2, k2, rounds = o.pointer_doubling(succ, w, ranks)
+    dt_pd = time.perf_counter() - t0
+    er, ek = o.list_closed_form(n_sample, seed, w)
+    exact = bool((r == er).all() and (k == ek).all() and (r2 == er).all() and (k2 == ek).all())
     return {"value": n_sample / dt, "unit": "elements/s", "cores": ranks, "kind": "port",
-            "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}), sync pointer "
-                      f"doubling on {ranks} emulated ranks, {rounds} rounds, {dt:.2f} s"}
+            "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}): sparse ruling set "
+                      f"(ultimate-sanders f=2, spawn, post-invert, local contraction, 2 rounds + PD) on "
+                      f"{ranks} emulated ranks {dt:.2f} s; sync pointer doubling {rounds} rounds {dt_pd:.2f} s "
+                      f"({n_sample / dt_pd:.3e} elements/s); bit-exact vs closed form: {exact}"}
+
+
+def c1_bench(ctx, seed, reps=20):
+    """C1: one random list of 2^20 elements, unit weights.  GPU sparse ruling
+    set and pointer doubling beside the oracle's restatement of the reference's
+    sync pointer doubling on 4 emulated MPI ranks; all outputs compared."""
+    import numpy as np
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as o
+    n = 1 << 20
+    succ_h, w_h = o.gen_list(n, seed, False)
+    succ = torch.from_numpy(succ_h).cuda()
+    root = torch.empty(n, dtype=torch.uint64, device="cuda")
+    rank = torch.empty(n, dtype=torch.int64, device="cuda")
+    out = {"n": n, "weights": "unit (set_initial_ranking_state)"}
+    for algo in ("srs", "pd"):
+        ctx.rank_list(succ, None, root, rank, algorithm=algo)
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ctx.rank_list(succ, None, root, rank, algorithm=algo)
+            torch.cuda.synchronize()
+            t.append(time.perf_counter() - t0)
+        out[f"gpu_{algo}_ms"] = 1e3 * sorted(t)[len(t) // 2]
+    t0 = time.perf_counter()
+    cr, ck, rounds = o.pointer_doubling(succ_h, w_h, 4)
+    out["cpu_pd_4ranks_ms"] = 1e3 * (time.perf_counter() - t0)
+    out["cpu_pd_rounds"] = rounds
+    out["bit_exact_gpu_vs_cpu"] = bool((root.cpu().numpy() == cr).all() and (rank.cpu().numpy() == ck).all())
+    return out
 
 
 def serve_bench(ctx, jobs, seed, slots):
@@ -302,6 +343,7 @@ def main():
         out["bit_exact_vs_closed_form"] = check
     del succ, w, root, rank
     torch.cuda.empty_cache()
+    out["c1"] = c1_bench(ctx, args.seed)
     if not args.no_serve:
         out["serve"] = serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots)
     if not args.no_cpu_baseline:

@@ -45,6 +45,7 @@ def lib():
             "orc_pack": (c.c_int, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_unpack": (None, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_euler_tour_rooting": (c.c_int, [u64p, i64p, c.c_uint64]),
+            "orc_sparse_ruling_set": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int, c.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -143,5 +144,14 @@ def euler_tour_rooting(parent):
     s = np.array(parent, dtype=np.uint64, copy=True)
     r = np.empty(len(s), np.int64)
     rc = lib().orc_euler_tour_rooting(_p64(s), _p64(r), len(s))
+    assert rc == 0
+    return s, r
+
+
+def sparse_ruling_set(succ, w, p=4, levels=2):
+    """The reference's distributed sparse ruling set on p emulated ranks."""
+    s = np.array(succ, dtype=np.uint64, copy=True)
+    r = np.array(w, dtype=np.int64, copy=True)
+    rc = lib().orc_sparse_ruling_set(_p64(s), _p64(r), len(s), p, levels)
     assert rc == 0
     return s, r

@@ -225,3 +225,22 @@ def test_euler_tour_forest_and_paths():
     br, bd = bfs_depth(relab)
     er, ed = o.euler_tour_rooting(relab)
     assert (er == br).all() and (ed == bd).all()
+
+
+@pytest.mark.parametrize("p", [1, 2, 4, 7])
+@pytest.mark.parametrize("n,nlists", [(1, 1), (10, 1), (100, 3), (5000, 40), (20000, 1), (30000, 3000)])
+def test_sparse_ruling_set_ranks(p, n, nlists):
+    """The distributed sparse-ruling-set restatement on p emulated ranks gives
+    the sequential walk's (root, rank) on list forests."""
+    succ, w = random_forest_lists(n, nlists, 11 * n + p)
+    r, k = o.sparse_ruling_set(succ, w, p)
+    er, ek = o.sequential_walk(succ, w)
+    assert (r == er).all() and (k == ek).all()
+
+
+def test_sparse_ruling_set_generated_lists():
+    for local in (False, True):
+        succ, w = o.gen_list(100003, 9, True, local=local)
+        r, k = o.sparse_ruling_set(succ, w, 4)
+        er, ek = o.sequential_walk(succ, w)
+        assert (r == er).all() and (k == ek).all()
