@@ -2,14 +2,19 @@
 n = 2^27 elements on one MI355X (sparse ruling set through the C ABI).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--algorithm srs|pd]
+    python bench.py --workload tree|segmented
 
 Inputs: a seeded random list (successors of a Feistel permutation, int64
 weights uniform in [1,100], 0 on the tail) generated on the device and
 resident in HBM before the timed region.  One step = one kc_rank_list call
-(pack, ruler chase, recursion, propagation), synchronous.  Prints one JSON
-line.  The CPU baseline is the oracle's restatement of the reference's sync
-pointer doubling on emulated MPI ranks (one thread per rank), timed on a
-bounded sample of the same workload.
+(pack, ruler chase, recursion, propagation), synchronous.  The last step's
+output is compared bit for bit with the generator's closed form.  Prints one
+JSON line.  The random-gather ceiling of the device is measured in the same
+run (u64 gather through a random permutation of 2^27) and the dominant
+kernel is reported against both it and the HBM peak.  The CPU baseline is
+the oracle's restatement of the reference's MPI path (sparse ruling set and
+sync pointer doubling on emulated ranks, one thread per rank, plus the
+sequential walk on one core), timed on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -46,41 +51,74 @@ def parse():
     ap.add_argument("--spacing-deep", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
-    ap.add_argument("--check", action="store_true", help="verify the last step against the closed form")
+    ap.add_argument("--no-check", action="store_true", help="skip the bit-exact check of the last step")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c1", action="store_true", help="skip the C1 section (2^20 unit-weight list vs CPU)")
     ap.add_argument("--serve-jobs", type=int, default=300)
     ap.add_argument("--serve-slots", type=int, default=3)
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may run on (the box's CPU share), capped at 16."""
+    try:
+        c = len(os.sched_getaffinity(0))
+    except AttributeError:
+        c = os.cpu_count() or 1
+    return max(1, min(16, c))
+
+
 def cpu_baseline(n_sample, seed):
     """The oracle's restatement of the reference's CPU path on emulated MPI
     ranks (one thread per rank): the sparse ruling set in the submission
-    configuration (sparse_ruling_set.cpp:335-393) as the value, sync pointer
-    doubling (pointer_doubling_generic.hpp:608-700) beside it."""
+    configuration (sparse_ruling_set.cpp:335-393) on all host cores as the
+    value; the same on 4 ranks, sync pointer doubling
+    (pointer_doubling_generic.hpp:608-700) and the sequential walk
+    (local_pointer_chasing.cpp:18-59, one core) beside it."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as o
-    ranks = max(1, min(16, os.cpu_count() or 1))
+    ranks = host_cores()
     succ, w = o.gen_list(n_sample, seed, True)
-    t0 = time.perf_counter()
-    r, k = o.sparse_ruling_set(succ, w, ranks)
-    dt = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    r2, k2, rounds = o.pointer_doubling(succ, w, ranks)
-    dt_pd = time.perf_counter() - t0
     er, ek = o.list_closed_form(n_sample, seed, w)
-    exact = bool((r == er).all() and (k == ek).all() and (r2 == er).all() and (k2 == ek).all())
-    return {"value": n_sample / dt, "unit": "elements/s", "cores": ranks, "kind": "port",
-            "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}): sparse ruling set "
-                      f"(ultimate-sanders f=2, spawn, post-invert, local contraction, 2 rounds + PD) on "
-                      f"{ranks} emulated ranks {dt:.2f} s; sync pointer doubling {rounds} rounds {dt_pd:.2f} s "
-                      f"({n_sample / dt_pd:.3e} elements/s); bit-exact vs closed form: {exact}"}
+
+    def run(fn):
+        t0 = time.perf_counter()
+        res = fn()
+        dt = time.perf_counter() - t0
+        return dt, bool((res[0] == er).all() and (res[1] == ek).all())
+
+    dt, ok = run(lambda: o.sparse_ruling_set(succ, w, ranks))
+    dt4, ok4 = run(lambda: o.sparse_ruling_set(succ, w, 4))
+    dt_pd, ok_pd = run(lambda: o.pointer_doubling(succ, w, ranks))
+    dt_seq, ok_seq = run(lambda: o.sequential_walk(succ, w))
+    rate = lambda t: n_sample / t
+    return {"value": rate(dt), "unit": "elements/s", "cores": ranks, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}) of the same generator: "
+                      f"sparse ruling set (ultimate-sanders f=2, spawn, post-invert, local contraction, "
+                      f"2 rounds + PD) on {ranks} emulated ranks, {dt:.2f} s",
+            "srs_4_ranks_elements_per_s": rate(dt4),
+            "pd_elements_per_s": rate(dt_pd),
+            "sequential_walk_1_core_elements_per_s": rate(dt_seq),
+            "bit_exact_vs_closed_form": ok and ok4 and ok_pd and ok_seq}
 
 
 def c1_bench(ctx, seed, reps=20):
     """C1: one random list of 2^20 elements, unit weights.  GPU sparse ruling
-    set and pointer doubling beside the oracle's restatement of the reference's
-    sync pointer doubling on 4 emulated MPI ranks; all outputs compared."""
+    set and pointer doubling beside the oracle's restatements of the reference's
+    sync pointer doubling and sparse ruling set on 4 emulated MPI ranks; all
+    outputs compared."""
     import numpy as np
     import torch
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -105,7 +143,12 @@ def c1_bench(ctx, seed, reps=20):
     cr, ck, rounds = o.pointer_doubling(succ_h, w_h, 4)
     out["cpu_pd_4ranks_ms"] = 1e3 * (time.perf_counter() - t0)
     out["cpu_pd_rounds"] = rounds
-    out["bit_exact_gpu_vs_cpu"] = bool((root.cpu().numpy() == cr).all() and (rank.cpu().numpy() == ck).all())
+    t0 = time.perf_counter()
+    sr, sk = o.sparse_ruling_set(succ_h, w_h, 4)
+    out["cpu_srs_4ranks_ms"] = 1e3 * (time.perf_counter() - t0)
+    gr, gk = root.cpu().numpy(), rank.cpu().numpy()
+    out["bit_exact_gpu_vs_cpu"] = bool((gr == cr).all() and (gk == ck).all()
+                                       and (gr == sr).all() and (gk == sk).all())
     return out
 
 
@@ -215,12 +258,31 @@ def tree_main(args, ctx):
                       "tour_build": args.tour_build},
            "effective_gbps": 24 * n / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
-    if args.check:
+    if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
         er, ed = o.tree_closed_form(n, args.seed)
         out["bit_exact_vs_closed_form"] = bool((root.cpu().numpy() == er).all() and (depth.cpu().numpy() == ed).all())
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = tree_cpu_baseline(min(args.cpu_sample, n, 1 << 22), args.seed)
     print(json.dumps(out), flush=True)
+
+
+def tree_cpu_baseline(n_sample, seed):
+    """The oracle's restatement of the reference's Euler-tour rooting
+    (eulertour.cpp:33-127, 195-278; one address space, one core) on a bounded
+    sample of the same tree generator."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as o
+    parent = o.gen_tree(n_sample, seed)
+    er, ed = o.tree_closed_form(n_sample, seed)
+    t0 = time.perf_counter()
+    r, d = o.euler_tour_rooting(parent)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "vertices/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"random rooted tree n={n_sample} of the same generator, Euler-tour rooting "
+                      f"(tour build, linking, ranking by walk, map back) on one core, {dt:.2f} s",
+            "bit_exact_vs_closed_form": bool((r == er).all() and (d == ed).all())}
 
 
 def segmented_main(args, ctx):
@@ -250,7 +312,7 @@ def segmented_main(args, ctx):
                       "segments": args.segments, "total_elements": total, "seed": args.seed},
            "effective_gbps": (16 * total + 8 * args.segments) / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
-    if args.check:
+    if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
         ok = True
@@ -308,9 +370,10 @@ def main():
         alg_bytes = BYTES_PER_ELEMENT * n
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     whole_gbps = BYTES_PER_ELEMENT * n / (ms_per_step * 1e-3) / 1e9
+    traffic = CHASE_TRAFFIC_BYTES_PER_ELEMENT * n if dom == "chase_ruler_L0" else None
 
     check = None
-    if args.check:
+    if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
         er, ek = o.list_closed_form(n, args.seed, w.cpu().numpy())
@@ -334,7 +397,7 @@ def main():
                    "n": n, "algorithm": args.algorithm, "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": CHASE_TRAFFIC_BYTES_PER_ELEMENT * n if dom == "chase_ruler_L0" else None,
+                     "traffic": traffic,
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "whole_call_effective_gbps": whole_gbps,
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
@@ -343,7 +406,23 @@ def main():
         out["bit_exact_vs_closed_form"] = check
     del succ, w, root, rank
     torch.cuda.empty_cache()
-    out["c1"] = c1_bench(ctx, args.seed)
+    # random-gather ceiling: dst[i] = src[idx[i]], idx a random permutation of
+    # 2^27, credited 24 B per element (idx 8 + src 8 + dst 8)
+    g_gather, g_ms = ctx.probe_random_gather(1 << 27, 5)
+    out["gather_ceiling"] = {"gbps": g_gather, "ms": g_ms, "random_accesses_per_s": (1 << 27) / (g_ms * 1e-3),
+                             "probe": "u64 gather through a random permutation of 2^27, 24 B/element"}
+    out["record_access_ceilings"] = {
+        pat: {"records_per_s": r, "ms": ms}
+        for pat in ("read", "read_write", "walk_read", "walk_claim")
+        for r, ms in [ctx.probe_record_access(pat, 1 << 27, 3)]}
+    out["roofline"]["frac_of_gather_ceiling"] = achieved / g_gather
+    out["whole_call_frac_of_gather_ceiling"] = whole_gbps / g_gather
+    out["whole_call_frac_of_hbm_peak"] = whole_gbps / HBM_PEAK_GBPS
+    if traffic is not None:
+        out["roofline"]["amplification"] = traffic / alg_bytes
+        out["roofline"]["traffic_gbps"] = traffic / (dom_ms * 1e-3) / 1e9
+    if not args.no_c1:
+        out["c1"] = c1_bench(ctx, args.seed)
     if not args.no_serve:
         out["serve"] = serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots)
     if not args.no_cpu_baseline:

@@ -180,10 +180,12 @@ kc_status kc_root_tree(void *ctx, const uint64_t *parent, uint64_t n, uint64_t *
 kc_status kc_rank_via_euler_tour(void *ctx, const kc_tree_cfg *cfg, uint64_t *succ_array,
                                  int64_t *rank_array, uint64_t n, void *stream);
 /* The tour successor array alone (kascade::compute_euler_tour,
- * eulertour.cpp:379-392), in this library's element numbering: element v < n
- * is the upward edge (v, parent v) or, for a root, the tour's terminal dummy;
- * element n + v is the downward edge (parent v, v).  tour_succ_out has 2n
- * entries, tour_weight_out (may be NULL) the +1/-1/0 weights. */
+ * eulertour.cpp:379-392), in this library's element numbering: element 2v is
+ * the upward edge (v, parent v) or, for a root, the tour's terminal dummy;
+ * element 2v + 1 is the downward edge (parent v, v) (an isolated self loop for
+ * a root).  tour_succ_out has 2n entries, tour_weight_out (may be NULL) the
+ * +1/-1/0 weights.  Children are grouped by a stable sort, so the tour is the
+ * same on every call. */
 kc_status kc_compute_euler_tour(void *ctx, const uint64_t *parent, uint64_t n,
                                 uint64_t *tour_succ_out, int64_t *tour_weight_out, void *stream);
 
@@ -218,6 +220,15 @@ kc_status kc_probe_random_gather_mode(void *ctx, uint64_t n, uint32_t reps, int 
  * function of the table size (which cache level serves a random lookup). */
 kc_status kc_probe_table_gather(void *ctx, uint64_t n, uint64_t m, uint32_t reps,
                                 double *gathers_per_s, double *ms_out);
+/* Access patterns of 16-byte records over one random cycle of n records (n a
+ * power of two, >= 256): 0 independent random reads, 1 independent random
+ * writes, 2 independent read + 16-byte write of the same record, 3 dependent
+ * walks (n/256 walkers of 256 steps) reading only, 4 the same walks writing a
+ * 16-byte claim into every record (the ruler chase's pattern).  Records per
+ * second and the kernel time: the cost of a random write against a random
+ * read. */
+kc_status kc_probe_record_access(void *ctx, uint64_t n, int pattern, uint32_t reps,
+                                 double *records_per_s, double *ms_out);
 
 /* ---- serving ------------------------------------------------------------ */
 typedef enum kc_job_kind { KC_JOB_LIST = 0, KC_JOB_TREE = 1 } kc_job_kind;

@@ -189,8 +189,8 @@ inline void rank_via_euler_tour(EulerTourConfig const& config, std::span<idx_t> 
 }
 
 // kascade::compute_euler_tour (eulertour.hpp:32-34): the tour successor array
-// in this library's numbering (element v = upward edge of v or a root's dummy,
-// element n + v = downward edge into v).
+// in this library's numbering (element 2v = upward edge of v or a root's
+// dummy, element 2v + 1 = downward edge into v).
 inline auto compute_euler_tour(std::span<const idx_t> parent_array, Context& ctx = Context::default_context())
     -> std::vector<idx_t> {
   std::vector<idx_t> tour(2 * parent_array.size());

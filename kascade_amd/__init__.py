@@ -101,6 +101,8 @@ _SIGS = {
                                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "kc_probe_table_gather": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double),
                                               ctypes.POINTER(ctypes.c_double)]),
+    "kc_probe_record_access": (ctypes.c_int, [_P, _U64, ctypes.c_int, ctypes.c_uint32,
+                                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -365,6 +367,17 @@ class Context:
         ms = ctypes.c_double()
         self._check(self._lib.kc_probe_table_gather(self.handle, n, m, reps, ctypes.byref(g), ctypes.byref(ms)))
         return g.value, ms.value
+
+    RECORD_PATTERNS = ("read", "write", "read_write", "walk_read", "walk_claim")
+
+    def probe_record_access(self, pattern, n=1 << 27, reps=5):
+        """Records per second and ms for one access pattern of 16-byte records
+        (see RECORD_PATTERNS) over a random cycle of n records."""
+        r = ctypes.c_double()
+        ms = ctypes.c_double()
+        self._check(self._lib.kc_probe_record_access(self.handle, n, self.RECORD_PATTERNS.index(pattern), reps,
+                                                     ctypes.byref(r), ctypes.byref(ms)))
+        return r.value, ms.value
 
     # ---- tuning ---------------------------------------------------------------
     def tuning_set(self, rows):

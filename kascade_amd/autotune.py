@@ -22,7 +22,7 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KNOBS = {
-    "spacing_top": [8, 16, 32, 64, 128],
+    "spacing_top": [8, 16, 32, 64, 128, 256, 512],
     "spacing_deep": [8, 16, 32],
     "chase_blocks_per_cu": [1, 2, 4, 8],
     "chase_chunk": [64, 256, 1024],

@@ -3,11 +3,12 @@
 //
 // Record layout of a batch with N_l list elements and N_t tree vertices:
 //   [0, N_l)                 list elements, ids = list base + local id
-//   [N_l, N_l + N_t)         upward tour edges / per-root dummies of the trees
-//   [N_l + N_t, N_l + 2N_t)  downward tour edges
-// (the same tour numbering as the single-tree path, shifted by N_l).  One
-// ranking of all N_l + 2N_t records serves every job; outputs are copied back
-// to each job's arrays with local root ids.
+//   N_l + 2t                 upward tour edge / per-root dummy of tree vertex t
+//   N_l + 2t + 1             downward tour edge into tree vertex t
+// (the same interleaved tour numbering as the single-tree path, shifted by
+// N_l; t = tree base + local id).  One ranking of all N_l + 2N_t records
+// serves every job; outputs are copied back to each job's arrays with local
+// root ids.
 #include <algorithm>
 
 #include "batch.hpp"
@@ -87,10 +88,10 @@ batch_tree_keys_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
     vals[t] = static_cast<uint32_t>(t);
     is_root[t] = root;
     if (root) {
-      rec[nl + t] = Rec{nl + t, 0};
-      rec[nl + nt + t] = Rec{nl + nt + t, 0};
+      rec[nl + 2 * t] = Rec{nl + 2 * t, 0};
+      rec[nl + 2 * t + 1] = Rec{nl + 2 * t + 1, 0};
     } else {
-      rec[nl + nt + t] = Rec{nl + t, -1};
+      rec[nl + 2 * t + 1] = Rec{nl + 2 * t, -1};
     }
   }
   wave_append(roots, root);
@@ -107,14 +108,15 @@ batch_tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __rest
   const uint32_t c = vals[k];
   const bool first = (k == 0) || keys[k - 1] != p;
   const bool last = (k + 1 == nt) || keys[k + 1] != p;
-  rec[nl + c] = Rec{last ? nl + p : nl + nt + vals[k + 1], 1};
-  if (first && !is_root[p]) rec[nl + nt + p] = Rec{nl + nt + c, -1};
+  rec[nl + 2 * uint64_t{c}] = Rec{last ? nl + 2 * uint64_t{p} : nl + 2 * uint64_t{vals[k + 1]} + 1, 1};
+  if (first && !is_root[p]) rec[nl + 2 * uint64_t{p} + 1] = Rec{nl + 2 * uint64_t{c} + 1, -1};
 }
 
-// Linked tour build for a batch of trees (child lists by atomic exchange).
+// Linked tour build for a batch of trees (child lists by atomic exchange;
+// sib[t] = the sibling inserted before t).
 __global__ void __launch_bounds__(kThreads)
-batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
-                       uint32_t* __restrict__ head, Rec* __restrict__ rec, uint32_t* err,
+batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt,
+                       uint32_t* __restrict__ head, uint32_t* __restrict__ sib, uint32_t* err,
                        unsigned long long* roots) {
   __shared__ uint32_t s_range[2];
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
@@ -124,25 +126,19 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
   if (t < nt) {
     const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
     const uint64_t i = t - j.base;
-    uint64_t p = j.succ[i];
-    if (p >= j.n) {
-      raise_err(err, kErrRange);
-      p = i;
-    }
-    root = (p == i);
-    if (root) {
-      rec[nl + t] = Rec{nl + t, 0};
-    } else {
-      const uint32_t prev = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
-      rec[nl + t] = Rec{prev != 0xFFFFFFFFu ? nl + nt + prev : nl + j.base + p, 1};
-    }
+    const uint64_t p = j.succ[i];
+    if (p >= j.n) raise_err(err, kErrRange);
+    root = (p == i) || p >= j.n;
+    if (!root) sib[t] = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
   }
   wave_append(roots, root);
 }
 
+// Both tour elements of tree vertex t as one 32-byte pair (see tour_pairs_kernel).
 __global__ void __launch_bounds__(kThreads)
-batch_tree_down_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
-                       const uint32_t* __restrict__ head, Rec* __restrict__ rec) {
+batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
+                        const uint32_t* __restrict__ head, const uint32_t* __restrict__ sib,
+                        Rec* __restrict__ rec) {
   __shared__ uint32_t s_range[2];
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
   block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
@@ -151,12 +147,18 @@ batch_tree_down_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
   const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
   const uint64_t i = t - j.base;
   const uint64_t p = j.succ[i];
+  Rec up, down;
   if (p == i || p >= j.n) {
-    rec[nl + nt + t] = Rec{nl + nt + t, 0};
-    return;
+    up = Rec{nl + 2 * t, 0};
+    down = Rec{nl + 2 * t + 1, 0};
+  } else {
+    const uint32_t s = sib[t];
+    const uint32_t h = head[t];
+    up = Rec{s != 0xFFFFFFFFu ? nl + 2 * uint64_t{s} + 1 : nl + 2 * (j.base + p), 1};
+    down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
   }
-  const uint32_t h = head[t];
-  rec[nl + nt + t] = Rec{h != 0xFFFFFFFFu ? nl + nt + h : nl + t, -1};
+  rec[nl + 2 * t] = up;
+  rec[nl + 2 * t + 1] = down;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
@@ -164,10 +166,12 @@ __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
   if (i < n) p[i] = value;
 }
 
-// Copies outputs back: list element e -> its job, tree element nl + t -> its job.
+// Copies outputs back: list element e -> its job (shift 0, step 1); tree
+// vertex t, whose upward edge is element nl + 2t -> its job (shift nl, step 2).
 __global__ void __launch_bounds__(kThreads)
 batch_scatter_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t count,
-                     uint64_t shift, const uint64_t* __restrict__ root, const int64_t* __restrict__ rank) {
+                     uint64_t shift, uint32_t step, const uint64_t* __restrict__ root,
+                     const int64_t* __restrict__ rank) {
   __shared__ uint32_t s_range[2];
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
   block_job_range(jobs, njobs, first, std::min(first + kThreads, count) - 1, s_range);
@@ -175,8 +179,9 @@ batch_scatter_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t
   if (e >= count) return;
   const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], e)];
   const uint64_t i = e - j.base;
-  j.root_out[i] = root[shift + e] - shift - j.base;
-  j.rank_out[i] = rank[shift + e];
+  const uint64_t x = shift + step * e;
+  j.root_out[i] = (root[x] - shift) / step - j.base;
+  j.rank_out[i] = rank[x];
 }
 
 unsigned bits_for(uint64_t x) {
@@ -209,8 +214,8 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   Scratch scratch(ctx);
   const uint64_t total = nl + 2 * nt;
   Rec* rec = scratch.get<Rec>(total);
-  uint64_t* root = scratch.get<uint64_t>(nl + nt);
-  int64_t* rank = scratch.get<int64_t>(nl + nt);
+  uint64_t* root = scratch.get<uint64_t>(total);
+  int64_t* rank = scratch.get<int64_t>(total);
   BatchJob* d_lists = lists.empty() ? nullptr : scratch.get<BatchJob>(lists.size());
   BatchJob* d_trees = trees.empty() ? nullptr : scratch.get<BatchJob>(trees.size());
   if (d_lists)
@@ -229,12 +234,13 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   if (nt && linked) {
     Phase ph(ctx, "batch_tour", st);
     uint32_t* head = scratch.get<uint32_t>(nt);
+    uint32_t* sib = scratch.get<uint32_t>(nt);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
                        0xFFFFFFFFu);
     hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, rec, ctx->d_err, ctr + 1);
-    hipLaunchKernelGGL(batch_tree_down_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, rec);
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
+    hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec);
   } else if (nt) {
     Phase ph(ctx, "batch_tour", st);
     keys = scratch.get<uint32_t>(nt);
@@ -252,15 +258,15 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   RankTuning tune = tuning_for(ctx, std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
-  rank_records(ctx, rec, total, nl + nt, root, rank, counts[0] + 2 * counts[1], tune, algo, st);
+  rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st);
   check_err(ctx, st);
   Phase ph(ctx, "batch_scatter", st);
   if (nl)
     hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st, d_lists,
-                       static_cast<uint32_t>(lists.size()), nl, uint64_t{0}, root, rank);
+                       static_cast<uint32_t>(lists.size()), nl, uint64_t{0}, 1u, root, rank);
   if (nt)
     hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, d_trees,
-                       static_cast<uint32_t>(trees.size()), nt, nl, root, rank);
+                       static_cast<uint32_t>(trees.size()), nt, nl, 2u, root, rank);
   KC_HIP(hipGetLastError());
 }
 

@@ -187,11 +187,28 @@ bool load_tuning_file(Context* ctx, const std::string& path);
 // Level-0 ranking of separate succ / weight arrays into root / rank arrays.
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint64_t* root, int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st);
-// Ranks packed records (ids < n, consumed) into root (ids) / rank; only the
-// first n_out outputs are written.  max_tails bounds the number of lists.
+// Where a ranking writes its outputs.  Element e produces output only if its
+// low `shift` bits are zero and e >> shift < n_out; it goes to index
+// (e >> shift) * stride, and root ids are reported >> shift.  shift = 0,
+// stride = 1 is the plain layout; stride 2 writes into the fields of an
+// array of records; shift 1 keeps only the even elements of an interleaved
+// Euler tour (element 2v = the upward edge of v).
+struct OutMap {
+  uint32_t stride;
+  uint32_t shift;
+};
+__host__ __device__ inline bool out_selected(const OutMap& om, uint64_t e, uint64_t n_out) {
+  return (e & ((uint64_t{1} << om.shift) - 1)) == 0 && (e >> om.shift) < n_out;
+}
+__host__ __device__ inline uint64_t out_pos(const OutMap& om, uint64_t e) {
+  return (e >> om.shift) * om.stride;
+}
+// Ranks packed records (ids < n, consumed) into root (ids) / rank; outputs as
+// `out_shift` selects (see OutMap; n_out outputs).  max_tails bounds the
+// number of lists.
 void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st);
+                  hipStream_t st, uint32_t out_shift = 0);
 void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
 // Packs succ/weight into records with validation (range, tail weight).

@@ -1,23 +1,26 @@
 // Euler-tour tree rooting (lib/kascade/eulertour.cpp:33-127, :235-278,
 // :337-377) on one device.
 //
-// Element numbering of the tour (2n elements, built without a hash map):
-//   element v      (v non-root): upward edge   (v -> parent v), weight +1
-//   element n + v  (v non-root): downward edge (parent v -> v), weight -1
-//   element r      (r a root)  : the tour's terminal dummy (rank 0, self loop),
+// Element numbering of the tour (2n elements, built without a hash map; the
+// two edges of a vertex share one 32-byte pair, so a leaf's down -> up step
+// stays inside the cache line the down edge was read from):
+//   element 2v     (v non-root): upward edge   (v -> parent v), weight +1
+//   element 2v + 1 (v non-root): downward edge (parent v -> v), weight -1
+//   element 2r     (r a root)  : the tour's terminal dummy (rank 0, self loop),
 //                                the per-root dummy of eulertour.cpp:92-104
-//   element n + r  (r a root)  : unused, an isolated self loop
-// Children are grouped by parent with a stable device radix sort (the CSR of
-// reverse_rooted_tree / make_graph, successor_utils.cpp:236-300).  With the
-// cyclic adjacency order [parent, children...] of each vertex the tour links
-// are (compute_euler_tour_impl, succ[(u_prev, v)] = (v, u_next)):
+//   element 2r + 1 (r a root)  : unused, an isolated self loop
+// Children are grouped by parent either with a stable device radix sort (the
+// CSR of reverse_rooted_tree / make_graph, successor_utils.cpp:236-300) or by
+// atomic exchange into per-parent lists.  With the cyclic adjacency order
+// [parent, children...] of each vertex the tour links are
+// (compute_euler_tour_impl, succ[(u_prev, v)] = (v, u_next)):
 //   down(c)   -> down(first child of c), or up(c) if c is a leaf
 //   up(c)     -> down(next sibling of c), or up(parent) after the last child;
-//                for a root parent that is the root's dummy (element = root)
-// Ranking the tour gives, for element v < n, root = the tree root (the dummy's
-// element id is the root's vertex id) and rank = depth(v): exactly the
-// (root, depth) that map_euler_tour_back writes.  Elements >= n are never
-// written out.
+//                for a root parent that is the root's dummy (element 2 root)
+// Ranking the tour gives, for element 2v, root = 2 x the tree root (the
+// dummy's element id) and rank = depth(v): exactly the (root, depth) that
+// map_euler_tour_back writes; the ranking's output map keeps the even
+// elements and halves the root ids.  Odd elements are never written out.
 #include "common.hpp"
 #include "device.hpp"
 
@@ -42,10 +45,10 @@ tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __re
     keys[v] = is_root ? static_cast<uint32_t>(n) : static_cast<uint32_t>(p);
     vals[v] = static_cast<uint32_t>(v);
     if (is_root) {
-      tour[v] = Rec{v, 0};          // terminal dummy
-      tour[n + v] = Rec{n + v, 0};  // unused slot
+      tour[2 * v] = Rec{2 * v, 0};          // terminal dummy
+      tour[2 * v + 1] = Rec{2 * v + 1, 0};  // unused slot
     } else {
-      tour[n + v] = Rec{v, -1};     // leaf default: down(v) -> up(v)
+      tour[2 * v + 1] = Rec{2 * v, -1};     // leaf default: down(v) -> up(v)
     }
   }
   wave_append(roots, is_root);
@@ -62,51 +65,51 @@ tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__
   const bool first = (j == 0) || keys[j - 1] != p;
   const bool last = (j + 1 == n) || keys[j + 1] != p;
   // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
-  tour[c] = Rec{last ? static_cast<uint64_t>(p) : n + vals[j + 1], 1};
+  tour[2 * uint64_t{c}] = Rec{last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1};
   // down(p) -> down(first child); a root p has no down edge
-  if (first && parent[p] != p) tour[n + p] = Rec{n + c, -1};
+  if (first && parent[p] != p) tour[2 * uint64_t{p} + 1] = Rec{2 * uint64_t{c} + 1, -1};
 }
 
 // Linked build: children lists by atomic exchange on the parent's head slot
-// (one random atomic per vertex; every record write is coalesced).  The child
-// order is whatever order the exchanges happened in; any cyclic order of a
-// vertex's neighbours gives a valid tour and the same (root, depth).
+// (one random atomic per vertex); sib[v] = the sibling inserted before v.  The
+// child order is whatever order the exchanges happened in; any cyclic order of
+// a vertex's neighbours gives a valid tour and the same (root, depth).
 __global__ void __launch_bounds__(kThreads)
 tour_link_atomic_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __restrict__ head,
-                        Rec* __restrict__ tour, uint32_t* err, unsigned long long* roots) {
+                        uint32_t* __restrict__ sib, uint32_t* err, unsigned long long* roots) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool is_root = false;
   if (v < n) {
-    uint64_t p = parent[v];
-    if (p >= n) {
-      raise_err(err, kErrRange);
-      p = v;
-    }
-    is_root = (p == v);
-    if (is_root) {
-      tour[v] = Rec{v, 0};  // terminal dummy
-    } else {
-      const uint32_t prev = atomicExch(&head[p], static_cast<uint32_t>(v));
-      // up(v) -> down(previously inserted sibling), or up(p) / p's dummy
-      tour[v] = Rec{prev != 0xFFFFFFFFu ? n + prev : p, 1};
-    }
+    const uint64_t p = parent[v];
+    if (p >= n) raise_err(err, kErrRange);
+    is_root = (p == v) || p >= n;
+    if (!is_root) sib[v] = atomicExch(&head[p], static_cast<uint32_t>(v));
   }
   wave_append(roots, is_root);
 }
 
-// down(v) -> down(head child of v), or up(v) for a leaf; unused slot for roots.
+// Writes both tour elements of v as one 32-byte pair (coalesced):
+//   up(v)   -> down(sibling inserted before v), or up(p) / p's dummy
+//   down(v) -> down(head child of v), or up(v) for a leaf
+// and for a root the terminal dummy plus the unused slot.
 __global__ void __launch_bounds__(kThreads)
-tour_down_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_t* __restrict__ head,
-                 Rec* __restrict__ tour) {
+tour_pairs_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_t* __restrict__ head,
+                  const uint32_t* __restrict__ sib, Rec* __restrict__ tour) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (v >= n) return;
   const uint64_t p = parent[v];
+  Rec up, down;
   if (p == v || p >= n) {
-    tour[n + v] = Rec{n + v, 0};
-    return;
+    up = Rec{2 * v, 0};
+    down = Rec{2 * v + 1, 0};
+  } else {
+    const uint32_t s = sib[v];
+    const uint32_t h = head[v];
+    up = Rec{s != 0xFFFFFFFFu ? 2 * uint64_t{s} + 1 : 2 * p, 1};
+    down = Rec{h != 0xFFFFFFFFu ? 2 * uint64_t{h} + 1 : 2 * v, -1};
   }
-  const uint32_t h = head[v];
-  tour[n + v] = Rec{h != 0xFFFFFFFFu ? n + h : v, -1};
+  tour[2 * v] = up;
+  tour[2 * v + 1] = down;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
@@ -134,14 +137,15 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
   Scratch scratch(ctx);
   if (!sorted) {
     uint32_t* head = scratch.get<uint32_t>(n);
+    uint32_t* sib = scratch.get<uint32_t>(n);
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, head, n,
                        0xFFFFFFFFu);
     hipLaunchKernelGGL(tour_link_atomic_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
-                       parent, n, head, tour, ctx->d_err,
+                       parent, n, head, sib, ctx->d_err,
                        reinterpret_cast<unsigned long long*>(ctx->d_counters));
-    hipLaunchKernelGGL(tour_down_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
-                       head, tour);
+    hipLaunchKernelGGL(tour_pairs_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
+                       head, sib, tour);
     uint64_t roots = 0;
     read_counters(ctx, st, 0, 1, &roots);
     return roots;
@@ -180,7 +184,7 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
     throw KcError{KC_EINVAL, "invalid algorithm"};
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
   // rank_via_euler_tour_select_algorithm (eulertour.cpp:282-321)
-  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st);
+  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st, /*out_shift=*/1);
   check_err(ctx, st);
 }
 

@@ -43,6 +43,48 @@ gather_nt_kernel(const uint64_t* __restrict__ src, const uint64_t* __restrict__ 
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i < n) dst[i] = __builtin_nontemporal_load(&src[idx[i]]);
 }
+// 16-byte records on one random cycle: rec[idx[k]] = {idx[k + 1], 1}
+__global__ void __launch_bounds__(kThreads)
+cycle_init_kernel(Rec* __restrict__ rec, const uint64_t* __restrict__ idx, uint64_t n) {
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k < n) rec[idx[k]] = Rec{idx[(k + 1) & (n - 1)], 1};
+}
+
+// independent random record accesses at idx[i]: 0 read, 1 write, 2 read + claim write
+__global__ void __launch_bounds__(kThreads)
+record_access_kernel(Rec* __restrict__ rec, const uint64_t* __restrict__ idx, int64_t* __restrict__ out,
+                     uint64_t n, int pattern) {
+  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t p = idx[i];
+  if (pattern == 0) {
+    out[i] = rec[p].w;
+  } else if (pattern == 1) {
+    rec[p] = Rec{idx[(i + 1) & (n - 1)], 1};
+  } else {
+    const Rec v = rec[p];
+    rec[p] = Rec{v.s | kRootFlag, v.w};
+  }
+}
+
+// dependent walks: walker j follows the cycle for L steps from idx[j * L]
+// (the walkers cover the cycle exactly once); claim != 0 writes a 16-byte
+// claim into every visited record, as the ruler chase does
+__global__ void __launch_bounds__(kThreads)
+record_walk_kernel(Rec* __restrict__ rec, const uint64_t* __restrict__ idx, int64_t* __restrict__ out,
+                   uint64_t walkers, uint64_t L, int claim) {
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= walkers) return;
+  uint64_t cur = idx[j * L];
+  int64_t d = 0;
+  for (uint64_t t = 0; t < L; ++t) {
+    const Rec v = rec[cur];
+    if (claim) rec[cur] = Rec{v.s | kRootFlag, v.w};
+    d += v.w;
+    cur = v.s & kIdMask;
+  }
+  out[j] = d;
+}
 }  // namespace
 }  // namespace kc
 
@@ -146,6 +188,60 @@ extern "C" kc_status kc_probe_table_gather(void* vctx, uint64_t n, uint64_t m, u
     const double per = ms / reps;
     if (ms_out) *ms_out = per;
     if (gathers_per_s) *gathers_per_s = static_cast<double>(n) / (per * 1e-3);
+    return KC_OK;
+  } catch (const kc::KcError& e) {
+    ctx->last_error = e.what;
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    return KC_ENOMEM;
+  }
+}
+
+// Access patterns of 16-byte records over a random cycle of n records (n a
+// power of two): 0 independent reads, 1 independent writes, 2 independent
+// read + 16-byte write of the same record, 3 dependent walks (n/256 walkers of
+// 256 steps) reading only, 4 the same walks writing a claim into every record.
+// Reports records per second and the kernel time.
+extern "C" kc_status kc_probe_record_access(void* vctx, uint64_t n, int pattern, uint32_t reps,
+                                            double* records_per_s, double* ms_out) {
+  auto* ctx = static_cast<kc::Context*>(vctx);
+  constexpr uint64_t kWalk = 256;
+  if (!ctx || n < kWalk || (n & (n - 1)) != 0 || reps == 0 || pattern < 0 || pattern > 4)
+    return KC_EINVAL;
+  try {
+    KC_HIP(hipSetDevice(ctx->device));
+    kc::Scratch scratch(ctx);
+    kc::Rec* rec = scratch.get<kc::Rec>(n);
+    uint64_t* idx = scratch.get<uint64_t>(n);
+    int64_t* out = scratch.get<int64_t>(n);
+    hipStream_t st = ctx->stream;
+    const unsigned blocks = kc::blocks_for(n, 256);
+    hipLaunchKernelGGL(kc::perm_idx_kernel, dim3(blocks), dim3(256), 0, st, idx, n, 4242ull);
+    hipLaunchKernelGGL(kc::cycle_init_kernel, dim3(blocks), dim3(256), 0, st, rec, idx, n);
+    const uint64_t walkers = n / kWalk;
+    auto launch = [&] {
+      if (pattern <= 2)
+        hipLaunchKernelGGL(kc::record_access_kernel, dim3(blocks), dim3(256), 0, st, rec, idx, out, n,
+                           pattern);
+      else
+        hipLaunchKernelGGL(kc::record_walk_kernel, dim3(kc::blocks_for(walkers, 256)), dim3(256), 0, st,
+                           rec, idx, out, walkers, kWalk, pattern == 4 ? 1 : 0);
+    };
+    launch();
+    hipEvent_t a, b;
+    KC_HIP(hipEventCreate(&a));
+    KC_HIP(hipEventCreate(&b));
+    KC_HIP(hipEventRecord(a, st));
+    for (uint32_t r = 0; r < reps; r++) launch();
+    KC_HIP(hipEventRecord(b, st));
+    KC_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    KC_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    const double per = ms / reps;
+    if (ms_out) *ms_out = per;
+    if (records_per_s) *records_per_s = static_cast<double>(n) / (per * 1e-3);
     return KC_OK;
   } catch (const kc::KcError& e) {
     ctx->last_error = e.what;

@@ -192,7 +192,7 @@ map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uin
 __global__ void __launch_bounds__(kThreads)
 propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
                  const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
-                 uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride,
+                 uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
                  unsigned long long* u_ctr) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
@@ -205,14 +205,15 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
   if (v < n) {
     r = rec[v];
     if (r.s & kRootFlag) {
-      if (v < n_out) {
+      if (out_selected(om, v, n_out)) {
+        const uint64_t pos = out_pos(om, v);
         if (one_root) {
-          root[v * stride] = root0;
-          rank[v * stride] = T8[r.s & kIdMask] - r.w;
+          root[pos] = root0 >> om.shift;
+          rank[pos] = T8[r.s & kIdMask] - r.w;
         } else {
           const Rec t = T[r.s & kIdMask];
-          root[v * stride] = t.s;
-          rank[v * stride] = t.w - r.w;
+          root[pos] = t.s >> om.shift;
+          rank[pos] = t.w - r.w;
         }
       }
     } else {
@@ -280,15 +281,15 @@ unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__
 __global__ void __launch_bounds__(kThreads)
 unclaimed_fix_kernel(const uint64_t* __restrict__ u_ids, uint64_t nu, uint64_t n_out,
                      const Rec* __restrict__ ures, const Rec* __restrict__ ext,
-                     uint64_t* __restrict__ root, int64_t* __restrict__ rank, uint32_t stride) {
+                     uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om) {
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (k >= nu) return;
   const uint64_t v = u_ids[k];
-  if (v >= n_out) return;
+  if (!out_selected(om, v, n_out)) return;
   const Rec res = ures[k];
   const Rec e = ext[res.s];
-  root[v * stride] = e.s;
-  rank[v * stride] = res.w + e.w;
+  root[out_pos(om, v)] = e.s >> om.shift;
+  rank[out_pos(om, v)] = res.w + e.w;
 }
 
 // ---------------------------------------------------------------------------
@@ -300,7 +301,7 @@ constexpr uint32_t kSmallFlag = 0x80000000u;
 
 __global__ void __launch_bounds__(kSmallThreads)
 small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint64_t* __restrict__ root,
-                  int64_t* __restrict__ rank, uint32_t stride, uint32_t* err) {
+                  int64_t* __restrict__ rank, OutMap om, uint32_t* err) {
   __shared__ uint32_t s_succ[kSmallMax];
   __shared__ int64_t s_rank[kSmallMax];
   constexpr int kPer = kSmallMax / kSmallThreads;
@@ -355,9 +356,10 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
       break;
     }
   }
-  for (uint32_t i = threadIdx.x; i < n_out; i += kSmallThreads) {
-    root[static_cast<uint64_t>(i) * stride] = s_succ[i] & ~kSmallFlag;
-    rank[static_cast<uint64_t>(i) * stride] = s_rank[i];
+  for (uint32_t i = threadIdx.x; i < n; i += kSmallThreads) {
+    if (!out_selected(om, i, n_out)) continue;
+    root[out_pos(om, i)] = (s_succ[i] & ~kSmallFlag) >> om.shift;
+    rank[out_pos(om, i)] = s_rank[i];
   }
 }
 
@@ -408,13 +410,24 @@ pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
 }
 
 __global__ void __launch_bounds__(kThreads)
-pd_write_kernel(const Rec* __restrict__ a, uint64_t n_out, uint64_t* __restrict__ root,
-                int64_t* __restrict__ rank, uint32_t stride, int keep_flag) {
+pd_write_kernel(const Rec* __restrict__ a, uint64_t n, uint64_t n_out, uint64_t* __restrict__ root,
+                int64_t* __restrict__ rank, OutMap om) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (v >= n_out) return;
+  if (v >= n || !out_selected(om, v, n_out)) return;
   const Rec r = a[v];
-  root[v * stride] = keep_flag ? r.s : (r.s & ~kRootFlag);
-  rank[v * stride] = r.w;
+  root[out_pos(om, v)] = (r.s & ~kRootFlag) >> om.shift;
+  rank[out_pos(om, v)] = r.w;
+}
+
+// The doubling state after some rounds, root flags kept (bits.hpp:9-21).
+__global__ void __launch_bounds__(kThreads)
+pd_state_kernel(const Rec* __restrict__ a, uint64_t n, uint64_t* __restrict__ succ,
+                int64_t* __restrict__ rank) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (v >= n) return;
+  const Rec r = a[v];
+  succ[v] = r.s;
+  rank[v] = r.w;
 }
 
 void zero_counters(Context* ctx, hipStream_t st) {
@@ -441,11 +454,11 @@ unsigned log2_floor(uint64_t x) {
 }
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                       int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+                       int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                        int algorithm, int level, hipStream_t st);
 
 void pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                     int64_t* rank, uint32_t stride, hipStream_t st) {
+                     int64_t* rank, OutMap om, hipStream_t st) {
   Phase ph(ctx, "pointer_doubling", st);
   Scratch scratch(ctx);
   Rec* other = scratch.get<Rec>(n);
@@ -470,12 +483,13 @@ void pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_
     }
   }
   check_err(ctx, st);
-  hipLaunchKernelGGL(pd_write_kernel, dim3(blocks_for(n_out, kThreads)), dim3(kThreads), 0, st, a,
-                     n_out, root, rank, stride, 0);
+  const uint64_t n_sel = std::min<uint64_t>(n, n_out << om.shift);
+  hipLaunchKernelGGL(pd_write_kernel, dim3(blocks_for(n_sel, kThreads)), dim3(kThreads), 0, st, a,
+                     n_sel, n_out, root, rank, om);
 }
 
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-               int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+               int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                int level, hipStream_t st) {
   Scratch scratch(ctx);
   const uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
@@ -504,7 +518,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   const uint64_t m = nR + extra;
 
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
-  rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, 2, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
+  rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, OutMap{2, 0}, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
                     level + 1, st);
   int64_t* T8 = scratch.get<int64_t>(m);
   uint64_t* first = scratch.get<uint64_t>(1);
@@ -522,7 +536,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, T, T8, mixed_roots, root, rank, stride, u_ids, u_orig, ucap,
+                       n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
                        ctr + kSlotUnclaimed);
   }
   uint64_t nu = 0;
@@ -545,29 +559,29 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      rec, u_ids, u_orig, nu, T, urec, ext);
-  rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, 2, nu, tune,
+  rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, nu, tune,
                     nu > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1,
                     st);
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     u_ids, nu, n_out, ures, ext, root, rank, stride);
+                     u_ids, nu, n_out, ures, ext, root, rank, om);
 }
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                       int64_t* rank, uint32_t stride, uint64_t max_tails, const RankTuning& tune,
+                       int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                        int algorithm, int level, hipStream_t st) {
   if (n == 0) return;
   if (n <= tune.small_n && n <= kSmallMax) {
     Phase ph(ctx, "small_rank", st);
     hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec,
-                       static_cast<uint32_t>(n), static_cast<uint32_t>(n_out), root, rank, stride,
+                       static_cast<uint32_t>(n), static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om,
                        ctx->d_err);
     return;
   }
   if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
-    pd_records_impl(ctx, rec, n, n_out, root, rank, stride, st);
+    pd_records_impl(ctx, rec, n, n_out, root, rank, om, st);
     return;
   }
-  srs_level(ctx, rec, n, n_out, root, rank, stride, max_tails, tune, level, st);
+  srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st);
 }
 
 }  // namespace
@@ -631,8 +645,9 @@ void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t
 
 void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st) {
-  rank_records_impl(ctx, rec, n, n_out, root, rank, 1, max_tails, tune, algorithm, 0, st);
+                  hipStream_t st, uint32_t out_shift) {
+  rank_records_impl(ctx, rec, n, n_out, root, rank, OutMap{1, out_shift}, max_tails, tune, algorithm, 0,
+                    st);
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
@@ -650,7 +665,7 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
   pack_records(ctx, succ, w, n, rec, st);
   uint64_t tails = 0;
   read_counters(ctx, st, kSlotTails, 1, &tails);
-  rank_records_impl(ctx, rec, n, n, root, rank, 1, tails, tune, algo, 0, st);
+  rank_records_impl(ctx, rec, n, n, root, rank, OutMap{1, 0}, tails, tune, algo, 0, st);
   check_err(ctx, st);
 }
 
@@ -669,8 +684,7 @@ void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
     hipLaunchKernelGGL(pd_round_kernel, dim3(blocks), dim3(kThreads), 0, st, a, b, n, active);
     std::swap(a, b);
   }
-  hipLaunchKernelGGL(pd_write_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, succ_out, rank_out,
-                     1u, 1);
+  hipLaunchKernelGGL(pd_state_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, succ_out, rank_out);
   check_err(ctx, st);
 }
 

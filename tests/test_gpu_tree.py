@@ -85,16 +85,19 @@ def test_tour_structure(ctx):
     ts, tw = ctx.compute_euler_tour(dev(parent))
     ts, tw = host(ts), host(tw)
     roots = np.nonzero(parent == np.arange(n))[0]
-    # ranking the tour with the oracle gives the same depths as the tree
+    # ranking the tour with the oracle gives the same depths as the tree:
+    # element 2v is v's upward edge, whose tail is 2 x v's root
     r, k = o.sequential_walk(ts, tw)
     er, ed = o.euler_tour_rooting(parent)
-    assert (r[:n] == er).all() and (k[:n] == ed).all()
+    assert (r[0::2] == 2 * er).all() and (k[0::2] == ed).all()
     # tour heads: the first down edge of each root with children has rank 0
     indeg = np.bincount(ts[ts != np.arange(2 * n)].astype(np.int64), minlength=2 * n)
     assert indeg.max() <= 1                                  # a list
     for rt in roots:
-        assert ts[rt] == rt and tw[rt] == 0                  # dummy self loop
-        assert ts[n + rt] == n + rt                          # unused slot
+        assert ts[2 * rt] == 2 * rt and tw[2 * rt] == 0      # dummy self loop
+        assert ts[2 * rt + 1] == 2 * rt + 1                  # unused slot
+    nonroot = parent != np.arange(n)
+    assert (tw[0::2][nonroot] == 1).all() and (tw[1::2][nonroot] == -1).all()
     heads = np.nonzero((indeg == 0) & (ts != np.arange(2 * n)))[0]
     assert (k[heads] == 0).all()                             # sum of weights per tour is 0
     assert np.count_nonzero(tw == 1) == n - len(roots)
