@@ -154,10 +154,14 @@ def c1_bench(ctx, seed, reps=20):
 
 def serve_bench(ctx, jobs, seed, slots):
     """Mixed serving stream (60 % lists as C2, 40 % trees as C3, sizes
-    log-uniform in [2^10, 2^24], device-resident arrays): one-job-per-launch
-    capacity, then Poisson arrivals at 0.8 x that capacity through both modes,
-    then the coalescing server at 3 x that offered load.  Every batched
-    result must equal the one-job-per-launch result bit for bit."""
+    log-uniform in [2^10, 2^24], device-resident arrays).
+
+    Baseline: one job per launch, one job at a time (submit, rank, wait, next).
+    Also measured: one job per launch on `slots` concurrent streams, a
+    stronger baseline.  Capacities (all jobs submitted at once), then Poisson
+    arrivals at 0.8 x the serial capacity through every mode, then the
+    coalescing server at 3 x that offered load.  Every batched result must
+    equal the one-job-per-launch result bit for bit."""
     import numpy as np
     import torch
     import kascade_amd
@@ -176,8 +180,8 @@ def serve_bench(ctx, jobs, seed, slots):
             for m in ("one", "batched")}
     torch.cuda.synchronize()
 
-    def run(mode, arrivals):
-        sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one"), num_slots=slots)
+    def run(mode, arrivals, nslots=slots):
+        sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one"), num_slots=nslots)
         # warm the slots' scratch pools
         for k in range(min(8, jobs)):
             kind, s, w = pool[k]
@@ -198,10 +202,12 @@ def serve_bench(ctx, jobs, seed, slots):
         sv.close()
         return st
 
+    cap_serial = run("one", None, 1)["jobs_per_s"]
     cap_one = run("one", None)["jobs_per_s"]
     cap_batched = run("batched", None)["jobs_per_s"]
-    lam = 0.8 * cap_one
+    lam = 0.8 * cap_serial
     arr = np.cumsum(rng.exponential(1.0 / lam, size=jobs))
+    serial = run("one", arr, 1)
     one = run("one", arr)
     batched = run("batched", arr)
     arr3 = arr / 3.0
@@ -212,11 +218,15 @@ def serve_bench(ctx, jobs, seed, slots):
     return {
         "jobs": jobs, "mix": "60% list (int64 weights [1,100]) / 40% tree, n log-uniform in [2^10, 2^24], "
                              "arrays resident in HBM",
-        "capacity_one_job_per_launch_jobs_per_s": cap_one,
+        "capacity_one_job_per_launch_serial_jobs_per_s": cap_serial,
+        "capacity_one_job_per_launch_concurrent_jobs_per_s": cap_one,
+        "concurrent_streams": slots,
         "capacity_batched_jobs_per_s": cap_batched,
-        "capacity_ratio": cap_batched / cap_one,
+        "capacity_ratio": cap_batched / cap_serial,
+        "capacity_ratio_vs_concurrent_one_job": cap_batched / cap_one,
         "offered_load_jobs_per_s": lam,
-        "one_job_per_launch": {k: one[k] for k in keep},
+        "one_job_per_launch_serial": {k: serial[k] for k in keep},
+        "one_job_per_launch_concurrent": {k: one[k] for k in keep},
         "batched": {k: batched[k] for k in keep},
         "batched_at_3x_load": {"offered_load_jobs_per_s": 3 * lam, **{k: batched3[k] for k in keep}},
         "bit_exact_alone_vs_mix": same,

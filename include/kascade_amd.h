@@ -249,8 +249,8 @@ typedef struct kc_job {
 typedef struct kc_server_cfg {
   uint64_t mem_budget_bytes;   /* device-memory admission budget */
   uint32_t max_batch_jobs;     /* jobs coalesced into one segmented launch, 0 = auto */
-  uint32_t num_slots;          /* batches in flight on separate streams, 0 = auto */
-  uint64_t max_batch_elements; /* elements per batch, 0 = auto */
+  uint32_t num_slots;          /* batches in flight on separate streams; 0 = 3, or 1 with one_job_per_launch */
+  uint64_t max_batch_elements; /* elements per batch, 0 = 2^26 */
   uint32_t one_job_per_launch; /* 1: no coalescing (the baseline mode) */
   uint32_t reserved[3];
 } kc_server_cfg;
@@ -264,7 +264,10 @@ typedef struct kc_server_stats {
   double mean_latency_ms;
   double jobs_per_s;       /* completions / (last completion - first submit) */
   double elements_per_s;
-  uint64_t peak_device_bytes;
+  uint64_t peak_device_bytes; /* peak admitted footprint (the budget's accounting) */
+  uint64_t batches_rerun;     /* coalesced passes that failed and were re-run job by job */
+  uint64_t jobs_failed;       /* jobs that completed with an error status */
+  uint64_t scratch_bytes;     /* device memory the slots' scratch pools hold now */
 } kc_server_stats;
 
 kc_status kc_server_create(void *ctx, const kc_server_cfg *cfg, void **server_out);

@@ -59,7 +59,8 @@ class ServerStats(ctypes.Structure):
                 ("batches_launched", ctypes.c_uint64), ("p50_latency_ms", ctypes.c_double),
                 ("p99_latency_ms", ctypes.c_double), ("mean_latency_ms", ctypes.c_double),
                 ("jobs_per_s", ctypes.c_double), ("elements_per_s", ctypes.c_double),
-                ("peak_device_bytes", ctypes.c_uint64)]
+                ("peak_device_bytes", ctypes.c_uint64), ("batches_rerun", ctypes.c_uint64),
+                ("jobs_failed", ctypes.c_uint64), ("scratch_bytes", ctypes.c_uint64)]
 
 
 def lib_path():

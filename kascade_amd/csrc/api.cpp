@@ -41,6 +41,7 @@ void* Arena::alloc(size_t bytes) {
       return nullptr;
     }
     chunks_.push_back(Chunk{static_cast<char*>(p), want});
+    held_.fetch_add(want, std::memory_order_relaxed);
   }
   Block nb{chunk, offset, bytes};
   if (top_ < blocks_.size()) blocks_[top_] = nb; else blocks_.push_back(nb);
@@ -55,13 +56,10 @@ void Arena::trim() {
   for (auto& c : chunks_) (void)hipFree(c.base);
   chunks_.clear();
   blocks_.clear();
+  held_.store(0, std::memory_order_relaxed);
 }
 
-uint64_t Arena::held() const {
-  uint64_t s = 0;
-  for (auto& c : chunks_) s += c.size;
-  return s;
-}
+uint64_t Arena::held() const { return held_.load(std::memory_order_relaxed); }
 
 // -------------------------------------------------------------- Profiler ---
 hipEvent_t Profiler::get() {

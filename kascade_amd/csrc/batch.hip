@@ -255,7 +255,9 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   }
   uint64_t counts[2] = {0, 0};
   read_counters(ctx, st, 0, 2, counts);
-  RankTuning tune = tuning_for(ctx, std::max<uint64_t>(1, total / std::max<size_t>(1, lists.size() + trees.size())), cfg);
+  // size class of the whole batch; the ruler spacing per level is capped by
+  // the mean list length inside the ranking
+  RankTuning tune = tuning_for(ctx, total, cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
   rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st);

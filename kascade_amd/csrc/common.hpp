@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -55,6 +56,7 @@ class Arena {
   std::vector<Chunk> chunks_;
   std::vector<Block> blocks_;
   size_t top_ = 0;  // number of live blocks
+  std::atomic<uint64_t> held_{0};  // readable from other threads (server statistics)
 };
 
 // Per-phase device timing with HIP events on the launching stream (the
@@ -168,6 +170,11 @@ struct RankTuning {
   uint32_t small_n;       // sizes at or below this go to the one-workgroup solver
   uint32_t chase_blocks_per_cu;
   uint32_t chase_chunk;  // rulers a wave takes per work-counter atomic
+  // spacings set by the caller are used as given; otherwise each level caps
+  // its spacing so that the lists' head prefixes (about K elements per list,
+  // ranked as a sub-problem) stay within an eighth of the level
+  bool spacing_top_fixed;
+  bool spacing_deep_fixed;
 };
 // Per-size-class tuning table row: log2(n) class and the knobs picked for it
 // (written by kascade_amd/autotune.py into tuning_table.txt, which contexts

@@ -492,7 +492,14 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                int level, hipStream_t st) {
   Scratch scratch(ctx);
-  const uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
+  uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
+  if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
+    // a forest of max_tails lists: keep K <= (mean list length) / 8
+    const uint64_t mean = n / std::max<uint64_t>(1, max_tails);
+    uint32_t cap = 2;
+    while (cap < spacing && uint64_t{cap} * 16 <= mean) cap *= 2;
+    spacing = std::min(spacing, cap);
+  }
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t cap = nR + std::min<uint64_t>(max_tails, n);
@@ -559,7 +566,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      rec, u_ids, u_orig, nu, T, urec, ext);
-  rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, nu, tune,
+  // every unclaimed prefix belongs to a different list of this level
+  rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, std::min(nu, max_tails), tune,
                     nu > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1,
                     st);
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
@@ -623,11 +631,13 @@ RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
       if (!pow2(cfg->ruler_spacing) || cfg->ruler_spacing > (1u << 20))
         throw KcError{KC_EINVAL, "ruler_spacing must be a power of two in [2, 2^20]"};
       t.spacing_top = cfg->ruler_spacing;
+      t.spacing_top_fixed = true;
     }
     if (cfg->ruler_spacing_deep) {
       if (!pow2(cfg->ruler_spacing_deep) || cfg->ruler_spacing_deep > (1u << 20))
         throw KcError{KC_EINVAL, "ruler_spacing_deep must be a power of two in [2, 2^20]"};
       t.spacing_deep = cfg->ruler_spacing_deep;
+      t.spacing_deep_fixed = true;
     }
     if (cfg->max_levels) t.max_levels = cfg->max_levels;
   }

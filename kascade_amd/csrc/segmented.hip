@@ -103,8 +103,9 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   hipLaunchKernelGGL(seg_check_kernel, dim3(blocks_for(nseg, kThreads)), dim3(kThreads), 0, st,
                      seg_off, nseg, ctx->d_err);
   check_err(ctx, st);
-  // size class = mean segment size
-  RankTuning tune = tuning_for(ctx, std::max<uint64_t>(1, total / nseg), cfg);
+  // size class of the whole forest; the ruler spacing per level is capped by
+  // the mean list length inside the ranking
+  RankTuning tune = tuning_for(ctx, total, cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};

@@ -65,6 +65,8 @@ struct Server {
   std::vector<double> latencies_ms;
   uint64_t elements_done = 0;
   uint64_t batches = 0;
+  uint64_t batches_rerun = 0;
+  uint64_t jobs_failed = 0;
   bool have_first = false;
   Clock::time_point first_submit;
   Clock::time_point last_finish;
@@ -145,7 +147,8 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
   } catch (const std::bad_alloc&) {
     status = KC_ENOMEM;
   }
-  if (status != KC_OK) {
+  const bool rerun = status != KC_OK;
+  if (rerun) {
     // a bad job fails the coalesced pass: rerun the jobs one by one so that
     // only the offending job reports the error
     for (size_t k = 0; k < batch.size(); k++) per_job[k] = run_alone(ctx, batch[k]->job);
@@ -162,9 +165,12 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
     if (status == KC_OK) {
       sv->latencies_ms.push_back(std::chrono::duration<double, std::milli>(now - t->submitted).count());
       sv->elements_done += t->job.n;
+    } else {
+      sv->jobs_failed++;
     }
   }
   sv->batches++;
+  if (rerun) sv->batches_rerun++;
   sv->last_finish = now;
   sv->cv_done.notify_all();
   sv->cv_work.notify_all();
@@ -172,7 +178,9 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
 
 void worker(Server* sv, Slot* slot) {
   const uint64_t max_jobs = sv->cfg.one_job_per_launch ? 1 : (sv->cfg.max_batch_jobs ? sv->cfg.max_batch_jobs : 4096);
-  const uint64_t max_elems = sv->cfg.max_batch_elements ? sv->cfg.max_batch_elements : (uint64_t{1} << 25);
+  // 2^26 elements per batch: large enough for the ranking to run at its
+  // large-list rate, small enough that several batches overlap
+  const uint64_t max_elems = sv->cfg.max_batch_elements ? sv->cfg.max_batch_elements : (uint64_t{1} << 26);
   for (;;) {
     std::vector<Ticket*> batch;
     {
@@ -216,7 +224,7 @@ kc_status kc_server_create(void* vctx, const kc_server_cfg* cfg, void** server_o
   sv->device = ctx->device;
   sv->cfg = *cfg;
   if (sv->cfg.mem_budget_bytes == 0) sv->cfg.mem_budget_bytes = uint64_t{200} << 30;
-  uint32_t nslots = sv->cfg.one_job_per_launch ? 1 : (sv->cfg.num_slots ? sv->cfg.num_slots : 3);
+  uint32_t nslots = sv->cfg.num_slots ? sv->cfg.num_slots : (sv->cfg.one_job_per_launch ? 1 : 3);
   for (uint32_t i = 0; i < nslots; i++) {
     auto slot = std::make_unique<Slot>();
     if (kc_context_create(ctx->device, 0, &slot->ctx) != KC_OK) {
@@ -306,6 +314,9 @@ kc_status kc_server_stats_get(void* vsv, kc_server_stats* out) {
   s.elements_completed = sv->elements_done;
   s.batches_launched = sv->batches;
   s.peak_device_bytes = sv->peak_bytes;
+  s.batches_rerun = sv->batches_rerun;
+  s.jobs_failed = sv->jobs_failed;
+  for (auto& slot : sv->slots) s.scratch_bytes += kc_context_scratch_bytes(slot->ctx);
   if (!sv->latencies_ms.empty()) {
     std::vector<double> v = sv->latencies_ms;
     std::sort(v.begin(), v.end());
@@ -335,6 +346,8 @@ kc_status kc_server_reset_stats(void* vsv) {
   sv->latencies_ms.clear();
   sv->elements_done = 0;
   sv->batches = 0;
+  sv->batches_rerun = 0;
+  sv->jobs_failed = 0;
   sv->have_first = false;
   sv->peak_bytes = sv->bytes_in_use;
   return KC_OK;
