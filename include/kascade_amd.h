@@ -189,6 +189,57 @@ kc_status kc_rank_via_euler_tour(void *ctx, const kc_tree_cfg *cfg, uint64_t *su
 kc_status kc_compute_euler_tour(void *ctx, const uint64_t *parent, uint64_t n,
                                 uint64_t *tour_succ_out, int64_t *tour_weight_out, void *stream);
 
+/* ---- list utilities and statistics --------------------------------------- */
+/* *is_list_out = 1 iff no element has two predecessors (self loops ignored).
+ * Replaces kascade::is_list (successor_utils.cpp:23-63).  KC_EINVAL for ids
+ * >= n. */
+kc_status kc_is_list(void *ctx, const uint64_t *succ, uint64_t n, int32_t *is_list_out, void *stream);
+/* is_leaf_out[v] = 1 iff v has no predecessor (may be NULL), and the count.
+ * Replaces kascade::LeafInfo (successor_utils.cpp:72-123): as there, a root
+ * without predecessor counts as a leaf. */
+kc_status kc_leaf_info(void *ctx, const uint64_t *succ, uint64_t n, uint8_t *is_leaf_out,
+                       uint64_t *num_leaves_out, void *stream);
+/* pred_out[succ[u]] = u and dist_to_pred_out[succ[u]] = dist_to_succ[u] (1 when
+ * dist_to_succ is NULL); elements without predecessor get (self, 0).  The
+ * original roots (succ[i] == i) are written in ascending order to roots_out
+ * (up to roots_capacity; may be NULL) and counted in *num_roots_out.  Replaces
+ * kascade::reverse_list (successor_utils.cpp:159-232).  KC_EINVAL if an
+ * element has two predecessors. */
+kc_status kc_reverse_list(void *ctx, const uint64_t *succ, const int64_t *dist_to_succ, uint64_t n,
+                          uint64_t *pred_out, int64_t *dist_to_pred_out, uint64_t *roots_out,
+                          uint64_t roots_capacity, uint64_t *num_roots_out, void *stream);
+
+/* Statistics levels, as StatsLevel (bench/detail/stats.hpp:9-15). */
+typedef enum kc_stats_level {
+  KC_STATS_NONE = 1,
+  KC_STATS_BASIC = 2,
+  KC_STATS_REDUCED_EXTENSIVE = 3, /* aggregates without the per-tree list */
+  KC_STATS_EXTENSIVE = 4
+} kc_stats_level;
+
+/* BasicStats and ExtensiveStats (bench/detail/stats.hpp:17-39). */
+typedef struct kc_stats {
+  uint64_t num_trees;
+  int64_t max_rank;
+  double avg_rank;
+  uint64_t num_nontrivial_trees; /* trees of more than one element */
+  uint64_t nontrivial_size_sum;
+  int64_t nontrivial_rank_sum;
+  uint64_t max_size;
+  double avg_size;
+  double nontrivial_avg_size;
+  double nontrivial_avg_rank;
+} kc_stats;
+
+/* compute_stats (bench/detail/stats.cpp:147-162) over a ranking's outputs.
+ * With KC_STATS_EXTENSIVE the nontrivial trees' (root, size, max rank, rank
+ * sum) are written in ascending root order, up to tree_capacity entries (the
+ * arrays may be NULL); their number is out->num_nontrivial_trees. */
+kc_status kc_compute_stats(void *ctx, int32_t level, const uint64_t *root, const int64_t *rank, uint64_t n,
+                           kc_stats *out, uint64_t *tree_root_out, uint64_t *tree_size_out,
+                           int64_t *tree_max_rank_out, int64_t *tree_rank_sum_out, uint64_t tree_capacity,
+                           void *stream);
+
 /* ---- synthetic inputs (seeded; identical to the CPU generators) ---------- */
 /* Random list: order pi = seeded Feistel permutation, succ[pi(i)] = pi(i+1);
  * weighted: weights uniform in [1,100], 0 on the tail; else 1 / 0.

@@ -16,6 +16,7 @@
 
 #include <any>
 #include <cstdint>
+#include <optional>
 #include <span>
 #include <stdexcept>
 #include <string>
@@ -186,6 +187,107 @@ inline void rank_via_euler_tour(EulerTourConfig const& config, std::span<idx_t> 
   t.tour_build = config.reproducible_tour ? KC_TOUR_SORTED : KC_TOUR_LINKED;
   ctx.check(kc_rank_via_euler_tour(ctx.handle(), &t, succ_array.data(), rank_array.data(), succ_array.size(),
                                    nullptr));
+}
+
+// kascade::is_list (successor_utils.hpp): no element has two predecessors.
+inline auto is_list(std::span<const idx_t> succ_array, Context& ctx = Context::default_context()) -> bool {
+  std::int32_t out = 0;
+  ctx.check(kc_is_list(ctx.handle(), succ_array.data(), succ_array.size(), &out, nullptr));
+  return out != 0;
+}
+
+// kascade::LeafInfo (successor_utils.cpp:72-123): is_leaf(v) = no predecessor
+// (a root without predecessor counts as a leaf).
+class LeafInfo {
+ public:
+  explicit LeafInfo(std::span<const idx_t> succ_array, Context& ctx = Context::default_context())
+      : is_leaf_(succ_array.size()) {
+    ctx.check(kc_leaf_info(ctx.handle(), succ_array.data(), succ_array.size(), is_leaf_.data(), &num_leaves_,
+                           nullptr));
+  }
+  auto is_leaf(idx_t idx) const -> bool { return is_leaf_[idx] != 0; }
+  auto num_local_leaves() const -> std::size_t { return num_leaves_; }
+
+ private:
+  std::vector<std::uint8_t> is_leaf_;
+  std::uint64_t num_leaves_ = 0;
+};
+
+// kascade::reverse_list (successor_utils.cpp:159-232): fills pred_array and
+// dist_to_pred; returns the original roots (now leaves) in ascending order.
+inline auto reverse_list(std::span<const idx_t> succ_array, std::span<const rank_t> dist_to_succ,
+                         std::span<idx_t> pred_array, std::span<rank_t> dist_to_pred,
+                         Context& ctx = Context::default_context()) -> std::vector<idx_t> {
+  const std::size_t n = succ_array.size();
+  if (pred_array.size() < n || dist_to_pred.size() < n || (!dist_to_succ.empty() && dist_to_succ.size() < n))
+    throw Error(KC_EINVAL, "array sizes do not match");
+  std::vector<idx_t> roots(n);
+  std::uint64_t nroots = 0;
+  ctx.check(kc_reverse_list(ctx.handle(), succ_array.data(), dist_to_succ.empty() ? nullptr : dist_to_succ.data(),
+                            n, pred_array.data(), dist_to_pred.data(), roots.data(), n, &nroots, nullptr));
+  roots.resize(nroots);
+  return roots;
+}
+
+// compute_stats (bench/detail/stats.cpp:147-162) with the reference's
+// StatsLevel and result shapes (bench/detail/stats.hpp:9-44).
+enum class StatsLevel : std::uint8_t { invalid, none, basic, reduced_extensive, extensive };
+struct BasicStats {
+  std::size_t num_trees = 0;
+  std::int64_t max_rank = 0;
+  double avg_rank = 0.;
+};
+struct TreeStats {
+  idx_t root = 0;
+  std::size_t size = 0;
+  std::int64_t max_rank = 0;
+  std::int64_t rank_sum = 0;
+};
+struct ExtensiveStats {
+  std::size_t num_trees = 0;
+  std::size_t num_nontrivial_trees = 0;
+  std::size_t nontrivial_size_sum = 0;
+  std::int64_t nontrivial_rank_sum = 0;
+  std::size_t max_size = 0;
+  double avg_size = 0.;
+  double nontrivial_avg_size = 0.;
+  double nontrivial_avg_rank = 0.;
+  std::optional<std::vector<TreeStats>> per_tree_stats;  // ascending root order
+};
+struct Stats {
+  std::optional<BasicStats> basic_stats;
+  std::optional<ExtensiveStats> extensive_stats;
+};
+
+inline auto compute_stats(StatsLevel level, std::span<const idx_t> root_array, std::span<const rank_t> rank_array,
+                          Context& ctx = Context::default_context()) -> Stats {
+  if (level == StatsLevel::invalid) throw Error(KC_EINVAL, "invalid statistics level");
+  if (level == StatsLevel::none) return Stats{};
+  const std::size_t n = root_array.size();
+  kc_stats st{};
+  std::vector<idx_t> tr;
+  std::vector<std::uint64_t> ts;
+  std::vector<std::int64_t> tm, tsum;
+  if (level == StatsLevel::extensive) {
+    tr.resize(n);
+    ts.resize(n);
+    tm.resize(n);
+    tsum.resize(n);
+  }
+  ctx.check(kc_compute_stats(ctx.handle(), static_cast<std::int32_t>(level), root_array.data(), rank_array.data(),
+                             n, &st, tr.data(), ts.data(), tm.data(), tsum.data(), tr.size(), nullptr));
+  Stats out;
+  out.basic_stats = BasicStats{st.num_trees, st.max_rank, st.avg_rank};
+  if (level == StatsLevel::basic) return out;
+  ExtensiveStats e{st.num_trees, st.num_nontrivial_trees, st.nontrivial_size_sum, st.nontrivial_rank_sum,
+                   st.max_size, st.avg_size, st.nontrivial_avg_size, st.nontrivial_avg_rank, std::nullopt};
+  if (level == StatsLevel::extensive) {
+    std::vector<TreeStats> per(st.num_nontrivial_trees);
+    for (std::size_t i = 0; i < per.size(); i++) per[i] = TreeStats{tr[i], ts[i], tm[i], tsum[i]};
+    e.per_tree_stats = std::move(per);
+  }
+  out.extensive_stats = std::move(e);
+  return out;
 }
 
 // kascade::compute_euler_tour (eulertour.hpp:32-34): the tour successor array

@@ -54,6 +54,18 @@ class ServerCfg(ctypes.Structure):
                 ("one_job_per_launch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
+class Stats(ctypes.Structure):
+    _fields_ = [("num_trees", ctypes.c_uint64), ("max_rank", ctypes.c_int64), ("avg_rank", ctypes.c_double),
+                ("num_nontrivial_trees", ctypes.c_uint64), ("nontrivial_size_sum", ctypes.c_uint64),
+                ("nontrivial_rank_sum", ctypes.c_int64), ("max_size", ctypes.c_uint64),
+                ("avg_size", ctypes.c_double), ("nontrivial_avg_size", ctypes.c_double),
+                ("nontrivial_avg_rank", ctypes.c_double)]
+
+
+# StatsLevel names (bench/detail/serialization.hpp) -> kc_stats_level
+STATS_LEVELS = {"none": 1, "basic": 2, "extensive-reduced-output": 3, "extensive": 4}
+
+
 class ServerStats(ctypes.Structure):
     _fields_ = [("jobs_completed", ctypes.c_uint64), ("elements_completed", ctypes.c_uint64),
                 ("batches_launched", ctypes.c_uint64), ("p50_latency_ms", ctypes.c_double),
@@ -104,6 +116,11 @@ _SIGS = {
                                               ctypes.POINTER(ctypes.c_double)]),
     "kc_probe_record_access": (ctypes.c_int, [_P, _U64, ctypes.c_int, ctypes.c_uint32,
                                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+    "kc_is_list": (ctypes.c_int, [_P, _P, _U64, ctypes.POINTER(ctypes.c_int32), _P]),
+    "kc_leaf_info": (ctypes.c_int, [_P, _P, _U64, _P, ctypes.POINTER(_U64), _P]),
+    "kc_reverse_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, _P, _U64, ctypes.POINTER(_U64), _P]),
+    "kc_compute_stats": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _U64, ctypes.POINTER(Stats), _P, _P, _P, _P,
+                                        _U64, _P]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -179,7 +196,7 @@ def _like(a, dtype, n=None):
     if isinstance(a, np.ndarray):
         return np.empty(n, dtype)
     import torch
-    tmap = {np.uint64: torch.uint64, np.int64: torch.int64, np.uint32: torch.uint32}
+    tmap = {np.uint64: torch.uint64, np.int64: torch.int64, np.uint32: torch.uint32, np.uint8: torch.uint8}
     return torch.empty(n, dtype=tmap[np.dtype(dtype).type], device=a.device)
 
 
@@ -248,6 +265,61 @@ class Context:
                                                            _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
                                                            stream))
         return root_out, rank_out
+
+    # ---- successor utilities and statistics -------------------------------------
+    def is_list(self, succ, stream=None):
+        """kascade::is_list: no element has two predecessors."""
+        out = ctypes.c_int32()
+        self._check(self._lib.kc_is_list(self.handle, _ptr(succ, np.uint64), _numel(succ), ctypes.byref(out), stream))
+        return bool(out.value)
+
+    def leaf_info(self, succ, stream=None):
+        """kascade::LeafInfo: (is_leaf flags as uint8, number of leaves)."""
+        flags = _like(succ, np.uint8)
+        cnt = ctypes.c_uint64()
+        self._check(self._lib.kc_leaf_info(self.handle, _ptr(succ, np.uint64), _numel(succ), _ptr(flags, np.uint8),
+                                           ctypes.byref(cnt), stream))
+        return flags, cnt.value
+
+    def reverse_list(self, succ, dist_to_succ=None, stream=None):
+        """kascade::reverse_list: (pred, dist_to_pred, original roots ascending)."""
+        n = _numel(succ)
+        pred = _like(succ, np.uint64)
+        dpred = _like(succ, np.int64)
+        roots = _like(succ, np.uint64)
+        nr = ctypes.c_uint64()
+        self._check(self._lib.kc_reverse_list(self.handle, _ptr(succ, np.uint64), _ptr(dist_to_succ, np.int64), n,
+                                              _ptr(pred, np.uint64), _ptr(dpred, np.int64), _ptr(roots, np.uint64),
+                                              n, ctypes.byref(nr), stream))
+        return pred, dpred, roots[:nr.value]
+
+    def compute_stats(self, root, rank, level="basic", stream=None):
+        """compute_stats (bench/detail/stats.cpp): a dict shaped like the
+        reference's serialized Stats, plus the per-tree list for "extensive"."""
+        lv = STATS_LEVELS[level] if isinstance(level, str) else int(level)
+        n = _numel(root)
+        st = Stats()
+        cap = n if lv == 4 else 0
+        tr = ts = tm = tsum = None
+        if cap:
+            tr, ts = np.empty(cap, np.uint64), np.empty(cap, np.uint64)
+            tm, tsum = np.empty(cap, np.int64), np.empty(cap, np.int64)
+        self._check(self._lib.kc_compute_stats(self.handle, lv, _ptr(root, np.uint64), _ptr(rank, np.int64), n,
+                                               ctypes.byref(st), _ptr(tr, np.uint64), _ptr(ts, np.uint64),
+                                               _ptr(tm, np.int64), _ptr(tsum, np.int64), cap, stream))
+        out = {}
+        if lv >= 2:
+            out["basic_stats"] = {"num_trees": st.num_trees, "max_rank": st.max_rank, "avg_rank": st.avg_rank}
+        if lv >= 3:
+            ext = {k: getattr(st, k) for k in ("num_trees", "num_nontrivial_trees", "nontrivial_size_sum",
+                                               "nontrivial_rank_sum", "max_size", "avg_size",
+                                               "nontrivial_avg_size", "nontrivial_avg_rank")}
+            if lv == 4:
+                k = st.num_nontrivial_trees
+                ext["per_tree_stats"] = [{"root": int(tr[i]), "size": int(ts[i]), "max_rank": int(tm[i]),
+                                          "rank_sum": int(tsum[i])} for i in range(k)]
+            out["extensive_stats"] = ext
+        return out
 
     def pd_rounds(self, succ, weight, rounds, stream=None):
         so = _like(succ, np.uint64)

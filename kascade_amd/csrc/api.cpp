@@ -521,6 +521,80 @@ kc_status kc_set_initial_ranking_state(void* vctx, const uint64_t* succ, uint64_
   });
 }
 
+kc_status kc_is_list(void* vctx, const uint64_t* succ, uint64_t n, int32_t* is_list_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (!is_list_out || (n && !succ)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    *is_list_out = kc::is_list_device(ctx, s.dev(), n, st) ? 1 : 0;
+  });
+}
+
+kc_status kc_leaf_info(void* vctx, const uint64_t* succ, uint64_t n, uint8_t* is_leaf_out,
+                       uint64_t* num_leaves_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && !succ) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<uint8_t> lf(ctx, scratch, is_leaf_out, n, false, true, st);
+    const uint64_t count = kc::leaf_info_device(ctx, s.dev(), n, lf.dev(), st);
+    lf.finish();
+    KC_HIP(hipStreamSynchronize(st));
+    if (num_leaves_out) *num_leaves_out = count;
+  });
+}
+
+kc_status kc_reverse_list(void* vctx, const uint64_t* succ, const int64_t* dist_to_succ, uint64_t n,
+                          uint64_t* pred_out, int64_t* dist_to_pred_out, uint64_t* roots_out,
+                          uint64_t roots_capacity, uint64_t* num_roots_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!succ || !pred_out || !dist_to_pred_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, succ, n, true, false, st);
+    Staged<const int64_t> w(ctx, scratch, dist_to_succ, n, true, false, st);
+    Staged<uint64_t> p(ctx, scratch, pred_out, n, false, true, st);
+    Staged<int64_t> d(ctx, scratch, dist_to_pred_out, n, false, true, st);
+    const uint64_t cap = roots_out ? roots_capacity : 0;
+    Staged<uint64_t> r(ctx, scratch, roots_out, std::min(cap, n), false, true, st);
+    const uint64_t nroots =
+        kc::reverse_list_device(ctx, s.dev(), w.dev(), n, p.dev(), d.dev(), r.dev(), cap, st);
+    p.finish();
+    d.finish();
+    r.finish();
+    KC_HIP(hipStreamSynchronize(st));
+    if (num_roots_out) *num_roots_out = nroots;
+  });
+}
+
+kc_status kc_compute_stats(void* vctx, int32_t level, const uint64_t* root, const int64_t* rank, uint64_t n,
+                           kc_stats* out, uint64_t* tree_root_out, uint64_t* tree_size_out,
+                           int64_t* tree_max_rank_out, int64_t* tree_rank_sum_out, uint64_t tree_capacity,
+                           void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (!out || (n && (!root || !rank))) throw KcError{KC_EINVAL, "null array"};
+    if (level < KC_STATS_NONE || level > KC_STATS_EXTENSIVE) throw KcError{KC_EINVAL, "invalid statistics level"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> ro(ctx, scratch, root, n, true, false, st);
+    Staged<const int64_t> ra(ctx, scratch, rank, n, true, false, st);
+    const uint64_t cap = std::min(tree_capacity, n);
+    Staged<uint64_t> tr(ctx, scratch, tree_root_out, cap, false, true, st);
+    Staged<uint64_t> ts(ctx, scratch, tree_size_out, cap, false, true, st);
+    Staged<int64_t> tm(ctx, scratch, tree_max_rank_out, cap, false, true, st);
+    Staged<int64_t> tsum(ctx, scratch, tree_rank_sum_out, cap, false, true, st);
+    kc::compute_stats_device(ctx, ro.dev(), ra.dev(), n, level, out, tr.dev(), ts.dev(), tm.dev(), tsum.dev(),
+                             cap, st);
+    tr.finish();
+    ts.finish();
+    tm.finish();
+    tsum.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
 kc_status kc_pack(void* vctx, const uint64_t* succ, const int64_t* rank, uint64_t n,
                   const uint64_t* active, uint64_t nactive, uint64_t* psucc_out,
                   int64_t* prank_out, void* stream) {

@@ -222,6 +222,16 @@ void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
 void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, Rec* rec,
                   hipStream_t st);
 void validate_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
+// successor utilities and statistics (listutil.hip)
+bool is_list_device(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
+uint64_t leaf_info_device(Context* ctx, const uint64_t* succ, uint64_t n, uint8_t* is_leaf,
+                          hipStream_t st);
+uint64_t reverse_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
+                             uint64_t* pred, int64_t* dpred, uint64_t* roots, uint64_t roots_cap,
+                             hipStream_t st);
+void compute_stats_device(Context* ctx, const uint64_t* root, const int64_t* rank, uint64_t n, int level,
+                          kc_stats* out, uint64_t* tree_root, uint64_t* tree_size, int64_t* tree_max,
+                          int64_t* tree_sum, uint64_t tree_cap, hipStream_t st);
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
                       int64_t* depth, const kc_tree_cfg* cfg, hipStream_t st);
 void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,

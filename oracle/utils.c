@@ -1,0 +1,156 @@
+/*
+ * utils.c -- CPU restatement of KAscade's successor utilities and of the
+ * benchmark statistics.  TEST INFRASTRUCTURE ONLY: used by tests/ as the
+ * checker; never linked into, loaded by or called from the kascade_amd
+ * product library.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the niklas-uhl/kascade source tree); tests/test_oracle.py pins each one.
+ * One address space stands for all MPI ranks: the request / reply exchanges of
+ * the reference become direct array accesses, which changes no result.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* is_list, lib/kascade/successor_utils.cpp:31-63: every element that is not a
+ * root "requests" its successor; a successor requested twice has two
+ * predecessors.  Returns 1 (a list / forest of lists), 0, or -1 for an id >= n. */
+int orc_is_list(const uint64_t *succ, uint64_t n) {
+  uint8_t *seen = (uint8_t *)calloc(n ? n : 1, 1);
+  int ok = 1;
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t s = succ[i];
+    if (s >= n) { ok = -1; break; }
+    if (s == i) continue; /* :41-43 */
+    if (seen[s]) ok = 0;  /* :52-58 duplicate request */
+    seen[s] = 1;
+  }
+  free(seen);
+  return ok;
+}
+
+/* LeafInfo, successor_utils.cpp:72-123: has_pred[succ] for every non-root;
+ * is_leaf(v) = !has_pred[v] (:114-117), so a root without predecessor is a
+ * leaf too; num_local_leaves counts them.  Returns the count, -1 on a bad id. */
+int64_t orc_leaf_info(const uint64_t *succ, uint64_t n, uint8_t *is_leaf) {
+  uint8_t *has_pred = (uint8_t *)calloc(n ? n : 1, 1);
+  for (uint64_t i = 0; i < n; i++) {
+    if (succ[i] >= n) { free(has_pred); return -1; }
+    if (succ[i] != i) has_pred[succ[i]] = 1;
+  }
+  int64_t leaves = 0;
+  for (uint64_t v = 0; v < n; v++) {
+    is_leaf[v] = !has_pred[v];
+    leaves += !has_pred[v];
+  }
+  free(has_pred);
+  return leaves;
+}
+
+/* reverse_list, successor_utils.cpp:159-232: every node starts as its own
+ * predecessor with distance 0 (:216-218); each non-root u sends (pred = u,
+ * succ, dist_to_succ[u]) to its successor, which stores it (:219-230).  The
+ * original roots are returned in index order (:188-190).  dist == NULL stands
+ * for unit distances.  Returns the number of roots, -1 on a bad id or on a
+ * second predecessor (the KASSERT at :183). */
+int64_t orc_reverse_list(const uint64_t *succ, const int64_t *dist, uint64_t n, uint64_t *pred,
+                         int64_t *dist_to_pred, uint64_t *roots) {
+  for (uint64_t v = 0; v < n; v++) {
+    pred[v] = v;
+    dist_to_pred[v] = 0;
+  }
+  int64_t nroots = 0;
+  for (uint64_t u = 0; u < n; u++) {
+    uint64_t s = succ[u];
+    if (s >= n) return -1;
+    if (s == u) {
+      if (roots) roots[nroots] = u;
+      nroots++;
+      continue;
+    }
+    if (pred[s] != s) return -1;
+    pred[s] = u;
+    dist_to_pred[s] = dist ? dist[u] : 1;
+  }
+  return nroots;
+}
+
+/* Layout identical to kc_stats in include/kascade_amd.h. */
+typedef struct orc_stats {
+  uint64_t num_trees;
+  int64_t max_rank;
+  double avg_rank;
+  uint64_t num_nontrivial_trees;
+  uint64_t nontrivial_size_sum;
+  int64_t nontrivial_rank_sum;
+  uint64_t max_size;
+  double avg_size;
+  double nontrivial_avg_size;
+  double nontrivial_avg_rank;
+} orc_stats;
+
+static double safe_division(double a, double b) { return b == 0 ? 0.0 : a / b; } /* stats.cpp:18-22 */
+
+/* compute_stats, bench/detail/stats.cpp:147-162, with level as StatsLevel
+ * (none 1, basic 2, extensive-reduced-output 3, extensive 4):
+ *   basic (compute_basic_stats :115-145): roots counted by root[i] == i, the
+ *     maximum rank and the rank sum / n;
+ *   extensive (compute_per_tree_stats :24-113): per root value its size, rank
+ *     sum and maximum rank; the aggregates over trees of size > 1.
+ * The per-tree list (level 4) is written in ascending root order; the
+ * reference's order is its hash map's, which carries no meaning.  Root values
+ * must be < n.  Returns the number of per-tree entries, -1 on a bad root. */
+int64_t orc_compute_stats(const uint64_t *root, const int64_t *rank, uint64_t n, int level, orc_stats *out,
+                          uint64_t *tree_root, uint64_t *tree_size, int64_t *tree_max, int64_t *tree_sum) {
+  memset(out, 0, sizeof(*out));
+  if (n == 0 || level <= 1) return 0;
+  int64_t mx = INT64_MIN, sum = 0;
+  uint64_t roots = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    roots += root[i] == i;
+    if (rank[i] > mx) mx = rank[i];
+    sum += rank[i];
+  }
+  out->num_trees = roots;
+  out->max_rank = mx;
+  out->avg_rank = (double)sum / (double)n;
+  if (level == 2) return 0;
+  uint64_t *size = (uint64_t *)calloc(n, sizeof(uint64_t));
+  int64_t *rsum = (int64_t *)calloc(n, sizeof(int64_t));
+  int64_t *rmax = (int64_t *)malloc(n * sizeof(int64_t));
+  for (uint64_t i = 0; i < n; i++) rmax[i] = INT64_MIN;
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t r = root[i];
+    if (r >= n) { free(size); free(rsum); free(rmax); return -1; }
+    size[r]++;                               /* :34 */
+    rsum[r] += rank[i];                      /* :35 */
+    if (rank[i] > rmax[r]) rmax[r] = rank[i]; /* :36 */
+  }
+  uint64_t trees = 0;
+  int64_t k = 0;
+  for (uint64_t r = 0; r < n; r++) {
+    if (size[r] == 0) continue;
+    trees++;
+    if (size[r] > out->max_size) out->max_size = size[r];
+    if (size[r] > 1) { /* :63-69 */
+      out->num_nontrivial_trees++;
+      out->nontrivial_size_sum += size[r];
+      out->nontrivial_rank_sum += rsum[r];
+      if (level == 4) {
+        if (tree_root) tree_root[k] = r;
+        if (tree_size) tree_size[k] = size[r];
+        if (tree_max) tree_max[k] = rmax[r];
+        if (tree_sum) tree_sum[k] = rsum[r];
+      }
+      k++;
+    }
+  }
+  out->num_trees = trees;
+  uint64_t trivial = trees - out->num_nontrivial_trees; /* :92-100 */
+  out->avg_size = safe_division((double)(out->nontrivial_size_sum + trivial), (double)trees);
+  out->nontrivial_avg_size = safe_division((double)out->nontrivial_size_sum, (double)out->num_nontrivial_trees);
+  out->nontrivial_avg_rank = safe_division((double)out->nontrivial_rank_sum, (double)out->nontrivial_size_sum);
+  free(size); free(rsum); free(rmax);
+  return level == 4 ? k : 0;
+}

@@ -16,6 +16,9 @@ void orc_gen_list(uint64_t n, uint64_t seed, int weighted, uint64_t* succ, int64
 void orc_gen_tree(uint64_t n, uint64_t seed, uint64_t* parent);
 void orc_local_pointer_chasing(uint64_t* succ, int64_t* rank, uint64_t n);
 int orc_euler_tour_rooting(uint64_t* succ, int64_t* rank, uint64_t n);
+int orc_is_list(const uint64_t* succ, uint64_t n);
+int64_t orc_reverse_list(const uint64_t* succ, const int64_t* dist, uint64_t n, uint64_t* pred,
+                         int64_t* dist_to_pred, uint64_t* roots);
 }
 
 namespace ka = kascade_amd;
@@ -131,8 +134,45 @@ static void errors_throw() {
   EXPECT(thrown);
 }
 
+// successor utilities and statistics through the C++ layer (successor_utils.hpp,
+// bench/detail/stats.hpp shapes)
+static void utilities_match_oracle() {
+  const uint64_t n = 50000;
+  std::vector<ka::idx_t> succ(n);
+  std::vector<ka::rank_t> w(n);
+  orc_gen_list(n, 31, 1, succ.data(), w.data());
+  EXPECT(ka::is_list(succ) == (orc_is_list(succ.data(), n) == 1));
+  std::vector<ka::idx_t> two = {2, 2, 2};
+  EXPECT(!ka::is_list(two));
+  ka::LeafInfo info(succ);
+  EXPECT(info.num_local_leaves() == 1);
+  std::vector<ka::idx_t> pred(n), epred(n), eroots(n);
+  std::vector<ka::rank_t> dpred(n), edpred(n);
+  auto roots = ka::reverse_list(succ, w, pred, dpred);
+  const int64_t k = orc_reverse_list(succ.data(), w.data(), n, epred.data(), edpred.data(), eroots.data());
+  eroots.resize(static_cast<size_t>(k));
+  EXPECT(pred == epred);
+  EXPECT(dpred == edpred);
+  EXPECT(roots == eroots);
+  std::vector<ka::idx_t> root = succ;
+  std::vector<ka::rank_t> rank = w;
+  orc_local_pointer_chasing(root.data(), rank.data(), n);
+  auto st = ka::compute_stats(ka::StatsLevel::extensive, root, rank);
+  EXPECT(st.basic_stats && st.basic_stats->num_trees == 1);
+  EXPECT(st.extensive_stats && st.extensive_stats->per_tree_stats->size() == 1);
+  int64_t sum = 0, mx = 0;
+  for (auto r : rank) {
+    sum += r;
+    mx = r > mx ? r : mx;
+  }
+  EXPECT(st.basic_stats->max_rank == mx);
+  EXPECT((*st.extensive_stats->per_tree_stats)[0].rank_sum == sum);
+  EXPECT((*st.extensive_stats->per_tree_stats)[0].size == n);
+}
+
 int main() {
   pack_works();
+  utilities_match_oracle();
   pack_sanity_check();
   adapters_match_gather_rank();
   errors_throw();

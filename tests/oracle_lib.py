@@ -46,6 +46,11 @@ def lib():
             "orc_unpack": (None, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_euler_tour_rooting": (c.c_int, [u64p, i64p, c.c_uint64]),
             "orc_sparse_ruling_set": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int, c.c_int]),
+            "orc_is_list": (c.c_int, [u64p, c.c_uint64]),
+            "orc_leaf_info": (c.c_int64, [u64p, c.c_uint64, c.c_void_p]),
+            "orc_reverse_list": (c.c_int64, [u64p, c.c_void_p, c.c_uint64, u64p, i64p, c.c_void_p]),
+            "orc_compute_stats": (c.c_int64, [u64p, i64p, c.c_uint64, c.c_int, c.c_void_p, c.c_void_p,
+                                              c.c_void_p, c.c_void_p, c.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -155,3 +160,69 @@ def sparse_ruling_set(succ, w, p=4, levels=2):
     rc = lib().orc_sparse_ruling_set(_p64(s), _p64(r), len(s), p, levels)
     assert rc == 0
     return s, r
+
+
+def is_list(succ):
+    succ = np.ascontiguousarray(succ, dtype=np.uint64)
+    rc = lib().orc_is_list(_p64(succ), len(succ))
+    assert rc >= 0
+    return bool(rc)
+
+
+def leaf_info(succ):
+    succ = np.ascontiguousarray(succ, dtype=np.uint64)
+    flags = np.empty(len(succ), np.uint8)
+    k = lib().orc_leaf_info(_p64(succ), len(succ), flags.ctypes.data)
+    assert k >= 0
+    return flags, int(k)
+
+
+def reverse_list(succ, dist=None):
+    succ = np.ascontiguousarray(succ, dtype=np.uint64)
+    n = len(succ)
+    pred = np.empty(n, np.uint64)
+    dpred = np.empty(n, np.int64)
+    roots = np.empty(max(n, 1), np.uint64)
+    d = None if dist is None else np.ascontiguousarray(dist, dtype=np.int64)
+    k = lib().orc_reverse_list(_p64(succ), None if d is None else d.ctypes.data, n, _p64(pred), _p64(dpred),
+                               roots.ctypes.data)
+    if k < 0:
+        raise ValueError("not a list")
+    return pred, dpred, roots[:k]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [("num_trees", ctypes.c_uint64), ("max_rank", ctypes.c_int64), ("avg_rank", ctypes.c_double),
+                ("num_nontrivial_trees", ctypes.c_uint64), ("nontrivial_size_sum", ctypes.c_uint64),
+                ("nontrivial_rank_sum", ctypes.c_int64), ("max_size", ctypes.c_uint64),
+                ("avg_size", ctypes.c_double), ("nontrivial_avg_size", ctypes.c_double),
+                ("nontrivial_avg_rank", ctypes.c_double)]
+
+
+STATS_LEVELS = {"none": 1, "basic": 2, "extensive-reduced-output": 3, "extensive": 4}
+
+
+def compute_stats(root, rank, level="basic"):
+    """Same dict shape as kascade_amd.Context.compute_stats."""
+    root = np.ascontiguousarray(root, dtype=np.uint64)
+    rank = np.ascontiguousarray(rank, dtype=np.int64)
+    n = len(root)
+    lv = STATS_LEVELS[level]
+    st = _Stats()
+    tr, ts = np.empty(max(n, 1), np.uint64), np.empty(max(n, 1), np.uint64)
+    tm, tsum = np.empty(max(n, 1), np.int64), np.empty(max(n, 1), np.int64)
+    k = lib().orc_compute_stats(_p64(root), _p64(rank), n, lv, ctypes.byref(st), tr.ctypes.data, ts.ctypes.data,
+                                tm.ctypes.data, tsum.ctypes.data)
+    assert k >= 0
+    out = {}
+    if lv >= 2:
+        out["basic_stats"] = {"num_trees": st.num_trees, "max_rank": st.max_rank, "avg_rank": st.avg_rank}
+    if lv >= 3:
+        ext = {f: getattr(st, f) for f in ("num_trees", "num_nontrivial_trees", "nontrivial_size_sum",
+                                           "nontrivial_rank_sum", "max_size", "avg_size", "nontrivial_avg_size",
+                                           "nontrivial_avg_rank")}
+        if lv == 4:
+            ext["per_tree_stats"] = [{"root": int(tr[i]), "size": int(ts[i]), "max_rank": int(tm[i]),
+                                      "rank_sum": int(tsum[i])} for i in range(k)]
+        out["extensive_stats"] = ext
+    return out

@@ -244,3 +244,119 @@ def test_sparse_ruling_set_generated_lists():
         r, k = o.sparse_ruling_set(succ, w, 4)
         er, ek = o.sequential_walk(succ, w)
         assert (r == er).all() and (k == ek).all()
+
+
+# ---- successor utilities and statistics ------------------------------------
+def _indegree(succ):
+    n = len(succ)
+    nonroot = succ != np.arange(n, dtype=np.uint64)
+    return np.bincount(succ[nonroot].astype(np.int64), minlength=n)
+
+
+def _random_function(n, seed):
+    """Successor arrays with in-degree > 1 somewhere (random functions)."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, n, size=n).astype(np.uint64)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_is_list_and_leaf_info_brute_force(seed):
+    for n in (1, 2, 7, 300, 5000):
+        for succ in (random_forest_lists(n, max(1, n // 50), seed + n)[0], _random_function(n, seed + n),
+                     np.arange(n, dtype=np.uint64)):
+            indeg = _indegree(succ)
+            assert o.is_list(succ) == bool(indeg.max(initial=0) <= 1)
+            flags, k = o.leaf_info(succ)
+            assert (flags == (indeg == 0)).all() and k == int((indeg == 0).sum())
+
+
+def test_is_list_hand_cases():
+    assert o.is_list(np.array([1, 2, 2], np.uint64))          # 0 -> 1 -> 2 (tail)
+    assert not o.is_list(np.array([2, 2, 2], np.uint64))      # 0 and 1 both point at 2
+    assert o.is_list(np.array([0, 1, 2], np.uint64))          # singletons
+    flags, k = o.leaf_info(np.array([1, 2, 2], np.uint64))
+    assert list(flags) == [1, 0, 0] and k == 1                # only the head
+    flags, k = o.leaf_info(np.array([0], np.uint64))
+    assert list(flags) == [1] and k == 1                      # a lone root counts as a leaf
+
+
+@pytest.mark.parametrize("n,nlists", [(1, 1), (9, 2), (1000, 1), (5000, 37)])
+def test_reverse_list_invariants(n, nlists):
+    succ, w = random_forest_lists(n, nlists, 3 * n + nlists)
+    pred, dpred, roots = o.reverse_list(succ, w)
+    ids = np.arange(n, dtype=np.uint64)
+    # brute force: pred[succ[u]] = u for non-roots, else (self, 0)
+    ep, ed = ids.copy(), np.zeros(n, np.int64)
+    nonroot = succ != ids
+    ep[succ[nonroot]] = ids[nonroot]
+    ed[succ[nonroot]] = w[nonroot]
+    assert (pred == ep).all() and (dpred == ed).all()
+    assert (roots == np.nonzero(succ == ids)[0]).all()        # ascending original roots
+    # reversing twice gives the list back
+    succ2, w2, roots2 = o.reverse_list(pred, dpred)
+    assert (succ2 == succ).all() and (w2 == w).all()
+    assert set(roots2.tolist()) == set(np.nonzero(_indegree(succ) == 0)[0].tolist())
+    # rank along the list + rank along the reversed list = the list's total
+    r, k = o.sequential_walk(succ, w)
+    rr, kr = o.sequential_walk(pred, dpred)
+    head_of = {int(t): int(h) for h, t in zip(rr, r)}          # reversed root = original head
+    total = np.array([k[head_of[int(t)]] for t in r], np.int64)
+    assert (k + kr == total).all()
+
+
+def test_reverse_list_rejects_second_predecessor():
+    with pytest.raises(ValueError):
+        o.reverse_list(np.array([2, 2, 2], np.uint64))
+
+
+def _stats_brute(root, rank, level):
+    n = len(root)
+    out = {"basic_stats": {"num_trees": int((root == np.arange(n)).sum()), "max_rank": int(rank.max()),
+                           "avg_rank": float(rank.sum()) / n}}
+    if level == "basic":
+        return out
+    trees = {}
+    for r, k in zip(root.tolist(), rank.tolist()):
+        t = trees.setdefault(r, [0, 0, -2**63])
+        t[0] += 1
+        t[1] += k
+        t[2] = max(t[2], k)
+    nt = {r: t for r, t in trees.items() if t[0] > 1}
+    zs = sum(t[0] for t in nt.values())
+    zr = sum(t[1] for t in nt.values())
+    triv = len(trees) - len(nt)
+    ext = {"num_trees": len(trees), "num_nontrivial_trees": len(nt), "nontrivial_size_sum": zs,
+           "nontrivial_rank_sum": zr, "max_size": max(t[0] for t in trees.values()),
+           "avg_size": (zs + triv) / len(trees), "nontrivial_avg_size": zs / len(nt) if nt else 0.0,
+           "nontrivial_avg_rank": zr / zs if zs else 0.0}
+    if level == "extensive":
+        ext["per_tree_stats"] = [{"root": r, "size": t[0], "max_rank": t[2], "rank_sum": t[1]}
+                                 for r, t in sorted(nt.items())]
+    out["extensive_stats"] = ext
+    return out
+
+
+@pytest.mark.parametrize("level", ["basic", "extensive-reduced-output", "extensive"])
+@pytest.mark.parametrize("n,nlists", [(1, 1), (50, 50), (3000, 1), (3000, 200)])
+def test_compute_stats_brute_force(level, n, nlists):
+    succ, w = random_forest_lists(n, nlists, n + nlists)
+    root, rank = o.sequential_walk(succ, w)
+    assert o.compute_stats(root, rank, level) == _stats_brute(root, rank, level)
+
+
+def test_compute_stats_closed_form():
+    """One unit-weight list of n: ranks n-1..0, so avg (n-1)/2, rank sum n(n-1)/2."""
+    n = 1 << 16
+    succ, w = o.gen_list(n, 5, False)
+    root, rank = o.sequential_walk(succ, w)
+    st = o.compute_stats(root, rank, "extensive")
+    assert st["basic_stats"] == {"num_trees": 1, "max_rank": n - 1, "avg_rank": (n - 1) / 2}
+    e = st["extensive_stats"]
+    assert e["per_tree_stats"] == [{"root": int(root[0]), "size": n, "max_rank": n - 1,
+                                    "rank_sum": n * (n - 1) // 2}]
+    assert e["avg_size"] == n and e["nontrivial_avg_rank"] == (n - 1) / 2
+    # a tree: depths from the generator's closed form
+    parent = o.gen_tree(5000, 3)
+    er, ed = o.tree_closed_form(5000, 3)
+    e = o.compute_stats(er, ed, "extensive")["extensive_stats"]
+    assert e["per_tree_stats"][0]["rank_sum"] == int(ed.sum()) and e["max_size"] == 5000
