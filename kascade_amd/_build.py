@@ -47,9 +47,36 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_runner()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+RUNNER = os.path.join(HERE, "kascade_runner")
+
+
+def _git_tag():
+    try:
+        sha = subprocess.run(["git", "-C", HERE, "rev-parse", "--short=8", "HEAD"], capture_output=True,
+                             text=True, check=True).stdout.strip()
+        dirty = subprocess.run(["git", "-C", HERE, "status", "--porcelain"], capture_output=True,
+                               text=True).stdout.strip()
+        return sha + ("-dirty" if dirty else "")
+    except (OSError, subprocess.CalledProcessError):
+        return "unknown"
+
+
+def _build_runner():
+    """The benchmark harness executable (host code over the C ABI)."""
+    tmp = RUNNER + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++20", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           f'-DKC_GIT_TAG="{_git_tag()}"', os.path.join(CSRC, "runner.cpp"), "-o", tmp,
+           "-L", HERE, "-lkascade_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"runner build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, RUNNER)
 
 
 if __name__ == "__main__":
