@@ -255,6 +255,28 @@ kc_status kc_gen_lists_segmented(void *ctx, const uint64_t *seg_off, uint64_t ns
                                  const uint64_t *seeds, const uint8_t *local_flags, int weighted,
                                  uint32_t *succ_out, int64_t *weight_out, void *stream);
 
+/* ---- graph inputs (the reference's "bfs" / "eulertour" input processing) -- */
+typedef enum kc_graph_kind {
+  KC_GRAPH_GNM = 0, /* m edges, both endpoints uniform (KaGen gnm-undirected) */
+  KC_GRAPH_RMAT = 1 /* m edges, Graph500 R-MAT probabilities (0.57, 0.19, 0.19, 0.05) */
+} kc_graph_kind;
+/* m seeded random undirected edges on n vertices (identical to the CPU
+ * generator); a self loop's head moves to the next vertex. */
+kc_status kc_gen_graph(void *ctx, int32_t kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t *src_out,
+                       uint64_t *dst_out, void *stream);
+/* CSR of the undirected graph: xadj_out (n+1 entries), adj_out (2m entries,
+ * both directions of every edge; the order within a vertex is unspecified). */
+kc_status kc_build_csr(void *ctx, const uint64_t *src, const uint64_t *dst, uint64_t n, uint64_t m,
+                       uint64_t *xadj_out, uint64_t *adj_out, void *stream);
+/* BFS forest of a symmetric CSR graph (generate_bfs_tree,
+ * bench/input/bfs_forest_generation.cpp:52-76): isolated vertices are their
+ * own parents; every other component is a BFS tree rooted at its smallest
+ * vertex id (parent_out[root] = root).  Roots and depths are unique; a
+ * vertex's parent is one of its neighbours one level up.  KC_EINVAL for a
+ * non-symmetric adjacency. */
+kc_status kc_bfs_forest(void *ctx, const uint64_t *xadj, const uint64_t *adj, uint64_t n, uint64_t *parent_out,
+                        void *stream);
+
 /* ---- diagnostics ------------------------------------------------------- */
 /* Random-gather ceiling: dst[i] = src[idx[i]] over n uint64 with idx a seeded
  * random permutation (n = 2^27 exceeds every cache).  Reports the effective

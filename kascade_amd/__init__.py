@@ -121,6 +121,9 @@ _SIGS = {
     "kc_reverse_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, _P, _U64, ctypes.POINTER(_U64), _P]),
     "kc_compute_stats": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _U64, ctypes.POINTER(Stats), _P, _P, _P, _P,
                                         _U64, _P]),
+    "kc_gen_graph": (ctypes.c_int, [_P, ctypes.c_int32, _U64, _U64, _U64, _P, _P, _P]),
+    "kc_build_csr": (ctypes.c_int, [_P, _P, _P, _U64, _U64, _P, _P, _P]),
+    "kc_bfs_forest": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -320,6 +323,34 @@ class Context:
                                           "rank_sum": int(tsum[i])} for i in range(k)]
             out["extensive_stats"] = ext
         return out
+
+    # ---- graph inputs ----------------------------------------------------------------
+    GRAPH_KINDS = {"gnm": 0, "rmat": 1}
+
+    def gen_graph(self, kind, n, m, seed, stream=None):
+        """m seeded random undirected edges on n vertices (device tensors)."""
+        import torch
+        src = torch.empty(m, dtype=torch.uint64, device="cuda")
+        dst = torch.empty(m, dtype=torch.uint64, device="cuda")
+        self._check(self._lib.kc_gen_graph(self.handle, self.GRAPH_KINDS[kind], n, m, seed, _ptr(src, np.uint64),
+                                           _ptr(dst, np.uint64), stream))
+        return src, dst
+
+    def build_csr(self, src, dst, n, stream=None):
+        m = _numel(src)
+        xadj = _like(src, np.uint64, n + 1)
+        adj = _like(src, np.uint64, 2 * m)
+        self._check(self._lib.kc_build_csr(self.handle, _ptr(src, np.uint64), _ptr(dst, np.uint64), n, m,
+                                           _ptr(xadj, np.uint64), _ptr(adj, np.uint64), stream))
+        return xadj, adj
+
+    def bfs_forest(self, xadj, adj, stream=None):
+        """generate_bfs_tree: the BFS forest's parent array."""
+        n = _numel(xadj) - 1
+        parent = _like(xadj, np.uint64, n)
+        self._check(self._lib.kc_bfs_forest(self.handle, _ptr(xadj, np.uint64), _ptr(adj, np.uint64), n,
+                                            _ptr(parent, np.uint64), stream))
+        return parent
 
     def pd_rounds(self, succ, weight, rounds, stream=None):
         so = _like(succ, np.uint64)

@@ -595,6 +595,63 @@ kc_status kc_compute_stats(void* vctx, int32_t level, const uint64_t* root, cons
   });
 }
 
+kc_status kc_gen_graph(void* vctx, int32_t kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src_out,
+                       uint64_t* dst_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (kind != KC_GRAPH_GNM && kind != KC_GRAPH_RMAT) throw KcError{KC_EINVAL, "invalid graph kind"};
+    if (m && (n == 0 || !src_out || !dst_out)) throw KcError{KC_EINVAL, "null array or empty vertex set"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<uint64_t> s(ctx, scratch, src_out, m, false, true, st);
+    Staged<uint64_t> d(ctx, scratch, dst_out, m, false, true, st);
+    kc::gen_graph_device(ctx, kind, n, m, seed, s.dev(), d.dev(), st);
+    s.finish();
+    d.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_build_csr(void* vctx, const uint64_t* src, const uint64_t* dst, uint64_t n, uint64_t m,
+                       uint64_t* xadj_out, uint64_t* adj_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (!xadj_out || (m && (!src || !dst || !adj_out))) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    Staged<const uint64_t> s(ctx, scratch, src, m, true, false, st);
+    Staged<const uint64_t> d(ctx, scratch, dst, m, true, false, st);
+    Staged<uint64_t> x(ctx, scratch, xadj_out, n + 1, false, true, st);
+    Staged<uint64_t> a(ctx, scratch, adj_out, 2 * m, false, true, st);
+    kc::build_csr_device(ctx, s.dev(), d.dev(), n, m, x.dev(), a.dev(), st);
+    x.finish();
+    a.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_bfs_forest(void* vctx, const uint64_t* xadj, const uint64_t* adj, uint64_t n, uint64_t* parent_out,
+                        void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n && (!xadj || !parent_out)) throw KcError{KC_EINVAL, "null array"};
+    hipStream_t st = pick(ctx, stream);
+    Scratch scratch(ctx);
+    uint64_t m2 = 0;
+    if (n) {
+      // the adjacency length is xadj[n]
+      if (kc::is_device_ptr(xadj))
+        KC_HIP(hipMemcpy(&m2, xadj + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+      else
+        m2 = xadj[n];
+    }
+    if (m2 && !adj) throw KcError{KC_EINVAL, "null adjacency"};
+    Staged<const uint64_t> x(ctx, scratch, xadj, n + 1, true, false, st);
+    Staged<const uint64_t> a(ctx, scratch, adj, m2, true, false, st);
+    Staged<uint64_t> p(ctx, scratch, parent_out, n, false, true, st);
+    kc::bfs_forest_device(ctx, x.dev(), a.dev(), n, p.dev(), st);
+    p.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
 kc_status kc_pack(void* vctx, const uint64_t* succ, const int64_t* rank, uint64_t n,
                   const uint64_t* active, uint64_t nactive, uint64_t* psucc_out,
                   int64_t* prank_out, void* stream) {

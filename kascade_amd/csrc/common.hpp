@@ -222,6 +222,13 @@ void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
 void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, Rec* rec,
                   hipStream_t st);
 void validate_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
+// graph inputs (graph.hip)
+void gen_graph_device(Context* ctx, int kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src, uint64_t* dst,
+                      hipStream_t st);
+void build_csr_device(Context* ctx, const uint64_t* src, const uint64_t* dst, uint64_t n, uint64_t m,
+                      uint64_t* xadj, uint64_t* adj, hipStream_t st);
+void bfs_forest_device(Context* ctx, const uint64_t* xadj, const uint64_t* adj, uint64_t n, uint64_t* parent,
+                       hipStream_t st);
 // successor utilities and statistics (listutil.hip)
 bool is_list_device(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
 uint64_t leaf_info_device(Context* ctx, const uint64_t* succ, uint64_t n, uint8_t* is_leaf,

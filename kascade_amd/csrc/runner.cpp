@@ -144,8 +144,10 @@ struct Json {
 struct Config {
   std::string output_path = "stdout";
   uint64_t iterations = 1;
-  std::string input = "list";  // list | list-local | forest | tree
+  std::string input = "list";  // list | list-local | forest | tree | gnm | rmat
+  std::string input_processing = "bfs";  // graph inputs: bfs | eulertour
   uint64_t n = uint64_t{1} << 20;
+  uint64_t m = uint64_t{1} << 22;  // graph inputs: edges
   uint64_t seed = 1;
   uint64_t lists = 1000;  // forest input
   bool weighted = false;
@@ -164,8 +166,9 @@ void usage() {
   std::printf(
       "usage: kascade_runner [options]\n"
       "  --algorithm PointerDoubling|SparseRulingSet|EulerTour   (default SparseRulingSet)\n"
-      "  --input list|list-local|forest|tree                     (default list)\n"
-      "  --n N  --seed S  --lists L  --weighted\n"
+      "  --input list|list-local|forest|tree|gnm|rmat            (default list)\n"
+      "  --input-processing bfs|eulertour  (graph inputs: BFS forest, or its Euler tour)\n"
+      "  --n N  --m M  --seed S  --lists L  --weighted\n"
       "  --iterations K                                          (default 1)\n"
       "  --verify-level L  --verify-continue-on-mismatch        (default 1)\n"
       "  --statistics none|basic|extensive|extensive-reduced-output\n"
@@ -194,6 +197,8 @@ Config parse(int argc, char** argv) {
     else if (a == "--iterations" || a == "-i") c.iterations = num();
     else if (a == "--input") c.input = val();
     else if (a == "--n") c.n = num();
+    else if (a == "--m") c.m = num();
+    else if (a == "--input-processing") c.input_processing = val();
     else if (a == "--seed") c.seed = num();
     else if (a == "--lists") c.lists = num();
     else if (a == "--weighted") c.weighted = true;
@@ -210,8 +215,11 @@ Config parse(int argc, char** argv) {
   }
   if (c.algorithm != "PointerDoubling" && c.algorithm != "SparseRulingSet" && c.algorithm != "EulerTour")
     die("invalid algorithm " + c.algorithm);
-  if (c.input != "list" && c.input != "list-local" && c.input != "forest" && c.input != "tree")
+  if (c.input != "list" && c.input != "list-local" && c.input != "forest" && c.input != "tree" &&
+      c.input != "gnm" && c.input != "rmat")
     die("invalid input " + c.input);
+  if (c.input_processing != "bfs" && c.input_processing != "eulertour")
+    die("invalid input processing " + c.input_processing);
   if (c.statistics_level != "none" && c.statistics_level != "basic" && c.statistics_level != "extensive" &&
       c.statistics_level != "extensive-reduced-output")
     die("invalid statistics level " + c.statistics_level);
@@ -258,6 +266,31 @@ void generate(const Config& c, const Device& d, std::vector<uint64_t>& succ, std
   if (c.input == "tree") {
     d.check(kc_gen_tree(d.ctx, n, c.seed, ds, nullptr), "kc_gen_tree");
     hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
+  } else if (c.input == "gnm" || c.input == "rmat") {
+    // generate_input with "bfs" / "eulertour" processing (generation.cpp:74-90)
+    const int kind = c.input == "gnm" ? KC_GRAPH_GNM : KC_GRAPH_RMAT;
+    uint64_t* src = dalloc<uint64_t>(c.m);
+    uint64_t* dst = dalloc<uint64_t>(c.m);
+    uint64_t* xadj = dalloc<uint64_t>(n + 1);
+    uint64_t* adj = dalloc<uint64_t>(2 * c.m);
+    d.check(kc_gen_graph(d.ctx, kind, n, c.m, c.seed, src, dst, nullptr), "kc_gen_graph");
+    d.check(kc_build_csr(d.ctx, src, dst, n, c.m, xadj, adj, nullptr), "kc_build_csr");
+    d.check(kc_bfs_forest(d.ctx, xadj, adj, n, ds, nullptr), "kc_bfs_forest");
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    (void)hipFree(xadj);
+    (void)hipFree(adj);
+    if (c.input_processing == "eulertour") {
+      // compute_euler_tour (eulertour.cpp:379-392): the tour is the input list
+      uint64_t* tour = dalloc<uint64_t>(2 * n);
+      d.check(kc_compute_euler_tour(d.ctx, ds, n, tour, nullptr, nullptr), "kc_compute_euler_tour");
+      succ.assign(2 * n, 0);
+      w.assign(2 * n, 0);
+      hip_check(hipMemcpy(succ.data(), tour, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
+      (void)hipFree(tour);
+    } else {
+      hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
+    }
   } else if (c.input == "forest") {
     // `lists` lists of near-equal sizes laid out one after another
     std::vector<uint64_t> off(c.lists + 1), seeds(c.lists);
@@ -290,8 +323,9 @@ void generate(const Config& c, const Device& d, std::vector<uint64_t>& succ, std
     hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
     if (c.weighted) hip_check(hipMemcpy(w.data(), dw, n * 8, hipMemcpyDeviceToHost), "copy");
   }
-  if (!c.weighted || c.input == "tree")  // unit weights, as set_initial_ranking_state
-    for (uint64_t i = 0; i < n; i++) w[i] = succ[i] == i ? 0 : 1;
+  if (!c.weighted || c.input == "tree" || c.input == "gnm" || c.input == "rmat")
+    // unit weights, as set_initial_ranking_state
+    for (uint64_t i = 0; i < succ.size(); i++) w[i] = succ[i] == i ? 0 : 1;
   (void)hipFree(ds);
   (void)hipFree(dw);
 }
@@ -449,11 +483,10 @@ int main(int argc, char** argv) {
   const Config c = parse(argc, argv);
   Device d;
   if (kc_context_create(0, 0, &d.ctx) != KC_OK) die("kc_context_create failed");
-  const uint64_t n = c.n;
-
   std::vector<uint64_t> succ;
   std::vector<int64_t> w;
   generate(c, d, succ, w);
+  const uint64_t n = succ.size();  // 2 x the vertices for an Euler tour input
 
   uint64_t* d_root = dalloc<uint64_t>(n);
   int64_t* d_rank = dalloc<int64_t>(n);
@@ -539,10 +572,12 @@ int main(int argc, char** argv) {
   cfg["iterations"] = Json::uinteger(c.iterations);
   {
     std::ostringstream g;
-    g << c.input << ";n=" << n << ";seed=" << c.seed << ";weighted=" << (c.weighted ? 1 : 0);
+    const bool graph = c.input == "gnm" || c.input == "rmat";
+    g << c.input << ";n=" << c.n << ";seed=" << c.seed << ";weighted=" << (c.weighted ? 1 : 0);
     if (c.input == "forest") g << ";lists=" << c.lists;
+    if (graph) g << ";m=" << c.m;
     cfg["input"]["kagen_option_string"] = Json::str(g.str());
-    cfg["input"]["input_processing"] = Json::str("none");
+    cfg["input"]["input_processing"] = Json::str(graph ? c.input_processing : "none");
   }
   cfg["algorithm"] = Json::str(c.algorithm);
   cfg["sparse_ruling_set"]["ruler_spacing"] = Json::uinteger(c.ruler_spacing);

@@ -154,3 +154,88 @@ int64_t orc_compute_stats(const uint64_t *root, const int64_t *rank, uint64_t n,
   free(size); free(rsum); free(rmax);
   return level == 4 ? k : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Graph inputs (bench/input/generation.cpp:60-96, "bfs" processing).        */
+/* ------------------------------------------------------------------------ */
+uint64_t orc_splitmix64(uint64_t x);
+
+/* Seeded random undirected edges (not reference code: KaGen is unavailable;
+ * the same arithmetic runs on the device, kascade_amd/csrc/graph.hip).
+ * kind 0 GNM: endpoints uniform; kind 1 RMAT: one quadrant per bit of
+ * ceil(log2 n) bits with 16-bit thresholds for (0.57, 0.19, 0.19, 0.05), ids
+ * folded by modulo.  A self loop's head moves to the next vertex. */
+void orc_gen_graph(int kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t *src, uint64_t *dst) {
+  uint32_t bits = 0;
+  while (bits < 63 && (UINT64_C(1) << bits) < n) bits++;
+  for (uint64_t e = 0; e < m; e++) {
+    uint64_t a, b;
+    if (kind == 0) {
+      a = orc_splitmix64(seed ^ (2 * e + 1) * UINT64_C(0x9E3779B97F4A7C15)) % n;
+      b = orc_splitmix64(seed ^ (2 * e + 2) * UINT64_C(0xD1B54A32D192ED03)) % n;
+    } else {
+      a = 0;
+      b = 0;
+      for (uint32_t k = 0; k < bits; k++) {
+        uint32_t r = (uint32_t)(orc_splitmix64(seed ^ (e * 64 + k + 1) * UINT64_C(0x94D049BB133111EB)) & 0xFFFF);
+        uint32_t q = r < 37356 ? 0 : r < 49807 ? 1 : r < 62259 ? 2 : 3;
+        a = (a << 1) | (q >> 1);
+        b = (b << 1) | (q & 1);
+      }
+      a %= n;
+      b %= n;
+    }
+    if (a == b) b = (b + 1) % n;
+    src[e] = a;
+    dst[e] = b;
+  }
+}
+
+/* generate_bfs_tree, bench/input/bfs_forest_generation.cpp:52-76, on one rank:
+ *   handle_isolated_vertices (:40-50): a vertex without neighbours is its own
+ *   parent and visited;
+ *   then, while an unvisited vertex is left, the smallest one starts a
+ *   distributed_bfs (bench/input/bfs.hpp:30-74): the frontier holds (parent,
+ *   u) pairs, a pair whose u is unvisited marks u visited with that parent and
+ *   appends (u, v) for every neighbour v in adjacency order.
+ * The start vertex's parent is itself.  xadj has n + 1 entries. */
+void orc_bfs_forest(const uint64_t *xadj, const uint64_t *adj, uint64_t n, uint64_t *parent) {
+  uint8_t *visited = (uint8_t *)calloc(n ? n : 1, 1);
+  for (uint64_t v = 0; v < n; v++) {
+    if (xadj[v] == xadj[v + 1]) {
+      visited[v] = 1;
+      parent[v] = v;
+    }
+  }
+  uint64_t total = xadj[n];
+  uint64_t cap = total + n + 1;
+  uint64_t *fp = (uint64_t *)malloc(cap * sizeof(uint64_t)), *fu = (uint64_t *)malloc(cap * sizeof(uint64_t));
+  uint64_t *np = (uint64_t *)malloc(cap * sizeof(uint64_t)), *nu = (uint64_t *)malloc(cap * sizeof(uint64_t));
+  uint64_t next_start = 0;
+  for (;;) {
+    while (next_start < n && visited[next_start]) next_start++;
+    if (next_start >= n) break;
+    uint64_t nf = 1;
+    fp[0] = next_start;
+    fu[0] = next_start;
+    while (nf) {
+      uint64_t nn = 0;
+      for (uint64_t k = 0; k < nf; k++) {
+        uint64_t u = fu[k];
+        if (visited[u]) continue;
+        visited[u] = 1;
+        parent[u] = fp[k];
+        for (uint64_t j = xadj[u]; j < xadj[u + 1]; j++) {
+          np[nn] = u;
+          nu[nn] = adj[j];
+          nn++;
+        }
+      }
+      uint64_t *t;
+      t = fp; fp = np; np = t;
+      t = fu; fu = nu; nu = t;
+      nf = nn;
+    }
+  }
+  free(visited); free(fp); free(fu); free(np); free(nu);
+}

@@ -51,6 +51,8 @@ def lib():
             "orc_reverse_list": (c.c_int64, [u64p, c.c_void_p, c.c_uint64, u64p, i64p, c.c_void_p]),
             "orc_compute_stats": (c.c_int64, [u64p, i64p, c.c_uint64, c.c_int, c.c_void_p, c.c_void_p,
                                               c.c_void_p, c.c_void_p, c.c_void_p]),
+            "orc_gen_graph": (None, [c.c_int, c.c_uint64, c.c_uint64, c.c_uint64, u64p, u64p]),
+            "orc_bfs_forest": (None, [u64p, u64p, c.c_uint64, u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -226,3 +228,31 @@ def compute_stats(root, rank, level="basic"):
                                       "rank_sum": int(tsum[i])} for i in range(k)]
         out["extensive_stats"] = ext
     return out
+
+
+GRAPH_KINDS = {"gnm": 0, "rmat": 1}
+
+
+def gen_graph(kind, n, m, seed):
+    src = np.empty(max(m, 1), np.uint64)
+    dst = np.empty(max(m, 1), np.uint64)
+    lib().orc_gen_graph(GRAPH_KINDS[kind], n, m, seed, _p64(src), _p64(dst))
+    return src[:m], dst[:m]
+
+
+def csr(src, dst, n):
+    """Undirected CSR with each vertex's neighbours in ascending order."""
+    a = np.concatenate([src, dst]).astype(np.int64)
+    b = np.concatenate([dst, src]).astype(np.uint64)
+    order = np.lexsort((b, a))
+    xadj = np.zeros(n + 1, np.uint64)
+    np.cumsum(np.bincount(a, minlength=n), out=xadj[1:])
+    return xadj, np.ascontiguousarray(b[order])
+
+
+def bfs_forest(xadj, adj):
+    n = len(xadj) - 1
+    parent = np.empty(n, np.uint64)
+    adj = np.ascontiguousarray(adj, dtype=np.uint64) if len(adj) else np.zeros(1, np.uint64)
+    lib().orc_bfs_forest(_p64(np.ascontiguousarray(xadj, dtype=np.uint64)), _p64(adj), n, _p64(parent))
+    return parent

@@ -90,3 +90,25 @@ def test_runner_output_extension_and_errors(tmp_path):
     r = subprocess.run([RUNNER, "--input", "tree", "--n", "5000", "--algorithm", "SparseRulingSet",
                         "--validate-list"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "[error]" in r.stderr
+
+
+def test_runner_graph_inputs(tmp_path):
+    """--input gnm|rmat with "bfs" processing (a BFS forest, ranked by the
+    Euler tour) and "eulertour" processing (the forest's tour, ranked as a
+    list), verified against the independent reference run."""
+    say("runner graph inputs")
+    r, rep = run(tmp_path, "--algorithm", "EulerTour", "--input", "rmat", "--n", "65536", "--m", "400000",
+                 "--input-processing", "bfs", "--statistics", "basic")
+    assert r.returncode == 0 and "Rank array matches reference." in r.stderr
+    assert rep["config"]["input"]["input_processing"] == "bfs"
+    src, dst = o.gen_graph("rmat", 65536, 400000, 1)
+    xadj, adj = o.csr(src, dst, 65536)
+    parent = o.bfs_forest(xadj, adj)
+    import numpy as np
+    wts = (parent != np.arange(65536, dtype=np.uint64)).astype(np.int64)
+    er, ed = o.sequential_walk(parent, wts)
+    assert rep["stats"]["input_stats"]["basic_stats"] == o.compute_stats(er, ed, "basic")["basic_stats"]
+    r, rep = run(tmp_path, "--algorithm", "SparseRulingSet", "--input", "gnm", "--n", "100000", "--m", "150000",
+                 "--input-processing", "eulertour")
+    assert r.returncode == 0 and "Rank array matches reference." in r.stderr
+    assert rep["stats"]["input_distribution"]["total_size"] == 200000

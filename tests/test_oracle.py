@@ -360,3 +360,55 @@ def test_compute_stats_closed_form():
     er, ed = o.tree_closed_form(5000, 3)
     e = o.compute_stats(er, ed, "extensive")["extensive_stats"]
     assert e["per_tree_stats"][0]["rank_sum"] == int(ed.sum()) and e["max_size"] == 5000
+
+
+# ---- graph inputs (BFS forest) ----------------------------------------------
+def _bfs_brute(xadj, adj):
+    """Component minimum and BFS distance from it, by queue BFS."""
+    from collections import deque
+    n = len(xadj) - 1
+    root = np.full(n, -1, np.int64)
+    depth = np.zeros(n, np.int64)
+    for s in range(n):
+        if root[s] >= 0:
+            continue
+        root[s], depth[s] = s, 0
+        q = deque([s])
+        while q:
+            u = q.popleft()
+            for v in adj[xadj[u]:xadj[u + 1]].tolist():
+                if root[v] < 0:
+                    root[v], depth[v] = s, depth[u] + 1
+                    q.append(v)
+    return root.astype(np.uint64), depth
+
+
+@pytest.mark.parametrize("kind", ["gnm", "rmat"])
+@pytest.mark.parametrize("n,m", [(1, 0), (1, 3), (2, 1), (50, 20), (1000, 600), (1000, 5000), (4096, 4096)])
+def test_bfs_forest_matches_queue_bfs(kind, n, m):
+    src, dst = o.gen_graph(kind, n, m, n + 7 * m)
+    assert (src < n).all() and (dst < n).all()
+    if n > 1:
+        assert (src != dst).all()
+    xadj, adj = o.csr(src, dst, n)
+    parent = o.bfs_forest(xadj, adj)
+    er, ed = _bfs_brute(xadj, adj)
+    w = (parent != np.arange(n, dtype=np.uint64)).astype(np.int64)
+    r, d = o.sequential_walk(parent, w)          # a forest: root and depth
+    assert (r == er).all() and (d == ed).all()
+    for v in range(n):                           # parents are neighbours one level up
+        p = int(parent[v])
+        if p != v:
+            assert p in adj[xadj[v]:xadj[v + 1]].tolist() and ed[p] == ed[v] - 1
+
+
+def test_rmat_is_skewed():
+    """R-MAT concentrates edges on low ids: quadrant (0, 0) is chosen with
+    probability 0.57 per bit, so vertex 0's degree far exceeds the mean."""
+    n, m = 1 << 12, 1 << 16
+    src, dst = o.gen_graph("rmat", n, m, 3)
+    deg = np.bincount(np.concatenate([src, dst]).astype(np.int64), minlength=n)
+    assert deg[0] > 20 * deg.mean()
+    g_src, g_dst = o.gen_graph("gnm", n, m, 3)
+    gdeg = np.bincount(np.concatenate([g_src, g_dst]).astype(np.int64), minlength=n)
+    assert gdeg.max() < 4 * gdeg.mean()

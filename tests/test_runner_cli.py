@@ -18,7 +18,7 @@ def test_help_lists_the_reference_options():
 
 
 def test_bad_arguments_exit_with_an_error():
-    for args in (["--algorithm", "Nope"], ["--statistics", "all"], ["--input", "grid"], ["--n", "0"],
+    for args in (["--algorithm", "Nope"], ["--statistics", "all"], ["--input", "grid"], ["--input", "gnm", "--input-processing", "none"], ["--n", "0"],
                  ["--bogus"], ["--n", "12x"], ["--input", "forest", "--n", "5", "--lists", "9"]):
         r = run(*args)
         assert r.returncode == 2 and "[error]" in r.stderr, args
