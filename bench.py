@@ -414,6 +414,11 @@ def main():
     }
     if check is not None:
         out["bit_exact_vs_closed_form"] = check
+    # per-level trace of one more call (RulerTrace counterpart)
+    ctx.trace(True)
+    ctx.rank_list(succ, w, root, rank, **cfg)
+    out["ruler_trace"] = ctx.trace_read()
+    ctx.trace(False)
     del succ, w, root, rank
     torch.cuda.empty_cache()
     # random-gather ceiling: dst[i] = src[idx[i]], idx a random permutation of

@@ -84,6 +84,32 @@ kc_status kc_profile_reset(void *ctx);
 uint32_t kc_profile_read(void *ctx, uint32_t max_entries, const char **names, double *ms,
                          uint64_t *counts);
 
+/* Per-level trace of the rankings run while tracing is on (the counterpart of
+ * RulerTrace, lib/kascade/sparse_ruling_set_detail/trace.hpp:14-166):
+ * levels in the order they ran; subproblem 0 = the ruler sublist recursion,
+ * 1 = the unclaimed head prefixes.  For an SRS level, chains are the
+ * sublists the rulers chase (the ruler included, the stopping element
+ * excluded); the longest chain is the GPU's counterpart of the reference's
+ * ruler_chasing_rounds.  kc_trace_enable(ctx, 1) clears and starts;
+ * kc_trace_read returns the number of entries and fills up to max_entries. */
+typedef enum kc_trace_kind { KC_TRACE_SRS = 0, KC_TRACE_PD = 1, KC_TRACE_SMALL = 2 } kc_trace_kind;
+typedef struct kc_trace_level {
+  int32_t level;
+  int32_t kind;       /* kc_trace_kind */
+  int32_t subproblem; /* 0 ruler sublist, 1 unclaimed prefixes */
+  uint32_t rounds;    /* pointer doubling rounds (KC_TRACE_PD) */
+  uint64_t n;         /* elements at this level (local_subproblem_size) */
+  uint64_t spacing;   /* elements per ruler (KC_TRACE_SRS) */
+  uint64_t rulers;    /* rulers picked (local_num_ruler) */
+  uint64_t tail_rulers; /* list tails reached by chases, ranked as sublists of their own */
+  uint64_t unreached; /* head-prefix elements no ruler reached (num_unreached) */
+  uint64_t chain_length_min;
+  uint64_t chain_length_max;
+  double chain_length_avg;
+} kc_trace_level;
+kc_status kc_trace_enable(void *ctx, int on);
+uint32_t kc_trace_read(void *ctx, kc_trace_level *out, uint32_t max_entries);
+
 /* Per-size-class tuning (the autotuner's table): rows of 5 uint32
  * {log2(n) class, ruler spacing top, ruler spacing deep, chase workgroups per
  * CU, rulers per work-counter atomic}; a size uses the row of the largest

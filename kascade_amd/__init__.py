@@ -54,6 +54,17 @@ class ServerCfg(ctypes.Structure):
                 ("one_job_per_launch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
+class TraceLevel(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int32), ("kind", ctypes.c_int32), ("subproblem", ctypes.c_int32),
+                ("rounds", ctypes.c_uint32), ("n", ctypes.c_uint64), ("spacing", ctypes.c_uint64),
+                ("rulers", ctypes.c_uint64), ("tail_rulers", ctypes.c_uint64), ("unreached", ctypes.c_uint64),
+                ("chain_length_min", ctypes.c_uint64), ("chain_length_max", ctypes.c_uint64),
+                ("chain_length_avg", ctypes.c_double)]
+
+
+TRACE_KINDS = ("srs", "pd", "small")
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("num_trees", ctypes.c_uint64), ("max_rank", ctypes.c_int64), ("avg_rank", ctypes.c_double),
                 ("num_nontrivial_trees", ctypes.c_uint64), ("nontrivial_size_sum", ctypes.c_uint64),
@@ -124,6 +135,8 @@ _SIGS = {
     "kc_gen_graph": (ctypes.c_int, [_P, ctypes.c_int32, _U64, _U64, _U64, _P, _P, _P]),
     "kc_build_csr": (ctypes.c_int, [_P, _P, _P, _U64, _U64, _P, _P, _P]),
     "kc_bfs_forest": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P]),
+    "kc_trace_enable": (ctypes.c_int, [_P, ctypes.c_int]),
+    "kc_trace_read": (ctypes.c_uint32, [_P, ctypes.POINTER(TraceLevel), ctypes.c_uint32]),
     "kc_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerCfg), ctypes.POINTER(_P)]),
     "kc_server_destroy": (None, [_P]),
     "kc_submit": (ctypes.c_int, [_P, ctypes.POINTER(Job), ctypes.POINTER(_U64)]),
@@ -268,6 +281,21 @@ class Context:
                                                            _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
                                                            stream))
         return root_out, rank_out
+
+    # ---- per-level trace (RulerTrace) --------------------------------------------
+    def trace(self, on=True):
+        self._check(self._lib.kc_trace_enable(self.handle, int(bool(on))))
+
+    def trace_read(self):
+        k = self._lib.kc_trace_read(self.handle, None, 0)
+        arr = (TraceLevel * max(1, k))()
+        self._lib.kc_trace_read(self.handle, arr, k)
+        out = []
+        for t in arr[:k]:
+            d = {f: getattr(t, f) for f, _ in TraceLevel._fields_}
+            d["kind"] = TRACE_KINDS[t.kind]
+            out.append(d)
+        return out
 
     # ---- successor utilities and statistics -------------------------------------
     def is_list(self, succ, stream=None):

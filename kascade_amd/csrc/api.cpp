@@ -652,6 +652,22 @@ kc_status kc_bfs_forest(void* vctx, const uint64_t* xadj, const uint64_t* adj, u
   });
 }
 
+kc_status kc_trace_enable(void* vctx, int on) {
+  return guarded(vctx, [&](Context* ctx) {
+    ctx->trace_on = on != 0;
+    ctx->trace_subproblem = 0;
+    ctx->trace.clear();
+  });
+}
+
+uint32_t kc_trace_read(void* vctx, kc_trace_level* out, uint32_t max_entries) {
+  auto* ctx = static_cast<Context*>(vctx);
+  if (!ctx) return 0;
+  const uint32_t k = static_cast<uint32_t>(ctx->trace.size());
+  for (uint32_t i = 0; i < k && i < max_entries && out; i++) out[i] = ctx->trace[i];
+  return k;
+}
+
 kc_status kc_pack(void* vctx, const uint64_t* succ, const int64_t* rank, uint64_t n,
                   const uint64_t* active, uint64_t nactive, uint64_t* psucc_out,
                   int64_t* prank_out, void* stream) {

@@ -106,6 +106,10 @@ struct Context {
   uint64_t* h_pinned = nullptr;    // pinned host mirror of counters
   std::string last_error;
   int num_cus = 256;
+  // per-level trace (kc_trace_*)
+  bool trace_on = false;
+  int trace_subproblem = 0;
+  std::vector<kc_trace_level> trace;
 };
 
 // Scoped stack allocation from the arena.

@@ -27,6 +27,10 @@ enum Slot : int {
   kSlotExtra = 2,
   kSlotUnclaimed = 3,
   kSlotActive = 4,
+  // chase chain lengths: max, max of the complement (= ~min), sum
+  kSlotChainMax = 5,
+  kSlotChainMinC = 6,
+  kSlotChainSum = 7,
   kSlotCount = 8,
 };
 
@@ -75,8 +79,15 @@ __global__ void __launch_bounds__(kThreads)
 chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
-             uint32_t chunk) {
+             uint32_t chunk, unsigned long long* chain) {
   const uint64_t lt = lanemask_lt();
+  // chain lengths of this lane (trace statistics, one reduction per wave at exit)
+  uint64_t steps = 0, len_max = 0, len_minc = 0, len_sum = 0;
+  auto chain_end = [&]() {
+    len_max = steps > len_max ? steps : len_max;
+    len_minc = ~steps > len_minc ? ~steps : len_minc;
+    len_sum += steps;
+  };
   // Wave-uniform chunk of ruler ids [cbase, cend): one atomic per `chunk`
   // rulers, so idle lanes refill every step without touching the counter.
   uint64_t cbase = 0, cend = 0;
@@ -113,7 +124,20 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       }
     }
     if (__ballot(active) == 0) {
-      if (wave_done) break;
+      if (wave_done) {
+        for (int o = 32; o > 0; o >>= 1) {
+          const uint64_t a = __shfl_xor(len_max, o), b = __shfl_xor(len_minc, o);
+          len_max = a > len_max ? a : len_max;
+          len_minc = b > len_minc ? b : len_minc;
+          len_sum += __shfl_xor(len_sum, o);
+        }
+        if (__lane_id() == 0 && len_sum) {
+          atomicMax(&chain[0], static_cast<unsigned long long>(len_max));
+          atomicMax(&chain[1], static_cast<unsigned long long>(len_minc));
+          atomicAdd(&chain[2], static_cast<unsigned long long>(len_sum));
+        }
+        break;
+      }
       continue;
     }
     if (!active) continue;
@@ -123,9 +147,11 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       sub_orig[ruler] = cur;
       rec[cur] = Rec{ruler | kRootFlag, 0};
       first = false;
+      steps = 1;
       if (rr.s == cur) {
         sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
         active = false;
+        chain_end();
       } else {
         dist = rr.w;
         cur = rr.s;
@@ -137,6 +163,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       if (id == ruler) raise_err(err, kErrCycle);  // came back to its own ruler
       sub[ruler] = Rec{id, dist};
       active = false;
+      chain_end();
       continue;
     }
     const Rec r = rec[cur];
@@ -154,10 +181,12 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
         raise_err(err, kErrNotList);
       }
       active = false;
+      chain_end();
     } else {
       rec[cur] = Rec{ruler | kRootFlag, dist};
       dist += r.w;
       cur = r.s;
+      steps++;
     }
   }
 }
@@ -457,8 +486,8 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
                        int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                        int algorithm, int level, hipStream_t st);
 
-void pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                     int64_t* rank, OutMap om, hipStream_t st) {
+uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                         int64_t* rank, OutMap om, hipStream_t st) {
   Phase ph(ctx, "pointer_doubling", st);
   Scratch scratch(ctx);
   Rec* other = scratch.get<Rec>(n);
@@ -486,6 +515,7 @@ void pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_
   const uint64_t n_sel = std::min<uint64_t>(n, n_out << om.shift);
   hipLaunchKernelGGL(pd_write_kernel, dim3(blocks_for(n_sel, kThreads)), dim3(kThreads), 0, st, a,
                      n_sel, n_out, root, rank, om);
+  return rounds;
 }
 
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
@@ -518,13 +548,31 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(chase_kernel, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed,
                        nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
-                       tune.chase_chunk);
+                       tune.chase_chunk, ctr + kSlotChainMax);
   }
   uint64_t extra = 0;
   read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
   const uint64_t m = nR + extra;
+  size_t trace_at = 0;
+  if (ctx->trace_on) {
+    // the counter block was copied to the pinned mirror by read_counters
+    kc_trace_level t{};
+    t.level = level;
+    t.kind = KC_TRACE_SRS;
+    t.subproblem = ctx->trace_subproblem;
+    t.n = n;
+    t.spacing = spacing;
+    t.rulers = nR;
+    t.tail_rulers = extra;
+    t.chain_length_max = ctx->h_pinned[kSlotChainMax];
+    t.chain_length_min = nR ? ~ctx->h_pinned[kSlotChainMinC] : 0;
+    t.chain_length_avg = nR ? static_cast<double>(ctx->h_pinned[kSlotChainSum]) / static_cast<double>(nR) : 0;
+    trace_at = ctx->trace.size();
+    ctx->trace.push_back(t);
+  }
 
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
+  const int role = ctx->trace_subproblem;  // the sublist recursion inherits it
   rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, OutMap{2, 0}, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
                     level + 1, st);
   int64_t* T8 = scratch.get<int64_t>(m);
@@ -548,6 +596,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
+  if (ctx->trace_on) ctx->trace[trace_at].unreached = nu;
   if (nu == 0) return;
   if (nu > ucap) {
     uint64_t* ids2 = scratch.get<uint64_t>(nu);
@@ -567,9 +616,11 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      rec, u_ids, u_orig, nu, T, urec, ext);
   // every unclaimed prefix belongs to a different list of this level
+  ctx->trace_subproblem = 1;
   rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, std::min(nu, max_tails), tune,
                     nu > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1,
                     st);
+  ctx->trace_subproblem = role;
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      u_ids, nu, n_out, ures, ext, root, rank, om);
 }
@@ -579,6 +630,14 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
                        int algorithm, int level, hipStream_t st) {
   if (n == 0) return;
   if (n <= tune.small_n && n <= kSmallMax) {
+    if (ctx->trace_on) {
+      kc_trace_level t{};
+      t.level = level;
+      t.kind = KC_TRACE_SMALL;
+      t.subproblem = ctx->trace_subproblem;
+      t.n = n;
+      ctx->trace.push_back(t);
+    }
     Phase ph(ctx, "small_rank", st);
     hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec,
                        static_cast<uint32_t>(n), static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om,
@@ -586,7 +645,16 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
     return;
   }
   if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
-    pd_records_impl(ctx, rec, n, n_out, root, rank, om, st);
+    const uint32_t rounds = pd_records_impl(ctx, rec, n, n_out, root, rank, om, st);
+    if (ctx->trace_on) {
+      kc_trace_level t{};
+      t.level = level;
+      t.kind = KC_TRACE_PD;
+      t.subproblem = ctx->trace_subproblem;
+      t.n = n;
+      t.rounds = rounds;
+      ctx->trace.push_back(t);
+    }
     return;
   }
   srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st);

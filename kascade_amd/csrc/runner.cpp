@@ -540,6 +540,7 @@ int main(int argc, char** argv) {
     add_time(timer, {"root", "ingest_graph"}, since(t0));
     d.check(kc_profile_enable(d.ctx, 1), "kc_profile_enable");
     d.check(kc_profile_reset(d.ctx), "kc_profile_reset");
+    d.check(kc_trace_enable(d.ctx, 1), "kc_trace_enable");
     t0 = Clock::now();
     rank_in_place(d, c.algorithm, c, d_root, d_rank, n);
     hip_check(hipDeviceSynchronize(), "sync");
@@ -550,6 +551,32 @@ int main(int argc, char** argv) {
     const uint32_t np = kc_profile_read(d.ctx, 64, names, ms, counts);
     for (uint32_t k = 0; k < np && k < 64; k++) add_time(timer, phase_path(names[k]), ms[k] * 1e-3);
     d.check(kc_profile_enable(d.ctx, 0), "kc_profile_enable");
+    // RulerTrace counters (trace.hpp:31-119), one value per ruling-set level
+    Json counter = Json::object();
+    {
+      std::vector<kc_trace_level> tr(kc_trace_read(d.ctx, nullptr, 0));
+      kc_trace_read(d.ctx, tr.data(), static_cast<uint32_t>(tr.size()));
+      d.check(kc_trace_enable(d.ctx, 0), "kc_trace_enable");
+      auto add = [&](const char* name, std::initializer_list<const char*> modes, Json v) {
+        for (const char* mode : modes) {
+          Json& a = counter["root"][name][mode];
+          if (a.kind != Json::Arr) a = Json::array();
+          a.push(v);
+        }
+      };
+      for (const auto& t : tr) {
+        if (t.kind == KC_TRACE_PD) add("pointer_doubling_rounds", {"max"}, Json::uinteger(t.rounds));
+        if (t.kind != KC_TRACE_SRS) continue;
+        add("ruler_list_length_min", {"min"}, Json::uinteger(t.chain_length_min));
+        add("ruler_list_length_max", {"max"}, Json::uinteger(t.chain_length_max));
+        add("ruler_list_length_avg", {"max"}, Json::integer(static_cast<int64_t>(t.chain_length_avg)));
+        add("local_num_ruler", {"max", "min"}, Json::uinteger(t.rulers));
+        add("local_subproblem_size", {"sum", "max", "min"}, Json::uinteger(t.rulers + t.tail_rulers));
+        add("ruler_chasing_rounds", {"max", "min"}, Json::uinteger(t.chain_length_max));
+        add("num_unreached", {"sum", "max", "min"}, Json::uinteger(t.unreached));
+        add("num_tail_rulers", {"sum"}, Json::uinteger(t.tail_rulers));
+      }
+    }
     hip_check(hipMemcpy(root.data(), d_root, n * 8, hipMemcpyDeviceToHost), "copy");
     hip_check(hipMemcpy(rank.data(), d_rank, n * 8, hipMemcpyDeviceToHost), "copy");
     std::fprintf(stderr, "[info] Finished run %llu in %.6f secs.\n", static_cast<unsigned long long>(it),
@@ -558,7 +585,7 @@ int main(int argc, char** argv) {
     if (!stats.has("input_stats"))  // Report::push_stats keeps the first value
       stats["input_stats"] = stats_json(d, c, d_root, d_rank, n);
     timers.push(std::move(timer));
-    counters.push(Json::object());
+    counters.push(std::move(counter));
   }
 
   hipDeviceProp_t prop{};
