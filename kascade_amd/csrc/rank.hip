@@ -75,6 +75,8 @@ pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, ui
 // sublist link (ruler -> next ruler, gap) is the reversed-orientation message
 // of work_on_item :242-251, written directly in sublist ids.
 // ---------------------------------------------------------------------------
+// kTrace: also count chain lengths (the per-level trace); off in plain calls.
+template <bool kTrace>
 __global__ void __launch_bounds__(kThreads)
 chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
@@ -84,9 +86,11 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
   // chain lengths of this lane (trace statistics, one reduction per wave at exit)
   uint64_t steps = 0, len_max = 0, len_minc = 0, len_sum = 0;
   auto chain_end = [&]() {
-    len_max = steps > len_max ? steps : len_max;
-    len_minc = ~steps > len_minc ? ~steps : len_minc;
-    len_sum += steps;
+    if constexpr (kTrace) {
+      len_max = steps > len_max ? steps : len_max;
+      len_minc = ~steps > len_minc ? ~steps : len_minc;
+      len_sum += steps;
+    }
   };
   // Wave-uniform chunk of ruler ids [cbase, cend): one atomic per `chunk`
   // rulers, so idle lanes refill every step without touching the counter.
@@ -125,6 +129,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
     }
     if (__ballot(active) == 0) {
       if (wave_done) {
+        if constexpr (kTrace) {
         for (int o = 32; o > 0; o >>= 1) {
           const uint64_t a = __shfl_xor(len_max, o), b = __shfl_xor(len_minc, o);
           len_max = a > len_max ? a : len_max;
@@ -135,6 +140,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
           atomicMax(&chain[0], static_cast<unsigned long long>(len_max));
           atomicMax(&chain[1], static_cast<unsigned long long>(len_minc));
           atomicAdd(&chain[2], static_cast<unsigned long long>(len_sum));
+        }
         }
         break;
       }
@@ -147,7 +153,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       sub_orig[ruler] = cur;
       rec[cur] = Rec{ruler | kRootFlag, 0};
       first = false;
-      steps = 1;
+      if constexpr (kTrace) steps = 1;
       if (rr.s == cur) {
         sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
         active = false;
@@ -186,7 +192,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       rec[cur] = Rec{ruler | kRootFlag, dist};
       dist += r.w;
       cur = r.s;
-      steps++;
+      if constexpr (kTrace) steps++;
     }
   }
 }
@@ -546,7 +552,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
       std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
   {
     Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(chase_kernel, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed,
+    hipLaunchKernelGGL(ctx->trace_on ? chase_kernel<true> : chase_kernel<false>, dim3(chase_blocks),
+                       dim3(kThreads), 0, st, rec, n, logK, seed,
                        nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
                        tune.chase_chunk, ctr + kSlotChainMax);
   }
