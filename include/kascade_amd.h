@@ -71,8 +71,14 @@ void kc_context_destroy(void *ctx);
 const char *kc_last_error(void *ctx);
 /* Device memory currently held by the context's scratch pool, in bytes. */
 uint64_t kc_context_scratch_bytes(void *ctx);
-/* Releases the scratch pool. */
+/* Releases the scratch pool (frees its device memory). */
 kc_status kc_context_trim(void *ctx);
+/* kc_context_destroy hands the context's scratch chunks to a per-device
+ * cache (at most a quarter of the device's memory) that later contexts take
+ * from before they allocate, so servers and contexts that come and go do not
+ * allocate and free device memory each time.  This frees that cache.  No
+ * reference counterpart (the reference allocates per call with std::vector). */
+kc_status kc_release_cached_scratch(int device);
 
 /* Per-phase device timing (HIP events on the launching stream), the
  * counterpart of the reference's kamping::measurements::timer phases

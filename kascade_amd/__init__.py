@@ -99,6 +99,7 @@ _SIGS = {
     "kc_last_error": (ctypes.c_char_p, [_P]),
     "kc_context_scratch_bytes": (_U64, [_P]),
     "kc_context_trim": (ctypes.c_int, [_P]),
+    "kc_release_cached_scratch": (ctypes.c_int, [ctypes.c_int]),
     "kc_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "kc_profile_reset": (ctypes.c_int, [_P]),
     "kc_profile_read": (ctypes.c_uint32, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
@@ -529,6 +530,13 @@ class Context:
 
     def trim(self):
         self._check(self._lib.kc_context_trim(self.handle))
+
+
+def release_cached_scratch(device=0):
+    """Frees the scratch chunks that destroyed contexts left in the device's cache."""
+    st = load().kc_release_cached_scratch(device)
+    if st != KC_OK:
+        raise KascadeError(st, "kc_release_cached_scratch")
 
 
 def _alloc_pair(n, device, weights=True):

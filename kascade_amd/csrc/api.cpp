@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
 #include <new>
 #include <string>
@@ -17,11 +20,80 @@
 namespace kc {
 
 // ---------------------------------------------------------------- Arena ---
+namespace {
+
+// Idle chunks per device, largest first to go when the cache is over its cap
+// (a quarter of the device's memory).
+struct ChunkCache {
+  std::mutex mu;
+  std::multimap<size_t, char*> chunks;
+  size_t bytes = 0;
+  size_t cap = 0;
+};
+
+ChunkCache& chunk_cache(int device) {
+  static std::mutex mu;
+  static std::map<int, std::unique_ptr<ChunkCache>> caches;
+  std::lock_guard<std::mutex> lk(mu);
+  auto& c = caches[device];
+  if (!c) {
+    c = std::make_unique<ChunkCache>();
+    size_t free_b = 0, total_b = 0;
+    c->cap = hipMemGetInfo(&free_b, &total_b) == hipSuccess ? total_b / 4 : (size_t{16} << 30);
+  }
+  return *c;
+}
+
+constexpr size_t kMinChunk = size_t{64} << 20;
+
+size_t round_chunk(size_t b) { return (b + kMinChunk - 1) / kMinChunk * kMinChunk; }
+
+}  // namespace
+
 Arena::~Arena() { trim(); }
+
+bool Arena::new_chunk(size_t want) {
+  want = std::max(want, kMinChunk);
+  char* p = nullptr;
+  size_t size = 0;
+  {
+    ChunkCache& c = chunk_cache(device);
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.chunks.lower_bound(want);
+    if (it != c.chunks.end()) {
+      size = it->first;
+      p = it->second;
+      c.bytes -= size;
+      c.chunks.erase(it);
+    }
+  }
+  if (!p) {
+    if (hipMalloc(reinterpret_cast<void**>(&p), want) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    size = want;
+  }
+  chunks_.push_back(Chunk{p, size});
+  held_.fetch_add(size, std::memory_order_relaxed);
+  return true;
+}
+
+// Only with no live block: the previous call is over, but its kernels may
+// still run on any stream, so the device is synchronised before its chunks go.
+void Arena::consolidate() {
+  (void)hipDeviceSynchronize();
+  for (auto& c : chunks_) (void)hipFree(c.base);
+  chunks_.clear();
+  blocks_.clear();
+  held_.store(0, std::memory_order_relaxed);
+  new_chunk(round_chunk(peak_ + peak_ / 8));
+}
 
 void* Arena::alloc(size_t bytes) {
   bytes = (bytes + 255) & ~size_t{255};
   if (bytes == 0) bytes = 256;
+  if (top_ == 0 && chunks_.size() > 1) consolidate();
   // Next free position after the top block.
   size_t chunk = 0, offset = 0;
   if (top_ > 0) {
@@ -33,23 +105,39 @@ void* Arena::alloc(size_t bytes) {
     chunk++;
     offset = 0;
   }
-  if (chunk == chunks_.size()) {
-    size_t want = std::max(bytes, size_t{64} << 20);
-    void* p = nullptr;
-    if (hipMalloc(&p, want) != hipSuccess) {
-      (void)hipGetLastError();
-      return nullptr;
-    }
-    chunks_.push_back(Chunk{static_cast<char*>(p), want});
-    held_.fetch_add(want, std::memory_order_relaxed);
-  }
+  if (chunk == chunks_.size() && !new_chunk(bytes)) return nullptr;
   Block nb{chunk, offset, bytes};
   if (top_ < blocks_.size()) blocks_[top_] = nb; else blocks_.push_back(nb);
   top_++;
+  live_ += bytes;
+  peak_ = std::max(peak_, live_);
   return chunks_[chunk].base + offset;
 }
 
-void Arena::release(size_t mark) { top_ = std::min(top_, mark); }
+void Arena::release(size_t mark) {
+  while (top_ > mark) live_ -= blocks_[--top_].size;
+}
+
+void Arena::recycle() {
+  if (top_ != 0) return;  // never hand out live blocks
+  ChunkCache& c = chunk_cache(device);
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto& ch : chunks_) {
+      c.chunks.emplace(ch.size, ch.base);
+      c.bytes += ch.size;
+    }
+    while (c.bytes > c.cap && !c.chunks.empty()) {
+      auto it = std::prev(c.chunks.end());
+      (void)hipFree(it->second);
+      c.bytes -= it->first;
+      c.chunks.erase(it);
+    }
+  }
+  chunks_.clear();
+  blocks_.clear();
+  held_.store(0, std::memory_order_relaxed);
+}
 
 void Arena::trim() {
   if (top_ != 0) return;  // never free live blocks
@@ -57,6 +145,14 @@ void Arena::trim() {
   chunks_.clear();
   blocks_.clear();
   held_.store(0, std::memory_order_relaxed);
+}
+
+void release_chunk_cache(int device) {
+  ChunkCache& c = chunk_cache(device);
+  std::lock_guard<std::mutex> lk(c.mu);
+  for (auto& ch : c.chunks) (void)hipFree(ch.second);
+  c.chunks.clear();
+  c.bytes = 0;
 }
 
 uint64_t Arena::held() const { return held_.load(std::memory_order_relaxed); }
@@ -275,6 +371,7 @@ kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void** ctx_ou
   auto* ctx = new (std::nothrow) Context();
   if (!ctx) return KC_ENOMEM;
   ctx->device = device;
+  ctx->arena.device = device;
   ctx->budget = mem_budget_bytes;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&ctx->d_counters, kc::kCounterSlots * sizeof(uint64_t)) != hipSuccess ||
@@ -295,9 +392,10 @@ void kc_context_destroy(void* vctx) {
   Context* ctx = static_cast<Context*>(vctx);
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
-  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  // the scratch may have served calls on caller streams too
+  (void)hipDeviceSynchronize();
   ctx->arena.release(0);
-  ctx->arena.trim();
+  ctx->arena.recycle();
   if (ctx->d_counters) (void)hipFree(ctx->d_counters);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -319,6 +417,14 @@ kc_status kc_context_trim(void* vctx) {
     KC_HIP(hipStreamSynchronize(ctx->stream));
     ctx->arena.trim();
   });
+}
+
+kc_status kc_release_cached_scratch(int device) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return KC_EDEVICE;
+  if (hipSetDevice(device) != hipSuccess) return KC_EDEVICE;
+  kc::release_chunk_cache(device);
+  return KC_OK;
 }
 
 kc_status kc_rank_list(void* vctx, const uint64_t* succ, const int64_t* weight, uint64_t n,

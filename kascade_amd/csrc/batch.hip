@@ -100,16 +100,27 @@ batch_tree_keys_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
 __global__ void __launch_bounds__(kThreads)
 batch_tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
                        const uint8_t* __restrict__ is_root, uint64_t nt, uint64_t nl,
-                       Rec* __restrict__ rec) {
+                       Rec* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (k >= nt) return;
-  const uint32_t p = keys[k];
-  if (p == nt) return;
-  const uint32_t c = vals[k];
-  const bool first = (k == 0) || keys[k - 1] != p;
-  const bool last = (k + 1 == nt) || keys[k + 1] != p;
-  rec[nl + 2 * uint64_t{c}] = Rec{last ? nl + 2 * uint64_t{p} : nl + 2 * uint64_t{vals[k + 1]} + 1, 1};
-  if (first && !is_root[p]) rec[nl + 2 * uint64_t{p} + 1] = Rec{nl + 2 * uint64_t{c} + 1, -1};
+  bool head = false;
+  uint32_t c = 0;
+  if (k < nt) {
+    const uint32_t p = keys[k];
+    if (p != nt) {
+      c = vals[k];
+      const bool first = (k == 0) || keys[k - 1] != p;
+      const bool last = (k + 1 == nt) || keys[k + 1] != p;
+      rec[nl + 2 * uint64_t{c}] = Rec{last ? nl + 2 * uint64_t{p} : nl + 2 * uint64_t{vals[k + 1]} + 1, 1};
+      if (first) {
+        if (!is_root[p])
+          rec[nl + 2 * uint64_t{p} + 1] = Rec{nl + 2 * uint64_t{c} + 1, -1};
+        else
+          head = true;  // the root's tour starts at this down edge
+      }
+    }
+  }
+  const uint64_t h = wave_append(nheads, head);
+  if (head) heads[h] = nl + 2 * uint64_t{c} + 1;
 }
 
 // Linked tour build for a batch of trees (child lists by atomic exchange;
@@ -138,27 +149,33 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
 __global__ void __launch_bounds__(kThreads)
 batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
                         const uint32_t* __restrict__ head, const uint32_t* __restrict__ sib,
-                        Rec* __restrict__ rec) {
+                        Rec* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
   __shared__ uint32_t s_range[2];
   const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
   block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
   const uint64_t t = first + threadIdx.x;
-  if (t >= nt) return;
-  const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
-  const uint64_t i = t - j.base;
-  const uint64_t p = j.succ[i];
-  Rec up, down;
-  if (p == i || p >= j.n) {
-    up = Rec{nl + 2 * t, 0};
-    down = Rec{nl + 2 * t + 1, 0};
-  } else {
-    const uint32_t s = sib[t];
-    const uint32_t h = head[t];
-    up = Rec{s != 0xFFFFFFFFu ? nl + 2 * uint64_t{s} + 1 : nl + 2 * (j.base + p), 1};
-    down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
+  bool tour_head = false;
+  uint32_t h = 0xFFFFFFFFu;
+  if (t < nt) {
+    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
+    const uint64_t i = t - j.base;
+    const uint64_t p = j.succ[i];
+    h = head[t];
+    Rec up, down;
+    if (p == i || p >= j.n) {
+      up = Rec{nl + 2 * t, 0};
+      down = Rec{nl + 2 * t + 1, 0};
+      tour_head = h != 0xFFFFFFFFu;
+    } else {
+      const uint32_t s = sib[t];
+      up = Rec{s != 0xFFFFFFFFu ? nl + 2 * uint64_t{s} + 1 : nl + 2 * (j.base + p), 1};
+      down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
+    }
+    rec[nl + 2 * t] = up;
+    rec[nl + 2 * t + 1] = down;
   }
-  rec[nl + 2 * t] = up;
-  rec[nl + 2 * t + 1] = down;
+  const uint64_t k = wave_append(nheads, tour_head);
+  if (tour_head) heads[k] = nl + 2 * uint64_t{h} + 1;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
@@ -223,7 +240,8 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   if (d_trees)
     KC_HIP(hipMemcpyAsync(d_trees, trees.data(), trees.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st));
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
-  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
+  uint64_t* heads = nt ? scratch.get<uint64_t>(nt) : nullptr;
   if (nl) {
     Phase ph(ctx, "batch_pack", st);
     hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st,
@@ -240,7 +258,7 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
     hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
                        d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
     hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec);
+                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
   } else if (nt) {
     Phase ph(ctx, "batch_tour", st);
     keys = scratch.get<uint32_t>(nt);
@@ -251,16 +269,18 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
                        ctx->d_err, ctr + 1);
     radix_sort_pairs_u32(ctx, keys, vals, nt, bits_for(nt), st);
     hipLaunchKernelGGL(batch_tour_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, keys,
-                       vals, is_root, nt, nl, rec);
+                       vals, is_root, nt, nl, rec, heads, ctr + 2);
   }
-  uint64_t counts[2] = {0, 0};
-  read_counters(ctx, st, 0, 2, counts);
+  uint64_t counts[3] = {0, 0, 0};
+  read_counters(ctx, st, 0, 3, counts);
   // size class of the whole batch; the ruler spacing per level is capped by
   // the mean list length inside the ranking
   RankTuning tune = tuning_for(ctx, total, cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
-  rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st);
+  const TourRegion tr{nl, heads, counts[2]};
+  rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st, 0,
+               nt ? &tr : nullptr);
   check_err(ctx, st);
   Phase ph(ctx, "batch_scatter", st);
   if (nl)

@@ -32,16 +32,25 @@ enum ErrBits : uint32_t {
   kErrCycle = 8u,      // a cycle without a tail
 };
 
-// Scratch pool: grow-only chunks reused across calls, handed out as a stack.
+// Scratch pool: chunks reused across calls, handed out as a stack.  When a
+// call needed more than one chunk, the next call starts by replacing them with
+// one chunk of the largest size any call has used, so a steady workload runs
+// from a single chunk and allocates nothing.  Chunks of a destroyed context go
+// to a per-device cache that later contexts take from before they allocate.
 class Arena {
  public:
   ~Arena();
+  int device = 0;
   // Returns nullptr on allocation failure.
   void* alloc(size_t bytes);
   size_t mark() const { return top_; }
   void release(size_t mark);
+  // Hands every chunk to the device's cache (no live blocks; device idle).
+  void recycle();
+  // Frees every chunk (no live blocks).
   void trim();
   uint64_t held() const;
+  uint64_t peak() const { return peak_; }
 
  private:
   struct Chunk {
@@ -53,11 +62,17 @@ class Arena {
     size_t offset;
     size_t size;
   };
+  bool new_chunk(size_t want);
+  void consolidate();
   std::vector<Chunk> chunks_;
   std::vector<Block> blocks_;
-  size_t top_ = 0;  // number of live blocks
+  size_t top_ = 0;   // number of live blocks
+  size_t live_ = 0;  // bytes in live blocks
+  size_t peak_ = 0;  // most bytes live at once, over the arena's life
   std::atomic<uint64_t> held_{0};  // readable from other threads (server statistics)
 };
+// Frees the chunks cached for `device` (kc_release_cached_scratch).
+void release_chunk_cache(int device);
 
 // Per-phase device timing with HIP events on the launching stream (the
 // library's counterpart of the reference's kamping phase timers).
@@ -214,12 +229,25 @@ __host__ __device__ inline bool out_selected(const OutMap& om, uint64_t e, uint6
 __host__ __device__ inline uint64_t out_pos(const OutMap& om, uint64_t e) {
   return (e >> om.shift) * om.stride;
 }
+// Euler tours inside a ranking: elements [lo, n) are interleaved tour pairs
+// (lo + 2t = upward edge / dummy, lo + 2t + 1 = downward edge).  Only the
+// upward edges' results are ever read, and the head of every tour with a
+// root is known from the build (`heads`, nheads device ids).  The sparse
+// ruling set then starts a chase at every head (no unreached prefixes in the
+// tours) and writes no claim into downward edges; an unreached upward edge
+// is either an isolated root (a self loop: its own root, rank 0) or lies on a
+// cycle (an error).
+struct TourRegion {
+  uint64_t lo;
+  const uint64_t* heads;
+  uint64_t nheads;
+};
 // Ranks packed records (ids < n, consumed) into root (ids) / rank; outputs as
 // `out_shift` selects (see OutMap; n_out outputs).  max_tails bounds the
 // number of lists.
 void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st, uint32_t out_shift = 0);
+                  hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr);
 void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
 // Packs succ/weight into records with validation (range, tail weight).

@@ -56,18 +56,31 @@ tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __re
 
 __global__ void __launch_bounds__(kThreads)
 tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                 const uint64_t* __restrict__ parent, uint64_t n, Rec* __restrict__ tour) {
+                 const uint64_t* __restrict__ parent, uint64_t n, Rec* __restrict__ tour,
+                 uint64_t* __restrict__ heads, unsigned long long* nheads) {
   uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t p = keys[j];
-  if (p == n) return;  // roots sort last
-  const uint32_t c = vals[j];
-  const bool first = (j == 0) || keys[j - 1] != p;
-  const bool last = (j + 1 == n) || keys[j + 1] != p;
-  // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
-  tour[2 * uint64_t{c}] = Rec{last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1};
-  // down(p) -> down(first child); a root p has no down edge
-  if (first && parent[p] != p) tour[2 * uint64_t{p} + 1] = Rec{2 * uint64_t{c} + 1, -1};
+  bool head = false;
+  uint32_t c = 0;
+  if (j < n) {
+    const uint32_t p = keys[j];
+    if (p != n) {  // roots sort last
+      c = vals[j];
+      const bool first = (j == 0) || keys[j - 1] != p;
+      const bool last = (j + 1 == n) || keys[j + 1] != p;
+      // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
+      tour[2 * uint64_t{c}] = Rec{last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1};
+      // down(p) -> down(first child); a root p has no down edge, and its
+      // first child's down edge is where its tour starts
+      if (first) {
+        if (parent[p] != p)
+          tour[2 * uint64_t{p} + 1] = Rec{2 * uint64_t{c} + 1, -1};
+        else
+          head = true;
+      }
+    }
+  }
+  const uint64_t k = wave_append(nheads, head);
+  if (head) heads[k] = 2 * uint64_t{c} + 1;
 }
 
 // Linked build: children lists by atomic exchange on the parent's head slot
@@ -94,22 +107,29 @@ tour_link_atomic_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_
 // and for a root the terminal dummy plus the unused slot.
 __global__ void __launch_bounds__(kThreads)
 tour_pairs_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_t* __restrict__ head,
-                  const uint32_t* __restrict__ sib, Rec* __restrict__ tour) {
+                  const uint32_t* __restrict__ sib, Rec* __restrict__ tour,
+                  uint64_t* __restrict__ heads, unsigned long long* nheads) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (v >= n) return;
-  const uint64_t p = parent[v];
-  Rec up, down;
-  if (p == v || p >= n) {
-    up = Rec{2 * v, 0};
-    down = Rec{2 * v + 1, 0};
-  } else {
-    const uint32_t s = sib[v];
-    const uint32_t h = head[v];
-    up = Rec{s != 0xFFFFFFFFu ? 2 * uint64_t{s} + 1 : 2 * p, 1};
-    down = Rec{h != 0xFFFFFFFFu ? 2 * uint64_t{h} + 1 : 2 * v, -1};
+  bool tour_head = false;
+  uint32_t h = 0xFFFFFFFFu;
+  if (v < n) {
+    const uint64_t p = parent[v];
+    h = head[v];
+    Rec up, down;
+    if (p == v || p >= n) {
+      up = Rec{2 * v, 0};
+      down = Rec{2 * v + 1, 0};
+      tour_head = h != 0xFFFFFFFFu;  // a root's tour starts at its head child's down edge
+    } else {
+      const uint32_t s = sib[v];
+      up = Rec{s != 0xFFFFFFFFu ? 2 * uint64_t{s} + 1 : 2 * p, 1};
+      down = Rec{h != 0xFFFFFFFFu ? 2 * uint64_t{h} + 1 : 2 * v, -1};
+    }
+    tour[2 * v] = up;
+    tour[2 * v + 1] = down;
   }
-  tour[2 * v] = up;
-  tour[2 * v + 1] = down;
+  const uint64_t k = wave_append(nheads, tour_head);
+  if (tour_head) heads[k] = 2 * uint64_t{h} + 1;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
@@ -130,44 +150,49 @@ unsigned bits_for(uint64_t x) {
 // the tour is the same on every run; otherwise children are linked by atomic
 // exchange (faster; the tour's child order may differ between runs, the ranks
 // never do).
+// heads (capacity n) receives the element where each rooted tour starts (the
+// down edge of the root's first child); *nheads their number.
 static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour, bool sorted,
-                           hipStream_t st) {
+                           uint64_t* heads, uint64_t* nheads, hipStream_t st) {
   if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree rooting supports n < 2^32 - 1"};
   Phase ph(ctx, "tour_build", st);
   Scratch scratch(ctx);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  uint64_t counts[2] = {0, 0};
   if (!sorted) {
     uint32_t* head = scratch.get<uint32_t>(n);
     uint32_t* sib = scratch.get<uint32_t>(n);
-    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, head, n,
                        0xFFFFFFFFu);
     hipLaunchKernelGGL(tour_link_atomic_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
-                       parent, n, head, sib, ctx->d_err,
-                       reinterpret_cast<unsigned long long*>(ctx->d_counters));
+                       parent, n, head, sib, ctx->d_err, ctr);
     hipLaunchKernelGGL(tour_pairs_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
-                       head, sib, tour);
-    uint64_t roots = 0;
-    read_counters(ctx, st, 0, 1, &roots);
-    return roots;
+                       head, sib, tour, heads, ctr + 1);
+    read_counters(ctx, st, 0, 2, counts);
+    *nheads = counts[1];
+    return counts[0];
   }
   uint32_t* keys = scratch.get<uint32_t>(n);
   uint32_t* vals = scratch.get<uint32_t>(n);
-  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
   hipLaunchKernelGGL(tree_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent,
-                     n, keys, vals, tour, ctx->d_err,
-                     reinterpret_cast<unsigned long long*>(ctx->d_counters));
-  uint64_t roots = 0;
-  read_counters(ctx, st, 0, 1, &roots);
+                     n, keys, vals, tour, ctx->d_err, ctr);
   radix_sort_pairs_u32(ctx, keys, vals, n, bits_for(n), st);
   hipLaunchKernelGGL(tour_link_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, keys,
-                     vals, parent, n, tour);
-  return roots;
+                     vals, parent, n, tour, heads, ctr + 1);
+  read_counters(ctx, st, 0, 2, counts);
+  *nheads = counts[1];
+  return counts[0];
 }
 
 void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
                        hipStream_t st) {
   if (n == 0) return;
-  build_tour(ctx, parent, n, tour, true, st);
+  Scratch scratch(ctx);
+  uint64_t* heads = scratch.get<uint64_t>(n);
+  uint64_t nheads = 0;
+  build_tour(ctx, parent, n, tour, true, heads, &nheads, st);
 }
 
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
@@ -175,8 +200,10 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   if (n == 0) return;
   Scratch scratch(ctx);
   Rec* tour = scratch.get<Rec>(2 * n);
+  uint64_t* heads = scratch.get<uint64_t>(n);
+  uint64_t nheads = 0;
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
-  const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, st);
+  const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, heads, &nheads, st);
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
   RankTuning tune = tuning_for(ctx, 2 * n, rc);
   int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
@@ -184,7 +211,8 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
     throw KcError{KC_EINVAL, "invalid algorithm"};
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
   // rank_via_euler_tour_select_algorithm (eulertour.cpp:282-321)
-  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st, /*out_shift=*/1);
+  const TourRegion tr{0, heads, nheads};
+  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st, /*out_shift=*/1, &tr);
   check_err(ctx, st);
 }
 

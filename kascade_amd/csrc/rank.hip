@@ -79,10 +79,15 @@ pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, ui
 template <bool kTrace>
 __global__ void __launch_bounds__(kThreads)
 chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
+             const uint64_t* __restrict__ forced, uint64_t nF, uint64_t tour_lo,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
              uint32_t chunk, unsigned long long* chain) {
   const uint64_t lt = lanemask_lt();
+  // rulers [0, nR) sit at computed places; [nR, nRF) start at the tour heads
+  const uint64_t nRF = nR + nF;
+  // downward tour edges carry no claim (their results are never read)
+  auto claims = [&](uint64_t e) { return e < tour_lo || ((e - tour_lo) & 1) == 0; };
   // chain lengths of this lane (trace statistics, one reduction per wave at exit)
   uint64_t steps = 0, len_max = 0, len_minc = 0, len_sum = 0;
   auto chain_end = [&]() {
@@ -107,11 +112,11 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
         unsigned long long b = 0;
         if (__lane_id() == 0) b = atomicAdd(work_ctr, static_cast<unsigned long long>(chunk));
         b = __shfl(b, 0);
-        if (b >= nR) {
+        if (b >= nRF) {
           wave_done = true;
         } else {
           cbase = b;
-          cend = b + chunk < nR ? b + chunk : nR;
+          cend = b + chunk < nRF ? b + chunk : nRF;
         }
       }
       if (!wave_done) {
@@ -119,7 +124,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
         const uint64_t r = __popcll(need & lt);
         if (!active && r < avail) {
           ruler = cbase + r;
-          cur = ruler_of_block(ruler, n, logK, seed);
+          cur = ruler < nR ? ruler_of_block(ruler, n, logK, seed) : forced[ruler - nR];
           first = true;
           active = true;
         }
@@ -148,11 +153,18 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
     }
     if (!active) continue;
     if (first) {
+      first = false;
+      sub_orig[ruler] = cur;
+      if (ruler >= nR && is_ruler(cur, n, logK, seed)) {
+        // a tour head that is also a computed ruler is chased as that one;
+        // this id precedes it in the sublist at distance 0
+        sub[ruler] = Rec{cur >> logK, 0};
+        active = false;
+        continue;
+      }
       // the ruler's own record: claim it with distance 0
       const Rec rr = rec[cur];
-      sub_orig[ruler] = cur;
-      rec[cur] = Rec{ruler | kRootFlag, 0};
-      first = false;
+      if (claims(cur)) rec[cur] = Rec{ruler | kRootFlag, 0};
       if constexpr (kTrace) steps = 1;
       if (rr.s == cur) {
         sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
@@ -177,7 +189,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       raise_err(err, kErrNotList);  // reached twice: in-degree > 1
       active = false;
     } else if (r.s == cur) {
-      const uint64_t j = nR + atomicAdd(extra_ctr, 1ull);
+      const uint64_t j = nRF + atomicAdd(extra_ctr, 1ull);
       if (j < cap) {
         sub[j] = Rec{j, 0};
         sub_orig[j] = cur;
@@ -189,7 +201,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       active = false;
       chain_end();
     } else {
-      rec[cur] = Rec{ruler | kRootFlag, dist};
+      if (claims(cur)) rec[cur] = Rec{ruler | kRootFlag, dist};
       dist += r.w;
       cur = r.s;
       if constexpr (kTrace) steps++;
@@ -229,7 +241,7 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
                  const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
-                 unsigned long long* u_ctr) {
+                 unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   // All rulers in one list (one root): gather only the 8-byte rank, from a
   // table half the size, so more of it stays in L2.
@@ -251,8 +263,18 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
           rank[pos] = t.w - r.w;
         }
       }
-    } else {
+    } else if (v < tour_lo) {
       unclaimed = true;
+    } else if (((v - tour_lo) & 1) == 0) {
+      // an unreached upward edge of a tour: an isolated root, else a cycle
+      if (r.s == v) {
+        if (out_selected(om, v, n_out)) {
+          root[out_pos(om, v)] = v >> om.shift;
+          rank[out_pos(om, v)] = 0;
+        }
+      } else {
+        raise_err(err, kErrCycle);
+      }
     }
   }
   const uint64_t k = wave_append(u_ctr, unclaimed);
@@ -272,7 +294,7 @@ collect_unclaimed_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t* __restrict
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   Rec r{0, 0};
   bool mine = false;
-  if (v < n) {
+  if (v < n) {  // n: the end of the list region (tours are never collected)
     r = rec[v];
     mine = !(r.s & (kRootFlag | kAuxFlag));
   }
@@ -490,7 +512,7 @@ unsigned log2_floor(uint64_t x) {
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                        int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-                       int algorithm, int level, hipStream_t st);
+                       int algorithm, int level, hipStream_t st, const TourRegion* tours = nullptr);
 
 uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                          int64_t* rank, OutMap om, hipStream_t st) {
@@ -526,7 +548,10 @@ uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uin
 
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-               int level, hipStream_t st) {
+               int level, hipStream_t st, const TourRegion* tours) {
+  const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
+  const uint64_t nF = tours ? tours->nheads : 0;
+  const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
   uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
   if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
@@ -538,7 +563,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   }
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
-  const uint64_t cap = nR + std::min<uint64_t>(max_tails, n);
+  const uint64_t cap = nR + nF + std::min<uint64_t>(max_tails, n);
   Rec* sub = scratch.get<Rec>(cap);
   Rec* T = scratch.get<Rec>(cap);
   uint64_t* sub_orig = scratch.get<uint64_t>(cap);
@@ -546,7 +571,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
 
   zero_counters(ctx, st);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
-  const uint64_t lanes_wanted = nR;
+  const uint64_t lanes_wanted = nR + nF;
   const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
   const unsigned chase_blocks = static_cast<unsigned>(
       std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
@@ -554,12 +579,13 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(ctx->trace_on ? chase_kernel<true> : chase_kernel<false>, dim3(chase_blocks),
                        dim3(kThreads), 0, st, rec, n, logK, seed,
-                       nR, cap, sub, sub_orig, ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
+                       nR, tours ? tours->heads : nullptr, nF, tour_lo, cap, sub, sub_orig,
+                       ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
                        tune.chase_chunk, ctr + kSlotChainMax);
   }
   uint64_t extra = 0;
   read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
-  const uint64_t m = nR + extra;
+  const uint64_t m = nR + nF + extra;
   size_t trace_at = 0;
   if (ctx->trace_on) {
     // the counter block was copied to the pinned mirror by read_counters
@@ -569,11 +595,11 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     t.subproblem = ctx->trace_subproblem;
     t.n = n;
     t.spacing = spacing;
-    t.rulers = nR;
+    t.rulers = nR + nF;
     t.tail_rulers = extra;
     t.chain_length_max = ctx->h_pinned[kSlotChainMax];
-    t.chain_length_min = nR ? ~ctx->h_pinned[kSlotChainMinC] : 0;
-    t.chain_length_avg = nR ? static_cast<double>(ctx->h_pinned[kSlotChainSum]) / static_cast<double>(nR) : 0;
+    t.chain_length_min = nR + nF ? ~ctx->h_pinned[kSlotChainMinC] : 0;
+    t.chain_length_avg = nR + nF ? static_cast<double>(ctx->h_pinned[kSlotChainSum]) / static_cast<double>(nR + nF) : 0;
     trace_at = ctx->trace.size();
     ctx->trace.push_back(t);
   }
@@ -592,14 +618,14 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
   // unclaimed elements are list prefixes: few unless the lists are short
-  const uint64_t ucap = std::min<uint64_t>(n, n / 8 + 65536);
+  const uint64_t ucap = std::min<uint64_t>(list_end, list_end / 8 + 65536);
   uint64_t* u_ids = scratch.get<uint64_t>(ucap);
   Rec* u_orig = scratch.get<Rec>(ucap);
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
                        n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
-                       ctr + kSlotUnclaimed);
+                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
@@ -611,8 +637,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     KC_HIP(hipMemcpyAsync(ids2, u_ids, ucap * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
     KC_HIP(hipMemcpyAsync(orig2, u_orig, ucap * sizeof(Rec), hipMemcpyDeviceToDevice, st));
     zero_counters(ctx, st);
-    hipLaunchKernelGGL(collect_unclaimed_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
-                       rec, n, ids2, orig2, ucap, nu, ctr + kSlotUnclaimed);
+    hipLaunchKernelGGL(collect_unclaimed_kernel, dim3(blocks_for(list_end, kThreads)), dim3(kThreads), 0, st,
+                       rec, list_end, ids2, orig2, ucap, nu, ctr + kSlotUnclaimed);
     u_ids = ids2;
     u_orig = orig2;
   }
@@ -634,7 +660,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                        int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-                       int algorithm, int level, hipStream_t st) {
+                       int algorithm, int level, hipStream_t st, const TourRegion* tours) {
   if (n == 0) return;
   if (n <= tune.small_n && n <= kSmallMax) {
     if (ctx->trace_on) {
@@ -664,7 +690,8 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
     }
     return;
   }
-  srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st);
+  srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st,
+            tours && tours->lo < n ? tours : nullptr);
 }
 
 }  // namespace
@@ -730,9 +757,9 @@ void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t
 
 void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st, uint32_t out_shift) {
+                  hipStream_t st, uint32_t out_shift, const TourRegion* tours) {
   rank_records_impl(ctx, rec, n, n_out, root, rank, OutMap{1, out_shift}, max_tails, tune, algorithm, 0,
-                    st);
+                    st, tours);
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,

@@ -115,3 +115,58 @@ def test_server_error_isolation(ctx):
         sv.wait(bad[0])
     assert e.value.status == kascade_amd.KC_EINVAL
     sv.close()
+
+
+def test_server_cyclic_tree_isolated(ctx):
+    """A rootless 'tree' inside a coalesced batch fails alone with KC_ECYCLE;
+    the trees and lists batched with it are still exact."""
+    import kascade_amd
+    say("server cyclic tree")
+    sv = kascade_amd.Server(ctx, num_slots=1, max_batch_jobs=64)
+    good = make_jobs(12, 11, 6, 12)
+    subs = []
+    for i, (kind, succ, w) in enumerate(good):
+        subs.append(sv.submit(kind, succ.copy(), w.copy() if w is not None else None))
+        if i == 5:
+            cyc = np.roll(np.arange(700, dtype=np.uint64), 1)
+            bad = sv.submit("tree", cyc)
+    for (kind, succ, w), (t, ro, ra) in zip(good, subs):
+        sv.wait(t)
+        cr, ck = (o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ))
+        assert (ro == cr).all() and (ra == ck).all()
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        sv.wait(bad[0])
+    assert e.value.status == kascade_amd.KC_ECYCLE
+    sv.close()
+
+
+def test_scratch_reuse_across_contexts(ctx):
+    """A steady workload runs from one scratch chunk: after the first calls
+    the pool stops growing; contexts and servers that come and go reuse the
+    device cache, and every result stays exact."""
+    import torch
+    import kascade_amd
+    say("scratch reuse")
+    jobs = make_jobs(30, 21, 8, 17)
+    held = []
+    for rep in range(3):
+        c = kascade_amd.Context(0)
+        for _ in range(3):
+            for kind, succ, w in jobs[:6]:
+                r, k = (c.rank_list(dev(succ), dev(w)) if kind == "list" else c.root_tree(dev(succ)))
+                cr, ck = (o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ))
+                assert (host(r) == cr).all() and (host(k) == ck).all()
+            held.append(c.scratch_bytes())
+        c.close()
+        sv = kascade_amd.Server(ctx, num_slots=2)
+        subs = [sv.submit(kind, dev(succ), dev(w) if w is not None else None,
+                          torch.empty(len(succ), dtype=torch.uint64, device="cuda"),
+                          torch.empty(len(succ), dtype=torch.int64, device="cuda")) for kind, succ, w in jobs]
+        for (kind, succ, w), (t, ro, ra) in zip(jobs, subs):
+            sv.wait(t)
+            cr, ck = (o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ))
+            assert (host(ro) == cr).all() and (host(ra) == ck).all()
+        sv.close()
+    # within one context the pool is steady after its first pass over the jobs
+    assert held[1] == held[2] and held[4] == held[5] and held[7] == held[8], held
+    kascade_amd.release_cached_scratch(0)

@@ -125,3 +125,58 @@ def test_root_large_device_generated(ctx, n):
     root, depth = ctx.root_tree(parent)
     er, ed = o.tree_closed_form(n, 1234)
     assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+def random_forest(n, trees, seed):
+    """A forest of `trees` random recursive trees of random sizes (plus the
+    isolated roots that size 1 gives), relabelled by a seeded permutation."""
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.choice(np.arange(1, n), size=trees - 1, replace=False))
+    starts = np.concatenate([[0], cuts])
+    parent = np.arange(n, dtype=np.uint64)
+    for a, b in zip(starts, np.concatenate([cuts, [n]])):
+        for v in range(a + 1, b):
+            parent[v] = rng.integers(a, v)
+    sigma = rng.permutation(n)
+    out = np.empty(n, np.uint64)
+    out[sigma] = sigma[parent]
+    return out
+
+
+@pytest.mark.parametrize("build", ["linked", "sorted"])
+@pytest.mark.parametrize("spacing", [0, 2, 4, 64])
+@pytest.mark.parametrize("trees", [1, 37, 3000])
+def test_root_random_forest_spacings(ctx, build, spacing, trees):
+    """Every tour starts at a chased head: with small ruler spacings many heads
+    are also computed rulers, and with many trees most tours are short."""
+    say("random forest", build, spacing, trees)
+    parent = random_forest(60000, trees, 100 + trees)
+    root, depth = ctx.root_tree(dev(parent), tour_build=build, ruler_spacing=spacing)
+    er, ed = o.euler_tour_rooting(parent)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+@pytest.mark.parametrize("build", ["linked", "sorted"])
+@pytest.mark.parametrize("cycle", [2, 3, 50, 5000])
+def test_tree_cycle_beside_trees(ctx, build, cycle):
+    """A rootless component (a cycle with trees hanging off it) next to valid
+    trees is reported as a cycle, whether or not a ruler falls on it."""
+    import kascade_amd
+    say("cycle beside trees", build, cycle)
+    n = 40000
+    parent = random_forest(n, 20, 7).astype(np.uint64)
+    rng = np.random.default_rng(cycle)
+    extra = cycle + 300
+    ids = np.arange(n, n + extra, dtype=np.uint64)
+    p2 = np.empty(extra, np.uint64)
+    p2[:cycle] = np.roll(ids[:cycle], 1)                       # the cycle
+    for k in range(cycle, extra):
+        p2[k] = ids[rng.integers(0, k)]                        # hanging trees
+    full = np.concatenate([parent, p2])
+    sigma = rng.permutation(len(full))
+    out = np.empty(len(full), np.uint64)
+    out[sigma] = sigma[full.astype(np.int64)]
+    for spacing in (0, 2, 1024):
+        with pytest.raises(kascade_amd.KascadeError) as e:
+            ctx.root_tree(dev(out), tour_build=build, ruler_spacing=spacing)
+        assert e.value.status == kascade_amd.KC_ECYCLE, (build, cycle, spacing)
