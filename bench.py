@@ -160,8 +160,9 @@ def serve_bench(ctx, jobs, seed, slots):
     Also measured: one job per launch on `slots` concurrent streams, a
     stronger baseline.  Capacities (all jobs submitted at once), then Poisson
     arrivals at 0.8 x the serial capacity through every mode, then the
-    coalescing server at 3 x that offered load.  Every batched result must
-    equal the one-job-per-launch result bit for bit."""
+    coalescing server at 3 x that offered load.  Each server is measured at
+    steady state, after one untimed pass over the pool.  Every batched result
+    must equal the one-job-per-launch result bit for bit."""
     import numpy as np
     import torch
     import kascade_amd
@@ -182,10 +183,9 @@ def serve_bench(ctx, jobs, seed, slots):
 
     def run(mode, arrivals, nslots=slots):
         sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one"), num_slots=nslots)
-        # warm the slots' scratch pools
-        for k in range(min(8, jobs)):
-            kind, s, w = pool[k]
-            t, _, _ = sv.submit(kind, s, w, *outs[mode][k])
+        # steady state: one untimed pass over the pool sizes the slots' scratch
+        warm = [sv.submit(kind, s, w, *outs[mode][k])[0] for k, (kind, s, w) in enumerate(pool)]
+        for t in warm:
             sv.wait(t)
         sv.reset_stats()
         t0 = time.perf_counter()

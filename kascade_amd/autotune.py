@@ -3,7 +3,7 @@
     python -m kascade_amd.autotune [--classes 10,12,...,27] [--out kascade_amd/tuning_table.txt]
 
 For every size class c it generates a seeded random weighted list of 2^c
-elements on the device and picks, by coordinate descent over the knobs
+elements (at least 12288) on the device and picks, by coordinate descent over the knobs
 
     ruler spacing at the top level, ruler spacing below it,
     resident chase workgroups per CU, rulers per work-counter atomic,
@@ -45,9 +45,14 @@ def measure(ctx, succ, w, root, rank, cfg, reps):
     return statistics.median(times)
 
 
+# Lists of at most 8192 elements go to the one-workgroup solver, so the small
+# classes are timed where the sparse ruling set starts.
+MIN_TUNED_N = 12288
+
+
 def tune_class(ctx, c, reps, seed=12345):
     import torch
-    n = 1 << c
+    n = max(1 << c, MIN_TUNED_N)
     succ, w = ctx.gen_list(n, seed + c, weighted=True)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
@@ -85,7 +90,7 @@ def main():
     for c in [int(x) for x in args.classes.split(",")]:
         best, t = tune_class(ctx, c, args.reps)
         rows.append((c, best, t))
-        print(f"2^{c}: {best} {t * 1e3:.3f} ms ({(1 << c) / t:.3e} elements/s)", flush=True)
+        print(f"2^{c}: {best} {t * 1e3:.3f} ms ({max(1 << c, MIN_TUNED_N) / t:.3e} elements/s)", flush=True)
     with open(args.out, "w") as f:
         f.write("# kascade_amd tuning table (written by kascade_amd/autotune.py on the GPU)\n")
         f.write("# log2n spacing_top spacing_deep chase_blocks_per_cu chase_chunk   # median ms\n")

@@ -352,7 +352,7 @@ unclaimed_fix_kernel(const uint64_t* __restrict__ u_ids, uint64_t nu, uint64_t n
 // ---------------------------------------------------------------------------
 // One-workgroup solver for small levels: Jacobi pointer doubling in LDS.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kSmallMax = 4096;
+constexpr uint32_t kSmallMax = 8192;
 constexpr unsigned kSmallThreads = 1024;
 constexpr uint32_t kSmallFlag = 0x80000000u;
 
@@ -408,7 +408,7 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
     rounds++;
     const int more = __syncthreads_or(any);
     if (!more) break;
-    if (rounds > 14) {
+    if (rounds > 15) {
       if (threadIdx.x == 0) raise_err(err, kErrCycle);
       break;
     }
@@ -699,7 +699,7 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 // Built-in table (the autotuner's output for MI355X at the time of the last
 // sweep; a tuning_table.txt next to the library overrides it).
 const TuneRow kBuiltinTuning[] = {
-    {10, 128, 16, 2, 1024}, {12, 128, 16, 4, 256}, {14, 8, 16, 1, 64},
+    {10, 8, 16, 1, 64}, {12, 8, 16, 1, 64}, {14, 8, 16, 1, 64},
     {16, 8, 8, 1, 64}, {18, 8, 16, 8, 64}, {20, 8, 8, 1, 64},
     {22, 16, 16, 1, 64}, {24, 32, 16, 2, 64}, {26, 128, 8, 2, 64},
     {27, 128, 8, 4, 256},

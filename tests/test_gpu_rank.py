@@ -59,7 +59,7 @@ ALGOS = ["pd", "srs"]
 
 
 @pytest.mark.parametrize("algo", ALGOS)
-@pytest.mark.parametrize("n", [1, 2, 3, 100, 4096, 4097, 10000, 1 << 16, 300007])
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 4096, 4097, 8192, 8193, 10000, 1 << 16, 300007])
 def test_rank_random_list(ctx, algo, n):
     say("rank", algo, n)
     succ, w = o.gen_list(n, 99 + n, True)
@@ -239,7 +239,7 @@ def test_tuning_table(ctx):
         ctx.tuning_set([(10, 4, 4, 1, 64), (18, 128, 8, 2, 1024)])
         assert ctx.tuning_get(1 << 12)["spacing_top"] == 4
         assert ctx.tuning_get(1 << 20)["spacing_top"] == 128
-        for n in [5000, 300000]:
+        for n in [10000, 300000]:
             succ, w = o.gen_list(n, 17 + n, True)
             root, rank = ctx.rank_list(dev(succ), dev(w))
             er, ek = o.list_closed_form(n, 17 + n, w)
@@ -247,4 +247,4 @@ def test_tuning_table(ctx):
     finally:
         ctx.tuning_set([])
     assert ctx.tuning_get(1 << 20)["spacing_top"] in (8, 16, 32, 64, 128)
-    assert before["small_n"] == 4096
+    assert before["small_n"] == 8192

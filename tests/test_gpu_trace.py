@@ -26,7 +26,7 @@ def check_levels(trace, n):
             assert b["n"] == a["rulers"] + a["tail_rulers"] and b["level"] == a["level"] + 1
 
 
-@pytest.mark.parametrize("n", [5000, 1 << 20, 1 << 23])
+@pytest.mark.parametrize("n", [10000, 1 << 20, 1 << 23])
 def test_trace_single_list(ctx, n):
     say("trace list", n)
     succ, w = ctx.gen_list(n, 3, weighted=True)
@@ -62,7 +62,7 @@ def test_trace_pointer_doubling_rounds(ctx, n):
     tr = ctx.trace_read()
     ctx.trace(False)
     pd = [t for t in tr if t["kind"] == "pd"]
-    if n <= 4096:                                  # the one-workgroup solver takes small inputs
+    if n <= 8192:                                  # the one-workgroup solver takes small inputs
         assert [t["kind"] for t in tr] == ["small"]
     else:
         assert len(pd) == 1 and pd[0]["rounds"] == max(1, math.ceil(math.log2(n - 1)))
