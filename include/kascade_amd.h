@@ -50,6 +50,21 @@ typedef enum kc_algorithm {
   KC_ALGO_SPARSE_RULING_SET = 2 /* ruler chase + recursion, sparse_ruling_set.cpp:45-393 */
 } kc_algorithm;
 
+/* How the levels of a ranking are sized and launched (kc_rank_cfg.launch).
+ * No reference counterpart: the reference sizes each round with MPI
+ * collectives.  Device-sized levels read their element counts from device
+ * memory inside capacities fixed from (n, number of lists), so a ranking
+ * needs no host round trip until it ends and can be captured as a HIP
+ * graph.  An input whose unreached list prefixes exceed the capacity (far
+ * beyond what random inputs reach) is redone with host-sized levels. */
+typedef enum kc_launch {
+  KC_LAUNCH_AUTO = 0,   /* device-sized; a call repeated with the same arrays and
+                           sizes is captured as a HIP graph (once per context)
+                           and replayed */
+  KC_LAUNCH_DEVICE = 1, /* device-sized, kernels launched one by one */
+  KC_LAUNCH_HOST = 2    /* each level's counts read on the host to size the next */
+} kc_launch;
+
 /* Knobs of kascade::SparseRulingSetConfig / PointerDoublingConfig
  * (configuration.hpp:39-47, :65-95) that mean something on one GPU.  Zero
  * means "choose" (the per-size autotuned table). */
@@ -61,7 +76,7 @@ typedef struct kc_rank_cfg {
   uint32_t validate;        /* 1: reject in-degree > 1 before ranking (is_list, successor_utils.cpp:31-63) */
   uint32_t chase_blocks_per_cu; /* resident 256-lane chase workgroups per CU, 0 = auto */
   uint32_t chase_chunk;     /* rulers a wavefront takes per work-counter atomic, 0 = auto */
-  uint32_t reserved;
+  uint32_t launch;          /* kc_launch */
 } kc_rank_cfg;
 
 /* ---- context ------------------------------------------------------------ */
@@ -79,6 +94,10 @@ kc_status kc_context_trim(void *ctx);
  * allocate and free device memory each time.  This frees that cache.  No
  * reference counterpart (the reference allocates per call with std::vector). */
 kc_status kc_release_cached_scratch(int device);
+/* Launch statistics of the context since its creation (see kc_launch):
+ * out3 = {HIP graphs captured, graph replays, device-sized rankings redone
+ * host-sized after a capacity overflow}.  No reference counterpart. */
+kc_status kc_context_launch_stats(void *ctx, uint64_t *out3);
 
 /* Per-phase device timing (HIP events on the launching stream), the
  * counterpart of the reference's kamping::measurements::timer phases

@@ -32,7 +32,7 @@ class RankCfg(ctypes.Structure):
     _fields_ = [("algorithm", ctypes.c_int32), ("ruler_spacing", ctypes.c_uint32),
                 ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
                 ("validate", ctypes.c_uint32), ("chase_blocks_per_cu", ctypes.c_uint32),
-                ("chase_chunk", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("chase_chunk", ctypes.c_uint32), ("launch", ctypes.c_uint32)]
 
 
 class TreeCfg(ctypes.Structure):
@@ -100,6 +100,7 @@ _SIGS = {
     "kc_context_scratch_bytes": (_U64, [_P]),
     "kc_context_trim": (ctypes.c_int, [_P]),
     "kc_release_cached_scratch": (ctypes.c_int, [ctypes.c_int]),
+    "kc_context_launch_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "kc_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "kc_profile_reset": (ctypes.c_int, [_P]),
     "kc_profile_read": (ctypes.c_uint32, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
@@ -195,9 +196,13 @@ def _numel(a):
     return a.size if isinstance(a, np.ndarray) else a.numel()
 
 
+LAUNCHES = {"auto": 0, "device": 1, "host": 2}
+
+
 def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
-         chase_blocks_per_cu=0, chase_chunk=0):
+         chase_blocks_per_cu=0, chase_chunk=0, launch="auto"):
     c = RankCfg()
+    c.launch = LAUNCHES[launch] if isinstance(launch, str) else int(launch)
     c.chase_blocks_per_cu = chase_blocks_per_cu
     c.chase_chunk = chase_chunk
     c.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
@@ -530,6 +535,12 @@ class Context:
 
     def trim(self):
         self._check(self._lib.kc_context_trim(self.handle))
+
+    def launch_stats(self):
+        """{'graph_captures', 'graph_replays', 'host_redos'} since the context was created."""
+        out = (ctypes.c_uint64 * 3)()
+        self._check(self._lib.kc_context_launch_stats(self.handle, out))
+        return {"graph_captures": out[0], "graph_replays": out[1], "host_redos": out[2]}
 
 
 def release_cached_scratch(device=0):

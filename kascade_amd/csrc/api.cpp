@@ -155,6 +155,19 @@ void release_chunk_cache(int device) {
   c.bytes = 0;
 }
 
+void Arena::signature(std::vector<uint64_t>& out) const {
+  out.push_back(chunks_.size());
+  for (const auto& c : chunks_) {
+    out.push_back(reinterpret_cast<uint64_t>(c.base));
+    out.push_back(c.size);
+  }
+  out.push_back(top_);
+  if (top_ > 0) {
+    out.push_back(blocks_[top_ - 1].chunk);
+    out.push_back(blocks_[top_ - 1].offset + blocks_[top_ - 1].size);
+  }
+}
+
 uint64_t Arena::held() const { return held_.load(std::memory_order_relaxed); }
 
 // -------------------------------------------------------------- Profiler ---
@@ -229,6 +242,18 @@ void check_err(Context* ctx, hipStream_t st) {
                         hipMemcpyDeviceToHost, st));
   KC_HIP(hipStreamSynchronize(st));
   raise_from_bits(ctx, st, static_cast<uint32_t>(ctx->h_pinned[kErrSlot]));
+}
+
+bool check_err_overflow(Context* ctx, hipStream_t st) {
+  KC_HIP(hipMemcpyAsync(ctx->h_pinned + kErrSlot, ctx->d_counters + kErrSlot, sizeof(uint64_t),
+                        hipMemcpyDeviceToHost, st));
+  KC_HIP(hipStreamSynchronize(st));
+  const uint32_t e = static_cast<uint32_t>(ctx->h_pinned[kErrSlot]);
+  raise_from_bits(ctx, st, e & ~kErrOverflow);
+  if (!(e & kErrOverflow)) return false;
+  KC_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), st));
+  KC_HIP(hipStreamSynchronize(st));
+  return true;
 }
 
 void read_counters(Context* ctx, hipStream_t st, int first, int count, uint64_t* out) {
@@ -396,6 +421,7 @@ void kc_context_destroy(void* vctx) {
   (void)hipDeviceSynchronize();
   ctx->arena.release(0);
   ctx->arena.recycle();
+  if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->d_counters) (void)hipFree(ctx->d_counters);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -415,8 +441,21 @@ uint64_t kc_context_scratch_bytes(void* vctx) {
 kc_status kc_context_trim(void* vctx) {
   return guarded(vctx, [&](Context* ctx) {
     KC_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    ctx->graph_exec = nullptr;
+    ctx->graph_key.clear();
+    ctx->last_key.clear();
     ctx->arena.trim();
   });
+}
+
+kc_status kc_context_launch_stats(void* vctx, uint64_t* out3) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx || !out3) return KC_EINVAL;
+  out3[0] = ctx->graph_captures;
+  out3[1] = ctx->graph_replays;
+  out3[2] = ctx->host_redos;
+  return KC_OK;
 }
 
 kc_status kc_release_cached_scratch(int device) {

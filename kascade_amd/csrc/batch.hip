@@ -240,47 +240,55 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   if (d_trees)
     KC_HIP(hipMemcpyAsync(d_trees, trees.data(), trees.size() * sizeof(BatchJob), hipMemcpyHostToDevice, st));
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
-  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
   uint64_t* heads = nt ? scratch.get<uint64_t>(nt) : nullptr;
-  if (nl) {
-    Phase ph(ctx, "batch_pack", st);
-    hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st,
-                       d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
-  }
-  uint32_t* keys = nullptr;
   const bool linked = tour_build != KC_TOUR_SORTED;
+  uint32_t *head = nullptr, *sib = nullptr, *keys = nullptr, *vals = nullptr;
+  uint8_t* is_root = nullptr;
   if (nt && linked) {
-    Phase ph(ctx, "batch_tour", st);
-    uint32_t* head = scratch.get<uint32_t>(nt);
-    uint32_t* sib = scratch.get<uint32_t>(nt);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
-                       0xFFFFFFFFu);
-    hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
-    hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
+    head = scratch.get<uint32_t>(nt);
+    sib = scratch.get<uint32_t>(nt);
   } else if (nt) {
-    Phase ph(ctx, "batch_tour", st);
     keys = scratch.get<uint32_t>(nt);
-    uint32_t* vals = scratch.get<uint32_t>(nt);
-    uint8_t* is_root = scratch.get<uint8_t>(nt);
-    hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
-                       d_trees, static_cast<uint32_t>(trees.size()), nt, nl, keys, vals, is_root, rec,
-                       ctx->d_err, ctr + 1);
-    radix_sort_pairs_u32(ctx, keys, vals, nt, bits_for(nt), st);
-    hipLaunchKernelGGL(batch_tour_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, keys,
-                       vals, is_root, nt, nl, rec, heads, ctr + 2);
+    vals = scratch.get<uint32_t>(nt);
+    is_root = scratch.get<uint8_t>(nt);
   }
-  uint64_t counts[3] = {0, 0, 0};
-  read_counters(ctx, st, 0, 3, counts);
   // size class of the whole batch; the ruler spacing per level is capped by
   // the mean list length inside the ranking
   RankTuning tune = tuning_for(ctx, total, cfg);
   int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
-  const TourRegion tr{nl, heads, counts[2]};
-  rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st, 0,
-               nt ? &tr : nullptr);
+  for (;;) {
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
+    if (nl) {
+      Phase ph(ctx, "batch_pack", st);
+      hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st,
+                         d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
+    }
+    if (nt && linked) {
+      Phase ph(ctx, "batch_tour", st);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
+                         0xFFFFFFFFu);
+      hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                         d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
+      hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                         d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
+    } else if (nt) {
+      Phase ph(ctx, "batch_tour", st);
+      hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+                         d_trees, static_cast<uint32_t>(trees.size()), nt, nl, keys, vals, is_root, rec,
+                         ctx->d_err, ctr + 1);
+      radix_sort_pairs_u32(ctx, keys, vals, nt, bits_for(nt), st);
+      hipLaunchKernelGGL(batch_tour_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, keys,
+                         vals, is_root, nt, nl, rec, heads, ctr + 2);
+    }
+    uint64_t counts[3] = {0, 0, 0};
+    read_counters(ctx, st, 0, 3, counts);
+    const TourRegion tr{nl, heads, counts[2]};
+    if (rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st, 0,
+                     nt ? &tr : nullptr))
+      break;
+    tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+  }
   check_err(ctx, st);
   Phase ph(ctx, "batch_scatter", st);
   if (nl)

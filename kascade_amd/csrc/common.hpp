@@ -30,6 +30,7 @@ enum ErrBits : uint32_t {
   kErrRootWeight = 2u, // nonzero weight on a tail
   kErrNotList = 4u,    // an element reached twice (in-degree > 1)
   kErrCycle = 8u,      // a cycle without a tail
+  kErrOverflow = 16u,  // a device-sized level outgrew a capacity (redo host-sized; never reported)
 };
 
 // Scratch pool: chunks reused across calls, handed out as a stack.  When a
@@ -51,6 +52,9 @@ class Arena {
   void trim();
   uint64_t held() const;
   uint64_t peak() const { return peak_; }
+  // Appends what decides the addresses of the next allocations (chunks and
+  // the stack position), so equal signatures hand out equal addresses.
+  void signature(std::vector<uint64_t>& out) const;
 
  private:
   struct Chunk {
@@ -125,6 +129,14 @@ struct Context {
   bool trace_on = false;
   int trace_subproblem = 0;
   std::vector<kc_trace_level> trace;
+  // HIP graph of the last repeated device-sized ranking, and the keys (every
+  // argument of the launch sequence) of that graph and of the last call
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint64_t> graph_key;
+  std::vector<uint64_t> last_key;
+  uint64_t graph_replays = 0;
+  uint64_t graph_captures = 0;
+  uint64_t host_redos = 0;  // device-sized rankings redone host-sized after an overflow
 };
 
 // Scoped stack allocation from the arena.
@@ -178,6 +190,9 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) {
 
 // Reads the device error word (synchronising `stream`) and throws on error.
 void check_err(Context* ctx, hipStream_t stream);
+// After a device-sized ranking (synchronising `stream`): raises input errors;
+// true when a capacity overflowed and the call must be redone host-sized.
+bool check_err_overflow(Context* ctx, hipStream_t stream);
 // Copies `count` counters starting at slot `first` to the host (synchronising).
 void read_counters(Context* ctx, hipStream_t stream, int first, int count, uint64_t* out);
 
@@ -194,6 +209,7 @@ struct RankTuning {
   // ranked as a sub-problem) stay within an eighth of the level
   bool spacing_top_fixed;
   bool spacing_deep_fixed;
+  uint32_t launch;  // kc_launch
 };
 // Per-size-class tuning table row: log2(n) class and the knobs picked for it
 // (written by kascade_amd/autotune.py into tuning_table.txt, which contexts
@@ -244,10 +260,12 @@ struct TourRegion {
 };
 // Ranks packed records (ids < n, consumed) into root (ids) / rank; outputs as
 // `out_shift` selects (see OutMap; n_out outputs).  max_tails bounds the
-// number of lists.
-void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                  int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr);
+// number of lists.  Device-sized launches (tune.launch) synchronise at the
+// end and return false when a capacity overflowed: the caller then rebuilds
+// the records and calls again with tune.launch = KC_LAUNCH_HOST.
+[[nodiscard]] bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                                int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
+                                hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr);
 void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
 // Packs succ/weight into records with validation (range, tail weight).

@@ -60,4 +60,17 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long* ctr, bool mi
 
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bits) { atomicOr(err, bits); }
 
+// Streaming accesses (read or written once, coalesced): non-temporal, so they
+// do not evict the gather tables the same kernel keeps in L2.
+__device__ __forceinline__ Rec load_stream(const Rec* p) {
+  Rec r;
+  r.s = __builtin_nontemporal_load(&p->s);
+  r.w = __builtin_nontemporal_load(&p->w);
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void store_stream(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+
 }  // namespace kc

@@ -201,18 +201,21 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   Scratch scratch(ctx);
   Rec* tour = scratch.get<Rec>(2 * n);
   uint64_t* heads = scratch.get<uint64_t>(n);
-  uint64_t nheads = 0;
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
-  const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, heads, &nheads, st);
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
   RankTuning tune = tuning_for(ctx, 2 * n, rc);
   int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
-  // rank_via_euler_tour_select_algorithm (eulertour.cpp:282-321)
-  const TourRegion tr{0, heads, nheads};
-  rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st, /*out_shift=*/1, &tr);
+  for (;;) {
+    uint64_t nheads = 0;
+    const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, heads, &nheads, st);
+    // rank_via_euler_tour_select_algorithm (eulertour.cpp:282-321)
+    const TourRegion tr{0, heads, nheads};
+    if (rank_records(ctx, tour, 2 * n, n, root, depth, 2 * roots, tune, algo, st, /*out_shift=*/1, &tr)) break;
+    tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+  }
   check_err(ctx, st);
 }
 

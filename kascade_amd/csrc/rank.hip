@@ -82,8 +82,13 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
              const uint64_t* __restrict__ forced, uint64_t nF, uint64_t tour_lo,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
-             uint32_t chunk, unsigned long long* chain) {
+             uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev) {
+  if (*err) return;  // an earlier kernel found invalid input: stop, as the host-sized path does
   const uint64_t lt = lanemask_lt();
+  if (n_dev) {  // device-sized level: n is a capacity, the size is in memory
+    n = *n_dev < n ? *n_dev : n;
+    nR = (n + (uint64_t{1} << logK) - 1) >> logK;
+  }
   // rulers [0, nR) sit at computed places; [nR, nRF) start at the tour heads
   const uint64_t nRF = nR + nF;
   // downward tour edges carry no claim (their results are never read)
@@ -222,8 +227,11 @@ __global__ void first_root_kernel(const Rec* __restrict__ T, uint64_t* __restric
 __global__ void __launch_bounds__(kThreads)
 map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m,
                  const uint64_t* __restrict__ first, int64_t* __restrict__ T8,
-                 uint32_t* __restrict__ mixed_roots) {
+                 uint32_t* __restrict__ mixed_roots, const uint64_t* __restrict__ m_dev,
+                 const uint32_t* __restrict__ err) {
+  if (*err) return;  // the sublist ranking failed: T is not valid
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (m_dev && *m_dev < m) m = *m_dev;
   if (i >= m) return;
   const Rec t = T[i];
   if (t.s != first[0] && *mixed_roots == 0) *mixed_roots = 1;
@@ -241,8 +249,11 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
                  const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
-                 unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err) {
+                 unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err,
+                 const uint64_t* __restrict__ n_dev, bool device_sized) {
+  if (*err) return;
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (n_dev && *n_dev < n) n = *n_dev;
   // All rulers in one list (one root): gather only the 8-byte rank, from a
   // table half the size, so more of it stays in L2.
   const bool one_root = *mixed_roots == 0;
@@ -250,17 +261,17 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
-    r = rec[v];
+    r = load_stream(&rec[v]);
     if (r.s & kRootFlag) {
       if (out_selected(om, v, n_out)) {
         const uint64_t pos = out_pos(om, v);
         if (one_root) {
-          root[pos] = root0 >> om.shift;
-          rank[pos] = T8[r.s & kIdMask] - r.w;
+          store_stream(&root[pos], root0 >> om.shift);
+          store_stream(&rank[pos], T8[r.s & kIdMask] - r.w);
         } else {
           const Rec t = T[r.s & kIdMask];
-          root[pos] = t.s >> om.shift;
-          rank[pos] = t.w - r.w;
+          store_stream(&root[pos], t.s >> om.shift);
+          store_stream(&rank[pos], t.w - r.w);
         }
       }
     } else if (v < tour_lo) {
@@ -278,10 +289,14 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
     }
   }
   const uint64_t k = wave_append(u_ctr, unclaimed);
-  if (unclaimed && k < ucap) {  // beyond the capacity: collected by a second pass
+  if (unclaimed && k < ucap) {
     u_ids[k] = v;
     u_orig[k] = r;
     rec[v].s = k | kAuxFlag;
+  } else if (unclaimed && device_sized) {
+    // beyond the capacity: host-sized levels collect the rest in a second
+    // pass; device-sized ones flag the call to be redone host-sized
+    raise_err(err, kErrOverflow);
   }
 }
 
@@ -312,8 +327,11 @@ collect_unclaimed_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t* __restrict
 __global__ void __launch_bounds__(kThreads)
 unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
                        const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T,
-                       Rec* __restrict__ urec, Rec* __restrict__ ext) {
+                       Rec* __restrict__ urec, Rec* __restrict__ ext,
+                       const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err) {
+  if (*err) return;
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (nu_dev && *nu_dev < nu) nu = *nu_dev;
   if (k >= nu) return;
   const uint64_t v = u_ids[k];
   const Rec r0 = u_orig[k];
@@ -327,19 +345,27 @@ unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__
   if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
     urec[k] = Rec{ms.s & kIdMask, r0.w};
     ext[k] = Rec{0, 0};
-  } else {
+  } else if (ms.s & kRootFlag) {
     // claimed: its final root / rank follow from its ruler's entry
     const Rec t = T[ms.s & kIdMask];
     urec[k] = Rec{k, 0};
     ext[k] = Rec{t.s, r0.w + t.w - ms.w};
+  } else {
+    // not collected: a device-sized level past its capacity, which is redone
+    // host-sized; any in-range values do
+    urec[k] = Rec{k, 0};
+    ext[k] = Rec{0, 0};
   }
 }
 
 __global__ void __launch_bounds__(kThreads)
 unclaimed_fix_kernel(const uint64_t* __restrict__ u_ids, uint64_t nu, uint64_t n_out,
                      const Rec* __restrict__ ures, const Rec* __restrict__ ext,
-                     uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om) {
+                     uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
+                     const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err) {
+  if (*err) return;  // the prefix ranking failed: ures is not valid
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (nu_dev && *nu_dev < nu) nu = *nu_dev;
   if (k >= nu) return;
   const uint64_t v = u_ids[k];
   if (!out_selected(om, v, n_out)) return;
@@ -358,7 +384,12 @@ constexpr uint32_t kSmallFlag = 0x80000000u;
 
 __global__ void __launch_bounds__(kSmallThreads)
 small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint64_t* __restrict__ root,
-                  int64_t* __restrict__ rank, OutMap om, uint32_t* err) {
+                  int64_t* __restrict__ rank, OutMap om, uint32_t* err, const uint64_t* __restrict__ n_dev) {
+  if (*err) return;
+  if (n_dev && *n_dev < n) {  // device-sized: every element is an output
+    n = static_cast<uint32_t>(*n_dev);
+    n_out = n;
+  }
   __shared__ uint32_t s_succ[kSmallMax];
   __shared__ int64_t s_rank[kSmallMax];
   constexpr int kPer = kSmallMax / kSmallThreads;
@@ -427,8 +458,10 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
 // update_via_read_offsets, pointer_doubling_generic.hpp:62-81).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kThreads)
-pd_flag_roots_kernel(Rec* __restrict__ a, uint64_t n, uint32_t* err) {
+pd_flag_roots_kernel(Rec* __restrict__ a, uint64_t n, uint32_t* err, const uint64_t* __restrict__ n_dev) {
+  if (*err) return;
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (n_dev && *n_dev < n) n = *n_dev;
   if (v >= n) return;
   Rec r = a[v];
   if ((r.s & (kRootFlag | kAuxFlag)) || r.s >= n) {
@@ -476,6 +509,71 @@ pd_write_kernel(const Rec* __restrict__ a, uint64_t n, uint64_t n_out, uint64_t*
   rank[out_pos(om, v)] = r.w;
 }
 
+// Device-sized doubling: a fixed number of rounds, buf[r & 1] -> buf[(r + 1) & 1].
+// done[r] = 1 when round r left no element unfinished; later rounds return
+// at once, and the result is the output of the first such round.
+__global__ void __launch_bounds__(kThreads)
+pd_round_ds_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
+                   const uint64_t* __restrict__ n_dev, uint32_t* __restrict__ active, uint32_t r,
+                   const uint32_t* __restrict__ err) {
+  if (*err) return;
+  if (r > 0 && active[r - 1] == 0) return;  // finished earlier
+  if (n_dev && *n_dev < n) n = *n_dev;
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool still = false;
+  if (v < n) {
+    const Rec x = a[v];
+    if (x.s & kRootFlag) {
+      b[v] = x;
+    } else {
+      const Rec t = a[x.s];
+      b[v] = Rec{t.s, x.w + t.w};
+      still = !(t.s & kRootFlag);
+    }
+  }
+  const uint64_t m = __ballot(still);
+  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) {
+    if (__hip_atomic_load(&active[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+      __hip_atomic_store(&active[r], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+pd_write_ds_kernel(const Rec* __restrict__ buf0, const Rec* __restrict__ buf1, uint64_t n,
+                   const uint64_t* __restrict__ n_dev, uint64_t n_out, uint64_t* __restrict__ root,
+                   int64_t* __restrict__ rank, OutMap om, const uint32_t* __restrict__ active,
+                   uint32_t rounds, uint32_t* err) {
+  if (*err) return;
+  uint32_t last = rounds;
+  for (uint32_t r = 0; r < rounds; r++)
+    if (active[r] == 0) {
+      last = r;
+      break;
+    }
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (last == rounds) {  // still unfinished after every round: a cycle
+    if (v == 0) raise_err(err, kErrCycle);
+    return;
+  }
+  if (n_dev) {
+    if (*n_dev < n) n = *n_dev;
+    n_out = n;
+  }
+  if (v >= n || !out_selected(om, v, n_out)) return;
+  const Rec x = ((last + 1) & 1) ? buf1[v] : buf0[v];
+  root[out_pos(om, v)] = (x.s & ~kRootFlag) >> om.shift;
+  rank[out_pos(om, v)] = x.w;
+}
+
+// m = rulers + tour heads + tails reached, the size of the ruler sublist.
+__global__ void level_size_kernel(const uint64_t* __restrict__ n_dev, uint64_t n, uint32_t logK,
+                                  uint64_t nF, const unsigned long long* __restrict__ extra,
+                                  uint64_t* __restrict__ m_out) {
+  if (threadIdx.x != 0) return;
+  if (n_dev && *n_dev < n) n = *n_dev;
+  m_out[0] = ((n + (uint64_t{1} << logK) - 1) >> logK) + nF + extra[0];
+}
+
 // The doubling state after some rounds, root flags kept (bits.hpp:9-21).
 __global__ void __launch_bounds__(kThreads)
 pd_state_kernel(const Rec* __restrict__ a, uint64_t n, uint64_t* __restrict__ succ,
@@ -520,7 +618,7 @@ uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uin
   Scratch scratch(ctx);
   Rec* other = scratch.get<Rec>(n);
   const unsigned blocks = blocks_for(n, kThreads);
-  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, rec, n, ctx->d_err);
+  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, rec, n, ctx->d_err, nullptr);
   Rec* a = rec;
   Rec* b = other;
   unsigned long long* active = reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotActive;
@@ -546,6 +644,19 @@ uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uin
   return rounds;
 }
 
+// Ruler spacing of a level: the tuned K, unless the caller fixed it, capped so
+// that a forest of max_tails lists keeps K <= (mean list length) / 16.
+uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails) {
+  uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
+  if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
+    const uint64_t mean = n / std::max<uint64_t>(1, max_tails);
+    uint32_t cap = 2;
+    while (cap < spacing && uint64_t{cap} * 16 <= mean) cap *= 2;
+    spacing = std::min(spacing, cap);
+  }
+  return spacing;
+}
+
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                int level, hipStream_t st, const TourRegion* tours) {
@@ -553,14 +664,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   const uint64_t nF = tours ? tours->nheads : 0;
   const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
-  uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
-  if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
-    // a forest of max_tails lists: keep K <= (mean list length) / 8
-    const uint64_t mean = n / std::max<uint64_t>(1, max_tails);
-    uint32_t cap = 2;
-    while (cap < spacing && uint64_t{cap} * 16 <= mean) cap *= 2;
-    spacing = std::min(spacing, cap);
-  }
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails);
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t cap = nR + nF + std::min<uint64_t>(max_tails, n);
@@ -581,7 +685,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                        dim3(kThreads), 0, st, rec, n, logK, seed,
                        nR, tours ? tours->heads : nullptr, nF, tour_lo, cap, sub, sub_orig,
                        ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
-                       tune.chase_chunk, ctr + kSlotChainMax);
+                       tune.chase_chunk, ctr + kSlotChainMax, nullptr);
   }
   uint64_t extra = 0;
   read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
@@ -613,7 +717,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   uint32_t* mixed_roots = scratch.get<uint32_t>(1);
   hipLaunchKernelGGL(first_root_kernel, dim3(1), dim3(64), 0, st, T, first, mixed_roots);
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, T,
-                     sub_orig, m, first, T8, mixed_roots);
+                     sub_orig, m, first, T8, mixed_roots, nullptr, ctx->d_err);
 
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
@@ -625,7 +729,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
                        n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
-                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err);
+                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
@@ -647,7 +751,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* ext = scratch.get<Rec>(nu);
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     rec, u_ids, u_orig, nu, T, urec, ext);
+                     rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err);
   // every unclaimed prefix belongs to a different list of this level
   ctx->trace_subproblem = 1;
   rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, std::min(nu, max_tails), tune,
@@ -655,7 +759,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                     st);
   ctx->trace_subproblem = role;
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     u_ids, nu, n_out, ures, ext, root, rank, om);
+                     u_ids, nu, n_out, ures, ext, root, rank, om, nullptr, ctx->d_err);
 }
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
@@ -674,7 +778,7 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
     Phase ph(ctx, "small_rank", st);
     hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec,
                        static_cast<uint32_t>(n), static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om,
-                       ctx->d_err);
+                       ctx->d_err, nullptr);
     return;
   }
   if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
@@ -692,6 +796,124 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
   }
   srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st,
             tours && tours->lo < n ? tours : nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Device-sized levels.  The host fixes every level's capacity from (n, number
+// of lists) alone; the sizes themselves stay in device memory (n_dev: the
+// level's element count, at most n), so the whole ranking is one launch
+// sequence with no host round trip, which a HIP graph can capture.  Each
+// level zeroes its own counters with a kernel (no memset nodes).
+// ---------------------------------------------------------------------------
+enum DsSlot : int { kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsChain = 4, kDsSlots = 8 };
+
+void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+             int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
+             hipStream_t st, const TourRegion* tours);
+
+void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+           int64_t* rank, OutMap om, hipStream_t st) {
+  Phase ph(ctx, "pointer_doubling", st);
+  Scratch scratch(ctx);
+  Rec* other = scratch.get<Rec>(n);
+  const uint32_t rounds = log2_floor(n) + 2;
+  uint32_t* active = scratch.get<uint32_t>(rounds);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(active),
+                     static_cast<int>((rounds + 1) / 2));
+  const unsigned blocks = blocks_for(n, kThreads);
+  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, rec, n, ctx->d_err, n_dev);
+  Rec* buf[2] = {rec, other};
+  for (uint32_t r = 0; r < rounds; r++)
+    hipLaunchKernelGGL(pd_round_ds_kernel, dim3(blocks), dim3(kThreads), 0, st, buf[r & 1], buf[(r + 1) & 1], n,
+                       n_dev, active, r, ctx->d_err);
+  hipLaunchKernelGGL(pd_write_ds_kernel, dim3(blocks_for(std::min<uint64_t>(n, n_out << om.shift), kThreads)),
+                     dim3(kThreads), 0, st, rec, other, n, n_dev, n_out, root, rank, om, active, rounds,
+                     ctx->d_err);
+}
+
+void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+                  int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int level,
+                  hipStream_t st, const TourRegion* tours) {
+  const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
+  const uint64_t nF = tours ? tours->nheads : 0;
+  const uint64_t list_end = std::min(n, tour_lo);
+  Scratch scratch(ctx);
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails);
+  const uint32_t logK = log2_floor(spacing);
+  const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
+  const uint64_t lists = std::min<uint64_t>(max_tails, n);
+  const uint64_t cap = nR + nF + lists;
+  Rec* sub = scratch.get<Rec>(cap);
+  Rec* T = scratch.get<Rec>(cap);
+  uint64_t* sub_orig = scratch.get<uint64_t>(cap);
+  int64_t* T8 = scratch.get<int64_t>(cap);
+  uint64_t* first = scratch.get<uint64_t>(1);
+  uint32_t* mixed_roots = scratch.get<uint32_t>(1);
+  unsigned long long* ctr = scratch.get<unsigned long long>(kDsSlots);
+  const uint32_t seed = static_cast<uint32_t>(splitmix64(0xC0FFEEull + level * 0x1234567ull + n));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, ctr, kDsSlots);
+
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  const unsigned chase_blocks = static_cast<unsigned>(
+      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (nR + nF + kThreads - 1) / kThreads)));
+  {
+    Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
+    hipLaunchKernelGGL(chase_kernel<false>, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR,
+                       tours ? tours->heads : nullptr, nF, tour_lo, cap, sub, sub_orig, ctr + kDsWork,
+                       ctr + kDsExtra, ctx->d_err, tune.chase_chunk, ctr + kDsChain, n_dev);
+  }
+  uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
+  hipLaunchKernelGGL(level_size_kernel, dim3(1), dim3(64), 0, st, n_dev, n, logK, nF, ctr + kDsExtra, m_dev);
+  // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
+  ds_rank(ctx, sub, cap, m_dev, cap, &T[0].s, &T[0].w, OutMap{2, 0}, lists, tune,
+          cap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1, st, nullptr);
+  hipLaunchKernelGGL(first_root_kernel, dim3(1), dim3(64), 0, st, T, first, mixed_roots);
+  hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, T, sub_orig, cap,
+                     first, T8, mixed_roots, m_dev, ctx->d_err);
+
+  // Unreached head prefixes: about K elements per list; the capacity leaves
+  // a wide margin (a prefix longer than 24 K has probability e^-24 on a
+  // random list) and an overflow is redone host-sized.
+  const uint64_t ucap = std::min<uint64_t>(list_end, 2 * uint64_t{spacing} * lists + 24 * uint64_t{spacing} + 4096);
+  uint64_t* u_ids = scratch.get<uint64_t>(std::max<uint64_t>(ucap, 1));
+  Rec* u_orig = scratch.get<Rec>(std::max<uint64_t>(ucap, 1));
+  {
+    Phase ph(ctx, kPropNames[std::min(level, 8)], st);
+    hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out, T,
+                       T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
+                       ctx->d_err, n_dev, true);
+  }
+  if (ucap == 0) return;  // tours only: nothing is left unreached
+  Phase ph(ctx, "unclaimed", st);
+  Rec* urec = scratch.get<Rec>(ucap);
+  Rec* ext = scratch.get<Rec>(ucap);
+  Rec* ures = scratch.get<Rec>(ucap);
+  const unsigned long long* nu_dev = ctr + kDsUnclaimed;
+  hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
+                     u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err);
+  ds_rank(ctx, urec, ucap, reinterpret_cast<const uint64_t*>(nu_dev), ucap, &ures[0].s, &ures[0].w, OutMap{2, 0},
+          std::min(ucap, max_tails), tune, ucap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
+          level + 1, st, nullptr);
+  hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, u_ids, ucap,
+                     n_out, ures, ext, root, rank, om, nu_dev, ctx->d_err);
+}
+
+void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+             int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
+             hipStream_t st, const TourRegion* tours) {
+  if (n == 0) return;
+  if (n <= tune.small_n && n <= kSmallMax) {
+    Phase ph(ctx, "small_rank", st);
+    hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, static_cast<uint32_t>(n),
+                       static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om, ctx->d_err, n_dev);
+    return;
+  }
+  if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
+    ds_pd(ctx, rec, n, n_dev, n_out, root, rank, om, st);
+    return;
+  }
+  ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st,
+               tours && tours->lo < n ? tours : nullptr);
 }
 
 }  // namespace
@@ -742,6 +964,8 @@ RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
       t.spacing_deep_fixed = true;
     }
     if (cfg->max_levels) t.max_levels = cfg->max_levels;
+    if (cfg->launch > KC_LAUNCH_HOST) throw KcError{KC_EINVAL, "invalid launch mode"};
+    t.launch = cfg->launch;
   }
   return t;
 }
@@ -755,11 +979,83 @@ void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t
                      reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotTails);
 }
 
-void rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+namespace {
+
+// Everything the device-sized launch sequence depends on: equal keys give the
+// same kernels with the same arguments.
+std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                                 int64_t* rank, uint64_t max_tails, const RankTuning& t, int algorithm,
+                                 uint32_t out_shift, const TourRegion* tours) {
+  std::vector<uint64_t> k = {reinterpret_cast<uint64_t>(rec), n, n_out, reinterpret_cast<uint64_t>(root),
+                             reinterpret_cast<uint64_t>(rank), max_tails, static_cast<uint64_t>(algorithm),
+                             out_shift, t.spacing_top, t.spacing_deep, t.max_levels, t.small_n,
+                             t.chase_blocks_per_cu, t.chase_chunk, t.spacing_top_fixed, t.spacing_deep_fixed,
+                             reinterpret_cast<uint64_t>(ctx->d_err), static_cast<uint64_t>(ctx->num_cus)};
+  if (tours) {
+    k.push_back(tours->lo);
+    k.push_back(reinterpret_cast<uint64_t>(tours->heads));
+    k.push_back(tours->nheads);
+  } else {
+    k.push_back(~uint64_t{0});
+  }
+  ctx->arena.signature(k);
+  return k;
+}
+
+}  // namespace
+
+bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
                   hipStream_t st, uint32_t out_shift, const TourRegion* tours) {
-  rank_records_impl(ctx, rec, n, n_out, root, rank, OutMap{1, out_shift}, max_tails, tune, algorithm, 0,
-                    st, tours);
+  const OutMap om{1, out_shift};
+  // the per-level trace reads every level's counts on the host
+  if (tune.launch == KC_LAUNCH_HOST || ctx->trace_on) {
+    rank_records_impl(ctx, rec, n, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours);
+    return true;
+  }
+  auto launch_all = [&] {
+    ds_rank(ctx, rec, n, nullptr, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours);
+  };
+  if (tune.launch == KC_LAUNCH_AUTO && !ctx->prof.on) {
+    // A call repeated with the same arguments (and scratch layout) is
+    // captured once as a HIP graph and replayed: one launch instead of one
+    // per kernel.  The first occurrence runs directly (its scratch is then
+    // allocated, so the capture allocates nothing).
+    std::vector<uint64_t> key = launch_key(ctx, rec, n, n_out, root, rank, max_tails, tune, algorithm, out_shift,
+                                           tours);
+    if (ctx->graph_exec && key == ctx->graph_key) {
+      KC_HIP(hipGraphLaunch(ctx->graph_exec, st));
+      ctx->graph_replays++;
+    } else if (key == ctx->last_key) {
+      hipGraph_t g = nullptr;
+      KC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_all();
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      KC_HIP(hipStreamEndCapture(st, &g));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      KC_HIP(e);
+      if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+      ctx->graph_exec = exec;
+      ctx->graph_key = key;
+      ctx->graph_captures++;
+      KC_HIP(hipGraphLaunch(exec, st));
+    } else {
+      launch_all();
+    }
+    ctx->last_key = std::move(key);
+  } else {
+    launch_all();
+  }
+  if (!check_err_overflow(ctx, st)) return true;
+  ctx->host_redos++;
+  return false;
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
@@ -774,10 +1070,13 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
   if (cfg && cfg->validate) validate_list(ctx, succ, n, st);
   Scratch scratch(ctx);
   Rec* rec = scratch.get<Rec>(n);
-  pack_records(ctx, succ, w, n, rec, st);
-  uint64_t tails = 0;
-  read_counters(ctx, st, kSlotTails, 1, &tails);
-  rank_records_impl(ctx, rec, n, n, root, rank, OutMap{1, 0}, tails, tune, algo, 0, st);
+  for (;;) {
+    pack_records(ctx, succ, w, n, rec, st);
+    uint64_t tails = 0;
+    read_counters(ctx, st, kSlotTails, 1, &tails);
+    if (rank_records(ctx, rec, n, n, root, rank, tails, tune, algo, st)) break;
+    tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+  }
   check_err(ctx, st);
 }
 
@@ -790,7 +1089,7 @@ void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
   pack_records(ctx, succ, w, n, a, st);
   check_err(ctx, st);
   const unsigned blocks = blocks_for(n, kThreads);
-  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, ctx->d_err);
+  hipLaunchKernelGGL(pd_flag_roots_kernel, dim3(blocks), dim3(kThreads), 0, st, a, n, ctx->d_err, nullptr);
   unsigned long long* active = reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotActive;
   for (uint32_t r = 0; r < rounds; r++) {
     hipLaunchKernelGGL(pd_round_kernel, dim3(blocks), dim3(kThreads), 0, st, a, b, n, active);

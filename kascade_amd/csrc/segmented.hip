@@ -113,13 +113,16 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   Scratch scratch(ctx);
   Rec* rec = scratch.get<Rec>(total);
   uint64_t* root_global = scratch.get<uint64_t>(total);
-  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
-  hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st, succ,
-                     seg_off, nseg, total, w, rec, ctx->d_err,
-                     reinterpret_cast<unsigned long long*>(ctx->d_counters));
-  uint64_t tails = 0;
-  read_counters(ctx, st, 0, 1, &tails);
-  rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st);
+  for (;;) {
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+    hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st, succ,
+                       seg_off, nseg, total, w, rec, ctx->d_err,
+                       reinterpret_cast<unsigned long long*>(ctx->d_counters));
+    uint64_t tails = 0;
+    read_counters(ctx, st, 0, 1, &tails);
+    if (rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st)) break;
+    tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+  }
   check_err(ctx, st);
   hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st,
                      root_global, seg_off, nseg, total, root);
