@@ -234,17 +234,25 @@ def serve_bench(ctx, jobs, seed, slots):
 
 
 def timed(ctx, fn, steps, warmup):
+    """Seconds per step over `steps` calls after `warmup` untimed ones, then
+    the per-phase device times (HIP events) from a second pass of `steps`
+    calls.  The timed pass runs as a user would (repeated calls replay their
+    captured HIP graph); the profiled pass launches kernel by kernel so that
+    every phase can be bracketed by events."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    ctx.profile(True)
-    ctx.profile_reset()
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
     phases = ctx.profile_read()
     ctx.profile(False)
     return dt / steps, phases
@@ -267,7 +275,8 @@ def tree_main(args, ctx):
            "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting (C3)", "n": n, "seed": args.seed,
                       "tour_build": args.tour_build},
            "effective_gbps": 24 * n / t / 1e9,
-           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
+           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
+           "launch": ctx.launch_stats()}
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
@@ -321,7 +330,8 @@ def segmented_main(args, ctx):
            "config": {"workload": f"{args.segments} lists log-uniform in [2^8, 2^20] (C4)",
                       "segments": args.segments, "total_elements": total, "seed": args.seed},
            "effective_gbps": (16 * total + 8 * args.segments) / t / 1e9,
-           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())}}
+           "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
+           "launch": ctx.launch_stats()}
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
@@ -353,20 +363,9 @@ def main():
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
     cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep)
-    for _ in range(args.warmup):
-        ctx.rank_list(succ, w, root, rank, **cfg)
-    torch.cuda.synchronize()
-    ctx.profile(True)
-    ctx.profile_reset()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.rank_list(succ, w, root, rank, **cfg)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    phases = ctx.profile_read()
-    ctx.profile(False)
-    ms_per_step = dt * 1e3 / args.steps
-    value = n / (dt / args.steps)
+    dt, phases = timed(ctx, lambda: ctx.rank_list(succ, w, root, rank, **cfg), args.steps, args.warmup)
+    ms_per_step = dt * 1e3
+    value = n / dt
 
     # dominant kernel and its roofline (HIP events on the library's stream)
     kernel_phases = {k: v for k, v in phases.items() if k not in ("unclaimed",)}
@@ -411,6 +410,7 @@ def main():
                      "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
         "whole_call_effective_gbps": whole_gbps,
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
+        "launch": ctx.launch_stats(),
     }
     if check is not None:
         out["bit_exact_vs_closed_form"] = check

@@ -644,8 +644,10 @@ uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uin
   return rounds;
 }
 
-// Ruler spacing of a level: the tuned K, unless the caller fixed it, capped so
-// that a forest of max_tails lists keeps K <= (mean list length) / 16.
+// Ruler spacing of a level: the tuned K, unless the caller fixed it, capped
+// for a forest of max_tails lists at the first power of two above
+// (mean list length) / 16, so the unreached head prefixes (about K per list)
+// stay within about an eighth of the level.
 uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails) {
   uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
   if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
