@@ -36,33 +36,47 @@ __device__ __forceinline__ void block_job_range(const BatchJob* __restrict__ job
   __syncthreads();
 }
 
+// A block covers kPer x kThreads consecutive items, so the two searches for
+// its job range (dependent loads over the whole job table) are paid once per
+// 4096 items; each thread then moves forward from its previous item's job.
+constexpr unsigned kPer = 16;
+__device__ __forceinline__ uint64_t chunk_first() {
+  return static_cast<uint64_t>(blockIdx.x) * kPer * kThreads;
+}
+__device__ __forceinline__ uint64_t chunk_last(uint64_t count) {
+  return std::min<uint64_t>(chunk_first() + uint64_t{kPer} * kThreads, count) - 1;
+}
+
 __global__ void __launch_bounds__(kThreads)
 batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nl,
                        Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
   __shared__ uint32_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  block_job_range(jobs, njobs, first, std::min(first + kThreads, nl) - 1, s_range);
-  const uint64_t e = first + threadIdx.x;
-  bool tail = false;
-  if (e < nl) {
-    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], e)];
-    const uint64_t i = e - j.base;
-    uint64_t s = j.succ[i];
-    if (s >= j.n) {
-      raise_err(err, kErrRange);
-      s = i;
+  block_job_range(jobs, njobs, chunk_first(), chunk_last(nl), s_range);
+  uint32_t jl = s_range[0];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t e = chunk_first() + q * kThreads + threadIdx.x;
+    bool tail = false;
+    if (e < nl) {
+      jl = job_of(jobs, jl, s_range[1], e);
+      const BatchJob j = jobs[jl];
+      const uint64_t i = e - j.base;
+      uint64_t s = j.succ[i];
+      if (s >= j.n) {
+        raise_err(err, kErrRange);
+        s = i;
+      }
+      tail = (s == i);
+      int64_t w;
+      if (j.w) {
+        w = j.w[i];
+        if (tail && w != 0) raise_err(err, kErrRootWeight);
+      } else {
+        w = tail ? 0 : 1;
+      }
+      rec[e] = Rec{j.base + s, w};
     }
-    tail = (s == i);
-    int64_t w;
-    if (j.w) {
-      w = j.w[i];
-      if (tail && w != 0) raise_err(err, kErrRootWeight);
-    } else {
-      w = tail ? 0 : 1;
-    }
-    rec[e] = Rec{j.base + s, w};
+    wave_append(tails, tail);
   }
-  wave_append(tails, tail);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -71,30 +85,33 @@ batch_tree_keys_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
                        uint8_t* __restrict__ is_root, Rec* __restrict__ rec, uint32_t* err,
                        unsigned long long* roots) {
   __shared__ uint32_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
-  const uint64_t t = first + threadIdx.x;
-  bool root = false;
-  if (t < nt) {
-    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
-    const uint64_t i = t - j.base;
-    uint64_t p = j.succ[i];
-    if (p >= j.n) {
-      raise_err(err, kErrRange);
-      p = i;
+  block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
+  uint32_t jl = s_range[0];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
+    bool root = false;
+    if (t < nt) {
+      jl = job_of(jobs, jl, s_range[1], t);
+      const BatchJob j = jobs[jl];
+      const uint64_t i = t - j.base;
+      uint64_t p = j.succ[i];
+      if (p >= j.n) {
+        raise_err(err, kErrRange);
+        p = i;
+      }
+      root = (p == i);
+      keys[t] = root ? static_cast<uint32_t>(nt) : static_cast<uint32_t>(j.base + p);
+      vals[t] = static_cast<uint32_t>(t);
+      is_root[t] = root;
+      if (root) {
+        rec[nl + 2 * t] = Rec{nl + 2 * t, 0};
+        rec[nl + 2 * t + 1] = Rec{nl + 2 * t + 1, 0};
+      } else {
+        rec[nl + 2 * t + 1] = Rec{nl + 2 * t, -1};
+      }
     }
-    root = (p == i);
-    keys[t] = root ? static_cast<uint32_t>(nt) : static_cast<uint32_t>(j.base + p);
-    vals[t] = static_cast<uint32_t>(t);
-    is_root[t] = root;
-    if (root) {
-      rec[nl + 2 * t] = Rec{nl + 2 * t, 0};
-      rec[nl + 2 * t + 1] = Rec{nl + 2 * t + 1, 0};
-    } else {
-      rec[nl + 2 * t + 1] = Rec{nl + 2 * t, -1};
-    }
+    wave_append(roots, root);
   }
-  wave_append(roots, root);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -130,19 +147,22 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
                        uint32_t* __restrict__ head, uint32_t* __restrict__ sib, uint32_t* err,
                        unsigned long long* roots) {
   __shared__ uint32_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
-  const uint64_t t = first + threadIdx.x;
-  bool root = false;
-  if (t < nt) {
-    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
-    const uint64_t i = t - j.base;
-    const uint64_t p = j.succ[i];
-    if (p >= j.n) raise_err(err, kErrRange);
-    root = (p == i) || p >= j.n;
-    if (!root) sib[t] = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
+  block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
+  uint32_t jl = s_range[0];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
+    bool root = false;
+    if (t < nt) {
+      jl = job_of(jobs, jl, s_range[1], t);
+      const BatchJob j = jobs[jl];
+      const uint64_t i = t - j.base;
+      const uint64_t p = j.succ[i];
+      if (p >= j.n) raise_err(err, kErrRange);
+      root = (p == i) || p >= j.n;
+      if (!root) sib[t] = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
+    }
+    wave_append(roots, root);
   }
-  wave_append(roots, root);
 }
 
 // Both tour elements of tree vertex t as one 32-byte pair (see tour_pairs_kernel).
@@ -151,31 +171,34 @@ batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint6
                         const uint32_t* __restrict__ head, const uint32_t* __restrict__ sib,
                         Rec* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
   __shared__ uint32_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  block_job_range(jobs, njobs, first, std::min(first + kThreads, nt) - 1, s_range);
-  const uint64_t t = first + threadIdx.x;
-  bool tour_head = false;
-  uint32_t h = 0xFFFFFFFFu;
-  if (t < nt) {
-    const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], t)];
-    const uint64_t i = t - j.base;
-    const uint64_t p = j.succ[i];
-    h = head[t];
-    Rec up, down;
-    if (p == i || p >= j.n) {
-      up = Rec{nl + 2 * t, 0};
-      down = Rec{nl + 2 * t + 1, 0};
-      tour_head = h != 0xFFFFFFFFu;
-    } else {
-      const uint32_t s = sib[t];
-      up = Rec{s != 0xFFFFFFFFu ? nl + 2 * uint64_t{s} + 1 : nl + 2 * (j.base + p), 1};
-      down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
+  block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
+  uint32_t jl = s_range[0];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
+    bool tour_head = false;
+    uint32_t h = 0xFFFFFFFFu;
+    if (t < nt) {
+      jl = job_of(jobs, jl, s_range[1], t);
+      const BatchJob j = jobs[jl];
+      const uint64_t i = t - j.base;
+      const uint64_t p = j.succ[i];
+      h = head[t];
+      Rec up, down;
+      if (p == i || p >= j.n) {
+        up = Rec{nl + 2 * t, 0};
+        down = Rec{nl + 2 * t + 1, 0};
+        tour_head = h != 0xFFFFFFFFu;  // a root's tour starts at its head child's down edge
+      } else {
+        const uint32_t sb = sib[t];
+        up = Rec{sb != 0xFFFFFFFFu ? nl + 2 * uint64_t{sb} + 1 : nl + 2 * (j.base + p), 1};
+        down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
+      }
+      rec[nl + 2 * t] = up;
+      rec[nl + 2 * t + 1] = down;
     }
-    rec[nl + 2 * t] = up;
-    rec[nl + 2 * t + 1] = down;
+    const uint64_t k = wave_append(nheads, tour_head);
+    if (tour_head) heads[k] = nl + 2 * uint64_t{h} + 1;
   }
-  const uint64_t k = wave_append(nheads, tour_head);
-  if (tour_head) heads[k] = nl + 2 * uint64_t{h} + 1;
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
@@ -190,15 +213,18 @@ batch_scatter_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t
                      uint64_t shift, uint32_t step, const uint64_t* __restrict__ root,
                      const int64_t* __restrict__ rank) {
   __shared__ uint32_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  block_job_range(jobs, njobs, first, std::min(first + kThreads, count) - 1, s_range);
-  const uint64_t e = first + threadIdx.x;
-  if (e >= count) return;
-  const BatchJob j = jobs[job_of(jobs, s_range[0], s_range[1], e)];
-  const uint64_t i = e - j.base;
-  const uint64_t x = shift + step * e;
-  j.root_out[i] = (root[x] - shift) / step - j.base;
-  j.rank_out[i] = rank[x];
+  block_job_range(jobs, njobs, chunk_first(), chunk_last(count), s_range);
+  uint32_t jl = s_range[0];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t e = chunk_first() + q * kThreads + threadIdx.x;
+    if (e >= count) return;
+    jl = job_of(jobs, jl, s_range[1], e);
+    const BatchJob j = jobs[jl];
+    const uint64_t i = e - j.base;
+    const uint64_t x = shift + step * e;
+    j.root_out[i] = (root[x] - shift) / step - j.base;
+    j.rank_out[i] = rank[x];
+  }
 }
 
 unsigned bits_for(uint64_t x) {
@@ -261,20 +287,20 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
     if (nl) {
       Phase ph(ctx, "batch_pack", st);
-      hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st,
+      hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
     }
     if (nt && linked) {
       Phase ph(ctx, "batch_tour", st);
       hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
                          0xFFFFFFFFu);
-      hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+      hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
-      hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+      hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
     } else if (nt) {
       Phase ph(ctx, "batch_tour", st);
-      hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st,
+      hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_trees, static_cast<uint32_t>(trees.size()), nt, nl, keys, vals, is_root, rec,
                          ctx->d_err, ctr + 1);
       radix_sort_pairs_u32(ctx, keys, vals, nt, bits_for(nt), st);
@@ -292,10 +318,10 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   check_err(ctx, st);
   Phase ph(ctx, "batch_scatter", st);
   if (nl)
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nl, kThreads)), dim3(kThreads), 0, st, d_lists,
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st, d_lists,
                        static_cast<uint32_t>(lists.size()), nl, uint64_t{0}, 1u, root, rank);
   if (nt)
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, d_trees,
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st, d_trees,
                        static_cast<uint32_t>(trees.size()), nt, nl, 2u, root, rank);
   KC_HIP(hipGetLastError());
 }

@@ -37,54 +37,70 @@ __device__ uint64_t upper_segment(const uint64_t* __restrict__ seg_off, uint64_t
   return lo;
 }
 
+// Each block covers kPer x kThreads consecutive elements: the two searches
+// for the block's segment range (dependent loads over all nseg offsets) are
+// paid once per 4096 elements, and each thread then moves forward from the
+// segment of its previous element.
+constexpr unsigned kPer = 16;
+
+__device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ seg_off, uint64_t nseg,
+                                            uint64_t total, uint64_t first, uint64_t* s_range) {
+  const uint64_t lastv = std::min<uint64_t>(first + uint64_t{kPer} * kThreads, total) - 1;
+  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
+  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kThreads)
 seg_pack_kernel(const uint32_t* __restrict__ succ, const uint64_t* __restrict__ seg_off,
                 uint64_t nseg, uint64_t total, const int64_t* __restrict__ w,
                 Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
   __shared__ uint64_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  const uint64_t lastv = std::min(first + kThreads, total) - 1;
-  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
-  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
-  __syncthreads();
-  const uint64_t i = first + threadIdx.x;
-  bool tail = false;
-  if (i < total) {
-    const uint64_t s = find_segment(seg_off, s_range[0], s_range[1], i);
-    const uint64_t base = seg_off[s];
-    const uint64_t size = seg_off[s + 1] - base;
-    uint64_t loc = succ[i];
-    if (loc >= size) {
-      raise_err(err, kErrRange);
-      loc = i - base;
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kPer * kThreads;
+  chunk_range(seg_off, nseg, total, first, s_range);
+  uint64_t lo = s_range[0];
+  const uint64_t hi = s_range[1];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t i = first + q * kThreads + threadIdx.x;
+    bool tail = false;
+    if (i < total) {
+      lo = find_segment(seg_off, lo, hi, i);
+      const uint64_t base = seg_off[lo];
+      const uint64_t size = seg_off[lo + 1] - base;
+      uint64_t loc = succ[i];
+      if (loc >= size) {
+        raise_err(err, kErrRange);
+        loc = i - base;
+      }
+      const uint64_t g = base + loc;
+      tail = (g == i);
+      int64_t wt;
+      if (w) {
+        wt = w[i];
+        if (tail && wt != 0) raise_err(err, kErrRootWeight);
+      } else {
+        wt = tail ? 0 : 1;
+      }
+      rec[i] = Rec{g, wt};
     }
-    const uint64_t g = base + loc;
-    tail = (g == i);
-    int64_t wt;
-    if (w) {
-      wt = w[i];
-      if (tail && wt != 0) raise_err(err, kErrRootWeight);
-    } else {
-      wt = tail ? 0 : 1;
-    }
-    rec[i] = Rec{g, wt};
+    wave_append(tails, tail);
   }
-  wave_append(tails, tail);
 }
 
 __global__ void __launch_bounds__(kThreads)
 seg_root_local_kernel(const uint64_t* __restrict__ root_global, const uint64_t* __restrict__ seg_off,
                       uint64_t nseg, uint64_t total, uint32_t* __restrict__ root_local) {
   __shared__ uint64_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kThreads;
-  const uint64_t lastv = std::min(first + kThreads, total) - 1;
-  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
-  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
-  __syncthreads();
-  const uint64_t i = first + threadIdx.x;
-  if (i >= total) return;
-  const uint64_t s = find_segment(seg_off, s_range[0], s_range[1], i);
-  root_local[i] = static_cast<uint32_t>(root_global[i] - seg_off[s]);
+  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kPer * kThreads;
+  chunk_range(seg_off, nseg, total, first, s_range);
+  uint64_t lo = s_range[0];
+  const uint64_t hi = s_range[1];
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t i = first + q * kThreads + threadIdx.x;
+    if (i >= total) return;
+    lo = find_segment(seg_off, lo, hi, i);
+    root_local[i] = static_cast<uint32_t>(root_global[i] - seg_off[lo]);
+  }
 }
 
 __global__ void seg_check_kernel(const uint64_t* __restrict__ seg_off, uint64_t nseg, uint32_t* err) {
@@ -115,7 +131,7 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   uint64_t* root_global = scratch.get<uint64_t>(total);
   for (;;) {
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
-    hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st, succ,
+    hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st, succ,
                        seg_off, nseg, total, w, rec, ctx->d_err,
                        reinterpret_cast<unsigned long long*>(ctx->d_counters));
     uint64_t tails = 0;
@@ -124,7 +140,7 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
     tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
   }
   check_err(ctx, st);
-  hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kThreads)), dim3(kThreads), 0, st,
+  hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st,
                      root_global, seg_off, nseg, total, root);
 }
 
