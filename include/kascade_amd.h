@@ -376,7 +376,8 @@ typedef struct kc_server_cfg {
   uint32_t num_slots;          /* batches in flight on separate streams; 0 = 3, or 1 with one_job_per_launch */
   uint64_t max_batch_elements; /* elements per batch, 0 = 2^26 */
   uint32_t one_job_per_launch; /* 1: no coalescing (the baseline mode) */
-  uint32_t reserved[3];
+  uint32_t tour_build;         /* kc_tour_build for tree jobs */
+  uint32_t reserved[2];
 } kc_server_cfg;
 
 typedef struct kc_server_stats {

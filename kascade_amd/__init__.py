@@ -51,7 +51,8 @@ class Job(ctypes.Structure):
 class ServerCfg(ctypes.Structure):
     _fields_ = [("mem_budget_bytes", ctypes.c_uint64), ("max_batch_jobs", ctypes.c_uint32),
                 ("num_slots", ctypes.c_uint32), ("max_batch_elements", ctypes.c_uint64),
-                ("one_job_per_launch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("one_job_per_launch", ctypes.c_uint32), ("tour_build", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class TraceLevel(ctypes.Structure):
@@ -562,7 +563,7 @@ class Server:
     """Continuous-batching request server (kc_server_*); jobs use host arrays."""
 
     def __init__(self, ctx, mem_budget_bytes=200 << 30, max_batch_jobs=0, num_slots=0,
-                 max_batch_elements=0, one_job_per_launch=False):
+                 max_batch_elements=0, one_job_per_launch=False, tour_build="auto"):
         self._ctx = ctx
         self._lib = ctx._lib
         cfg = ServerCfg()
@@ -571,6 +572,7 @@ class Server:
         cfg.num_slots = num_slots
         cfg.max_batch_elements = max_batch_elements
         cfg.one_job_per_launch = int(bool(one_job_per_launch))
+        cfg.tour_build = TOUR_BUILDS[tour_build]
         h = ctypes.c_void_p()
         ctx._check(self._lib.kc_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)))
         self.handle = h

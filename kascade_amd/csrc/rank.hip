@@ -8,6 +8,12 @@
 // every random access of the chase is one 16-byte load.  A claimed record is
 // overwritten in place with {ruler id | root flag, distance from the ruler}; an
 // unclaimed record keeps {successor, weight}.
+//
+// Two drivers run the same kernels.  Host-sized levels (srs_level,
+// rank_records_impl) read each level's counts on the host to size the next;
+// device-sized levels (ds_srs_level, ds_rank) fix capacities up front and keep
+// the counts in device memory, so a whole ranking is one launch sequence that
+// rank_records can capture and replay as a HIP graph.
 #include <algorithm>
 #include <cstdio>
 

@@ -81,10 +81,12 @@ uint64_t job_bytes(const kc_job& j) {
 }
 
 // One job as its own library call (the one-job-per-launch path).
-kc_status run_alone(kc::Context* ctx, const kc_job& j) {
+kc_status run_alone(kc::Context* ctx, const kc_job& j, uint32_t tour_build) {
   if (j.kind == KC_JOB_LIST)
     return kc_rank_list(ctx, j.succ, j.weight, j.n, j.root_out, j.rank_out, nullptr, ctx->stream);
-  return kc_root_tree(ctx, j.succ, j.n, j.root_out, j.rank_out, nullptr, ctx->stream);
+  kc_tree_cfg tc{};
+  tc.tour_build = static_cast<int32_t>(tour_build);
+  return kc_root_tree(ctx, j.succ, j.n, j.root_out, j.rank_out, &tc, ctx->stream);
 }
 
 void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
@@ -96,7 +98,7 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
     hipStream_t st = ctx->stream;
     kc::Scratch scratch(ctx);
     if (sv->cfg.one_job_per_launch || batch.size() == 1) {
-      for (size_t k = 0; k < batch.size(); k++) per_job[k] = run_alone(ctx, batch[k]->job);
+      for (size_t k = 0; k < batch.size(); k++) per_job[k] = run_alone(ctx, batch[k]->job, sv->cfg.tour_build);
     } else {
       std::vector<kc::BatchJob> jobs;
       struct Staging {
@@ -132,7 +134,7 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
         if (j.kind == KC_JOB_TREE) b.w = nullptr;
         jobs.push_back(b);
       }
-      kc::rank_batch_device(ctx, jobs, nullptr, KC_TOUR_AUTO, st);
+      kc::rank_batch_device(ctx, jobs, nullptr, static_cast<int>(sv->cfg.tour_build), st);
       for (auto& s : staged) {
         KC_HIP(hipMemcpyAsync(s.t->job.root_out, s.root, s.t->job.n * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         KC_HIP(hipMemcpyAsync(s.t->job.rank_out, s.rank, s.t->job.n * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -151,7 +153,7 @@ void run_batch(Server* sv, void* vctx, std::vector<Ticket*>& batch) {
   if (rerun) {
     // a bad job fails the coalesced pass: rerun the jobs one by one so that
     // only the offending job reports the error
-    for (size_t k = 0; k < batch.size(); k++) per_job[k] = run_alone(ctx, batch[k]->job);
+    for (size_t k = 0; k < batch.size(); k++) per_job[k] = run_alone(ctx, batch[k]->job, sv->cfg.tour_build);
   }
   const Clock::time_point now = Clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
@@ -219,6 +221,7 @@ extern "C" {
 kc_status kc_server_create(void* vctx, const kc_server_cfg* cfg, void** server_out) {
   auto* ctx = static_cast<kc::Context*>(vctx);
   if (!ctx || !cfg || !server_out) return KC_EINVAL;
+  if (cfg->tour_build > KC_TOUR_LINKED) return KC_EINVAL;
   *server_out = nullptr;
   auto* sv = new Server();
   sv->device = ctx->device;

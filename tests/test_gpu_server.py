@@ -45,13 +45,14 @@ def alone(ctx, kind, succ, w):
     return host(r), host(k)
 
 
-@pytest.mark.parametrize("mode", ["batched", "one_job_per_launch"])
+@pytest.mark.parametrize("mode", ["batched", "batched_sorted_tours", "one_job_per_launch"])
 def test_server_mix_matches_alone(ctx, mode):
     import torch
     import kascade_amd
     say("server", mode)
     jobs = make_jobs(120, 5)
-    sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one_job_per_launch"), num_slots=3)
+    sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one_job_per_launch"), num_slots=3,
+                            tour_build="sorted" if mode == "batched_sorted_tours" else "auto")
     tickets = []
     for kind, succ, w in jobs:
         s = dev(succ)
@@ -64,7 +65,7 @@ def test_server_mix_matches_alone(ctx, mode):
         sv.wait(t)
     st = sv.stats()
     assert st["jobs_completed"] == len(jobs)
-    if mode == "batched":
+    if mode != "one_job_per_launch":
         assert st["batches_launched"] < len(jobs)
     for (kind, succ, w), (t, s, ww, ro, ra) in zip(jobs, tickets):
         er, ek = alone(ctx, kind, succ, w)
