@@ -929,10 +929,10 @@ void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t
 // Built-in table (the autotuner's output for MI355X at the time of the last
 // sweep; a tuning_table.txt next to the library overrides it).
 const TuneRow kBuiltinTuning[] = {
-    {10, 8, 16, 1, 64}, {12, 8, 16, 1, 64}, {14, 8, 16, 1, 64},
-    {16, 8, 8, 1, 64}, {18, 8, 16, 8, 64}, {20, 8, 8, 1, 64},
-    {22, 16, 16, 1, 64}, {24, 32, 16, 2, 64}, {26, 128, 8, 2, 64},
-    {27, 128, 8, 4, 256},
+    {10, 8, 32, 4, 64}, {12, 8, 8, 2, 64}, {14, 8, 32, 4, 64},
+    {16, 16, 32, 8, 64}, {18, 8, 8, 8, 64}, {20, 16, 16, 4, 64},
+    {22, 16, 8, 1, 64}, {24, 32, 8, 2, 64}, {26, 64, 16, 2, 64},
+    {27, 128, 8, 2, 256},
 };
 
 RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
