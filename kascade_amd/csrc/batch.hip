@@ -201,6 +201,15 @@ batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint6
   }
 }
 
+// block_job[b] = the job of item 256 b (the JobOut lookup table).
+__global__ void __launch_bounds__(kThreads)
+job_block_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t count,
+                 uint32_t* __restrict__ block_job) {
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (b >= (count + 255) / 256) return;
+  block_job[b] = job_of(jobs, 0, njobs - 1, b * 256);
+}
+
 __global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (i < n) p[i] = value;
@@ -256,9 +265,17 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   if (nt >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree vertices per batch must be < 2^32 - 1"};
   Scratch scratch(ctx);
   const uint64_t total = nl + 2 * nt;
+  // size class of the whole batch; the ruler spacing per level is capped by
+  // the mean list length inside the ranking
+  RankTuning tune = tuning_for(ctx, total, cfg);
+  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
+  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  // With the ruling set at the top level the ranking writes every job's
+  // results in place; otherwise into batch-wide arrays that are scattered.
+  const bool in_place = top_level_is_ruling_set(total, tune, algo);
   Rec* rec = scratch.get<Rec>(total);
-  uint64_t* root = scratch.get<uint64_t>(total);
-  int64_t* rank = scratch.get<int64_t>(total);
+  uint64_t* root = in_place ? nullptr : scratch.get<uint64_t>(total);
+  int64_t* rank = in_place ? nullptr : scratch.get<int64_t>(total);
   BatchJob* d_lists = lists.empty() ? nullptr : scratch.get<BatchJob>(lists.size());
   BatchJob* d_trees = trees.empty() ? nullptr : scratch.get<BatchJob>(trees.size());
   if (d_lists)
@@ -278,11 +295,26 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
     vals = scratch.get<uint32_t>(nt);
     is_root = scratch.get<uint8_t>(nt);
   }
-  // size class of the whole batch; the ruler spacing per level is capped by
-  // the mean list length inside the ranking
-  RankTuning tune = tuning_for(ctx, total, cfg);
-  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
-  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  JobOut jo;
+  if (in_place) {
+    jo.lists = d_lists;
+    jo.trees = d_trees;
+    jo.nlists = static_cast<uint32_t>(lists.size());
+    jo.ntrees = static_cast<uint32_t>(trees.size());
+    jo.nl = nl;
+    if (nl) {
+      uint32_t* lb = scratch.get<uint32_t>((nl + 255) / 256);
+      hipLaunchKernelGGL(job_block_kernel, dim3(blocks_for((nl + 255) / 256, kThreads)), dim3(kThreads), 0, st,
+                         d_lists, jo.nlists, nl, lb);
+      jo.list_block = lb;
+    }
+    if (nt) {
+      uint32_t* tb = scratch.get<uint32_t>((nt + 255) / 256);
+      hipLaunchKernelGGL(job_block_kernel, dim3(blocks_for((nt + 255) / 256, kThreads)), dim3(kThreads), 0, st,
+                         d_trees, jo.ntrees, nt, tb);
+      jo.tree_block = tb;
+    }
+  }
   for (;;) {
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
     if (nl) {
@@ -311,11 +343,12 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
     read_counters(ctx, st, 0, 3, counts);
     const TourRegion tr{nl, heads, counts[2]};
     if (rank_records(ctx, rec, total, total, root, rank, counts[0] + 2 * counts[1], tune, algo, st, 0,
-                     nt ? &tr : nullptr))
+                     nt ? &tr : nullptr, in_place ? &jo : nullptr))
       break;
     tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
   }
   check_err(ctx, st);
+  if (in_place) return;
   Phase ph(ctx, "batch_scatter", st);
   if (nl)
     hipLaunchKernelGGL(batch_scatter_kernel, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st, d_lists,

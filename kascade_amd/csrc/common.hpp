@@ -245,6 +245,34 @@ __host__ __device__ inline bool out_selected(const OutMap& om, uint64_t e, uint6
 __host__ __device__ inline uint64_t out_pos(const OutMap& om, uint64_t e) {
   return (e >> om.shift) * om.stride;
 }
+// One job of a mixed batch (device-side table entry).
+struct BatchJob {
+  uint64_t n;
+  const uint64_t* succ;  // list successors / tree parents, local ids (device)
+  const int64_t* w;      // list weights (device) or null
+  uint64_t* root_out;    // local root ids (device)
+  int64_t* rank_out;     // ranks / depths (device)
+  uint64_t base;         // set by rank_batch_device: offset in its kind's id space
+  uint32_t kind;         // kc_job_kind
+  uint32_t pad;
+};
+
+// Level-0 outputs of a mixed batch written straight into each job's arrays:
+// list element e (< nl) of the job whose base is the largest <= e, and tree
+// vertex t of tour element nl + 2t (downward edges nl + 2t + 1 have none).
+// list_block / tree_block hold the job of the first item of every block of
+// 256 items, so a lookup advances over a few jobs at most.  lists == nullptr:
+// outputs go through the OutMap instead.
+struct JobOut {
+  const BatchJob* lists = nullptr;
+  const BatchJob* trees = nullptr;
+  const uint32_t* list_block = nullptr;
+  const uint32_t* tree_block = nullptr;
+  uint32_t nlists = 0;
+  uint32_t ntrees = 0;
+  uint64_t nl = 0;
+};
+
 // Euler tours inside a ranking: elements [lo, n) are interleaved tour pairs
 // (lo + 2t = upward edge / dummy, lo + 2t + 1 = downward edge).  Only the
 // upward edges' results are ever read, and the head of every tour with a
@@ -265,7 +293,11 @@ struct TourRegion {
 // the records and calls again with tune.launch = KC_LAUNCH_HOST.
 [[nodiscard]] bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                                 int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                                hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr);
+                                hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr,
+                                const JobOut* jobs_out = nullptr);
+// True when rank_records ranks n records with the sparse ruling set at the top
+// level (the only top level that writes through a JobOut).
+bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm);
 void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
 // Packs succ/weight into records with validation (range, tail weight).

@@ -40,6 +40,34 @@ enum Slot : int {
   kSlotCount = 8,
 };
 
+// Writes element e's final (root, rank): through the job table of a mixed
+// batch, or at the OutMap position (non-temporal: written once, never read).
+__device__ __forceinline__ void emit(const JobOut& jo, OutMap om, uint64_t n_out, uint64_t* __restrict__ root,
+                                     int64_t* __restrict__ rank, uint64_t e, uint64_t root_id, int64_t r) {
+  if (jo.lists == nullptr && jo.trees == nullptr) {
+    if (!out_selected(om, e, n_out)) return;
+    store_stream(&root[out_pos(om, e)], root_id >> om.shift);
+    store_stream(&rank[out_pos(om, e)], r);
+    return;
+  }
+  if (e < jo.nl) {
+    uint32_t j = jo.list_block[e >> 8];
+    while (j + 1 < jo.nlists && jo.lists[j + 1].base <= e) j++;
+    const uint64_t base = jo.lists[j].base;
+    store_stream(&jo.lists[j].root_out[e - base], root_id - base);
+    store_stream(&jo.lists[j].rank_out[e - base], r);
+    return;
+  }
+  const uint64_t x = e - jo.nl;
+  if (x & 1) return;  // downward tour edge
+  const uint64_t t = x >> 1;
+  uint32_t j = jo.tree_block[t >> 8];
+  while (j + 1 < jo.ntrees && jo.trees[j + 1].base <= t) j++;
+  const uint64_t base = jo.trees[j].base;
+  store_stream(&jo.trees[j].root_out[t - base], ((root_id - jo.nl) >> 1) - base);
+  store_stream(&jo.trees[j].rank_out[t - base], r);
+}
+
 __global__ void zero_u64_kernel(unsigned long long* p, int count) {
   int i = threadIdx.x;
   if (i < count) p[i] = 0;
@@ -256,7 +284,7 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
                  unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err,
-                 const uint64_t* __restrict__ n_dev, bool device_sized) {
+                 const uint64_t* __restrict__ n_dev, bool device_sized, JobOut jo) {
   if (*err) return;
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (n_dev && *n_dev < n) n = *n_dev;
@@ -269,29 +297,20 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
   if (v < n) {
     r = load_stream(&rec[v]);
     if (r.s & kRootFlag) {
-      if (out_selected(om, v, n_out)) {
-        const uint64_t pos = out_pos(om, v);
-        if (one_root) {
-          store_stream(&root[pos], root0 >> om.shift);
-          store_stream(&rank[pos], T8[r.s & kIdMask] - r.w);
-        } else {
-          const Rec t = T[r.s & kIdMask];
-          store_stream(&root[pos], t.s >> om.shift);
-          store_stream(&rank[pos], t.w - r.w);
-        }
+      if (one_root) {
+        emit(jo, om, n_out, root, rank, v, root0, T8[r.s & kIdMask] - r.w);
+      } else {
+        const Rec t = T[r.s & kIdMask];
+        emit(jo, om, n_out, root, rank, v, t.s, t.w - r.w);
       }
     } else if (v < tour_lo) {
       unclaimed = true;
     } else if (((v - tour_lo) & 1) == 0) {
       // an unreached upward edge of a tour: an isolated root, else a cycle
-      if (r.s == v) {
-        if (out_selected(om, v, n_out)) {
-          root[out_pos(om, v)] = v >> om.shift;
-          rank[out_pos(om, v)] = 0;
-        }
-      } else {
+      if (r.s == v)
+        emit(jo, om, n_out, root, rank, v, v, 0);
+      else
         raise_err(err, kErrCycle);
-      }
     }
   }
   const uint64_t k = wave_append(u_ctr, unclaimed);
@@ -368,17 +387,16 @@ __global__ void __launch_bounds__(kThreads)
 unclaimed_fix_kernel(const uint64_t* __restrict__ u_ids, uint64_t nu, uint64_t n_out,
                      const Rec* __restrict__ ures, const Rec* __restrict__ ext,
                      uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
-                     const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err) {
+                     const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err,
+                     JobOut jo) {
   if (*err) return;  // the prefix ranking failed: ures is not valid
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (nu_dev && *nu_dev < nu) nu = *nu_dev;
   if (k >= nu) return;
   const uint64_t v = u_ids[k];
-  if (!out_selected(om, v, n_out)) return;
   const Rec res = ures[k];
   const Rec e = ext[res.s];
-  root[out_pos(om, v)] = e.s >> om.shift;
-  rank[out_pos(om, v)] = res.w + e.w;
+  emit(jo, om, n_out, root, rank, v, e.s, res.w + e.w);
 }
 
 // ---------------------------------------------------------------------------
@@ -616,7 +634,8 @@ unsigned log2_floor(uint64_t x) {
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                        int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-                       int algorithm, int level, hipStream_t st, const TourRegion* tours = nullptr);
+                       int algorithm, int level, hipStream_t st, const TourRegion* tours = nullptr,
+                       const JobOut& jo = JobOut{});
 
 uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                          int64_t* rank, OutMap om, hipStream_t st) {
@@ -667,7 +686,7 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
 
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-               int level, hipStream_t st, const TourRegion* tours) {
+               int level, hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
   const uint64_t nF = tours ? tours->nheads : 0;
   const uint64_t list_end = std::min(n, tour_lo);
@@ -737,7 +756,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
                        n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
-                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false);
+                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false, jo);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
@@ -767,12 +786,12 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                     st);
   ctx->trace_subproblem = role;
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     u_ids, nu, n_out, ures, ext, root, rank, om, nullptr, ctx->d_err);
+                     u_ids, nu, n_out, ures, ext, root, rank, om, nullptr, ctx->d_err, jo);
 }
 
 void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                        int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
-                       int algorithm, int level, hipStream_t st, const TourRegion* tours) {
+                       int algorithm, int level, hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   if (n == 0) return;
   if (n <= tune.small_n && n <= kSmallMax) {
     if (ctx->trace_on) {
@@ -803,7 +822,7 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
     return;
   }
   srs_level(ctx, rec, n, n_out, root, rank, om, max_tails, tune, level, st,
-            tours && tours->lo < n ? tours : nullptr);
+            tours && tours->lo < n ? tours : nullptr, jo);
 }
 
 // ---------------------------------------------------------------------------
@@ -817,7 +836,7 @@ enum DsSlot : int { kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsCh
 
 void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
              int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
-             hipStream_t st, const TourRegion* tours);
+             hipStream_t st, const TourRegion* tours, const JobOut& jo = JobOut{});
 
 void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
            int64_t* rank, OutMap om, hipStream_t st) {
@@ -841,7 +860,7 @@ void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
 
 void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
                   int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int level,
-                  hipStream_t st, const TourRegion* tours) {
+                  hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
   const uint64_t nF = tours ? tours->nheads : 0;
   const uint64_t list_end = std::min(n, tour_lo);
@@ -889,7 +908,7 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out, T,
                        T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
-                       ctx->d_err, n_dev, true);
+                       ctx->d_err, n_dev, true, jo);
   }
   if (ucap == 0) return;  // tours only: nothing is left unreached
   Phase ph(ctx, "unclaimed", st);
@@ -903,12 +922,12 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
           std::min(ucap, max_tails), tune, ucap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
           level + 1, st, nullptr);
   hipLaunchKernelGGL(unclaimed_fix_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, u_ids, ucap,
-                     n_out, ures, ext, root, rank, om, nu_dev, ctx->d_err);
+                     n_out, ures, ext, root, rank, om, nu_dev, ctx->d_err, jo);
 }
 
 void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
              int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
-             hipStream_t st, const TourRegion* tours) {
+             hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   if (n == 0) return;
   if (n <= tune.small_n && n <= kSmallMax) {
     Phase ph(ctx, "small_rank", st);
@@ -921,7 +940,7 @@ void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t
     return;
   }
   ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st,
-               tours && tours->lo < n ? tours : nullptr);
+               tours && tours->lo < n ? tours : nullptr, jo);
 }
 
 }  // namespace
@@ -993,7 +1012,7 @@ namespace {
 // same kernels with the same arguments.
 std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                                  int64_t* rank, uint64_t max_tails, const RankTuning& t, int algorithm,
-                                 uint32_t out_shift, const TourRegion* tours) {
+                                 uint32_t out_shift, const TourRegion* tours, const JobOut& jo) {
   std::vector<uint64_t> k = {reinterpret_cast<uint64_t>(rec), n, n_out, reinterpret_cast<uint64_t>(root),
                              reinterpret_cast<uint64_t>(rank), max_tails, static_cast<uint64_t>(algorithm),
                              out_shift, t.spacing_top, t.spacing_deep, t.max_levels, t.small_n,
@@ -1006,23 +1025,37 @@ std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_
   } else {
     k.push_back(~uint64_t{0});
   }
+  for (const void* p : {static_cast<const void*>(jo.lists), static_cast<const void*>(jo.trees),
+                        static_cast<const void*>(jo.list_block), static_cast<const void*>(jo.tree_block)})
+    k.push_back(reinterpret_cast<uint64_t>(p));
+  k.push_back(jo.nlists);
+  k.push_back(jo.ntrees);
+  k.push_back(jo.nl);
   ctx->arena.signature(k);
   return k;
 }
 
 }  // namespace
 
+bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm) {
+  return !(n <= tune.small_n && n <= kSmallMax) && algorithm == KC_ALGO_SPARSE_RULING_SET &&
+         tune.max_levels > 0;
+}
+
 bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                   int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st, uint32_t out_shift, const TourRegion* tours) {
+                  hipStream_t st, uint32_t out_shift, const TourRegion* tours, const JobOut* jobs_out) {
   const OutMap om{1, out_shift};
+  const JobOut jo = jobs_out ? *jobs_out : JobOut{};
+  if (jobs_out && !top_level_is_ruling_set(n, tune, algorithm))
+    throw KcError{KC_EINVAL, "per-job outputs need the sparse ruling set at the top level"};
   // the per-level trace reads every level's counts on the host
   if (tune.launch == KC_LAUNCH_HOST || ctx->trace_on) {
-    rank_records_impl(ctx, rec, n, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours);
+    rank_records_impl(ctx, rec, n, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours, jo);
     return true;
   }
   auto launch_all = [&] {
-    ds_rank(ctx, rec, n, nullptr, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours);
+    ds_rank(ctx, rec, n, nullptr, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours, jo);
   };
   if (tune.launch == KC_LAUNCH_AUTO && !ctx->prof.on) {
     // A call repeated with the same arguments (and scratch layout) is
@@ -1030,7 +1063,7 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
     // per kernel.  The first occurrence runs directly (its scratch is then
     // allocated, so the capture allocates nothing).
     std::vector<uint64_t> key = launch_key(ctx, rec, n, n_out, root, rank, max_tails, tune, algorithm, out_shift,
-                                           tours);
+                                           tours, jo);
     if (ctx->graph_exec && key == ctx->graph_key) {
       KC_HIP(hipGraphLaunch(ctx->graph_exec, st));
       ctx->graph_replays++;
