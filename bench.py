@@ -2,7 +2,7 @@
 n = 2^27 elements on one MI355X (sparse ruling set through the C ABI).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--algorithm srs|pd]
-    python bench.py --workload tree|segmented
+    python bench.py --workload tree|segmented|serve
 
 Inputs: a seeded random list (successors of a Feistel permutation, int64
 weights uniform in [1,100], 0 on the tail) generated on the device and
@@ -39,9 +39,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["list", "tree", "segmented"], default="list",
+    ap.add_argument("--workload", choices=["list", "tree", "segmented", "serve"], default="list",
                     help="list: one random weighted list (the headline); tree: random rooted tree "
-                         "(Euler tour); segmented: 20,000 lists of log-uniform size in one batch")
+                         "(Euler tour); segmented: 20,000 lists of log-uniform size in one batch; "
+                         "serve: the mixed serving stream alone")
+    ap.add_argument("--serve-mode", choices=["all", "batched", "serial"], default="all",
+                    help="with --workload serve: every mode, or only the coalescing server (batched) "
+                         "or only the one-job-per-launch baseline (serial), each at capacity")
     ap.add_argument("--n", type=int, default=0, help="elements (list: 2^27, tree: 2^26 by default)")
     ap.add_argument("--segments", type=int, default=20000)
     ap.add_argument("--tour-build", choices=["auto", "linked", "sorted"], default="auto")
@@ -152,7 +156,7 @@ def c1_bench(ctx, seed, reps=20):
     return out
 
 
-def serve_bench(ctx, jobs, seed, slots):
+def serve_bench(ctx, jobs, seed, slots, mode="all"):
     """Mixed serving stream (60 % lists as C2, 40 % trees as C3, sizes
     log-uniform in [2^10, 2^24], device-resident arrays).
 
@@ -202,6 +206,11 @@ def serve_bench(ctx, jobs, seed, slots):
         sv.close()
         return st
 
+    if mode != "all":
+        # one mode at capacity, three timed passes (for profiling)
+        name = "one" if mode == "serial" else "batched"
+        passes = [run(name, None, 1 if mode == "serial" else slots) for _ in range(3)]
+        return {"jobs": jobs, "mode": mode, "capacity_jobs_per_s": [p["jobs_per_s"] for p in passes]}
     cap_serial = run("one", None, 1)["jobs_per_s"]
     cap_one = run("one", None)["jobs_per_s"]
     cap_batched = run("batched", None)["jobs_per_s"]
@@ -358,6 +367,11 @@ def main():
         return tree_main(args, ctx)
     if args.workload == "segmented":
         return segmented_main(args, ctx)
+    if args.workload == "serve":
+        out = {"metric": "serving capacity (jobs/s) and p50/p99 latency", "n_gpus": 1,
+               "serve": serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots, args.serve_mode)}
+        print(json.dumps(out), flush=True)
+        return
     n = args.n or (1 << 27)
     succ, w = ctx.gen_list(n, args.seed, weighted=True)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
