@@ -1068,25 +1068,34 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
       KC_HIP(hipGraphLaunch(ctx->graph_exec, st));
       ctx->graph_replays++;
     } else if (key == ctx->last_key) {
+      // Capture, instantiate, launch.  A capture or instantiation the runtime
+      // refuses leaves nothing executed: the sequence then runs directly.
       hipGraph_t g = nullptr;
-      KC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      try {
-        launch_all();
-      } catch (...) {
-        (void)hipStreamEndCapture(st, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      KC_HIP(hipStreamEndCapture(st, &g));
       hipGraphExec_t exec = nullptr;
-      const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      KC_HIP(e);
-      if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-      ctx->graph_exec = exec;
-      ctx->graph_key = key;
-      ctx->graph_captures++;
-      KC_HIP(hipGraphLaunch(exec, st));
+      hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        try {
+          launch_all();
+        } catch (...) {
+          (void)hipStreamEndCapture(st, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        e = hipStreamEndCapture(st, &g);
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        if (exec) (void)hipGraphExecDestroy(exec);
+        launch_all();
+      } else {
+        if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+        ctx->graph_exec = exec;
+        ctx->graph_key = key;
+        ctx->graph_captures++;
+        KC_HIP(hipGraphLaunch(exec, st));
+      }
     } else {
       launch_all();
     }
