@@ -155,3 +155,33 @@ def test_overflow_redone_host_sized(ctx):
     assert ctx.launch_stats()["host_redos"] - before == 2
     root, rank = ctx.rank_list(dev(succ), dev(w), ruler_spacing=2, launch="host")
     assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_launch_modes_edges(ctx, mode):
+    """Boundaries of the device-sized plan: the one-workgroup solver's limit
+    (8192 / 8193 records), forests made only of singletons or of 2-element
+    lists (every element a tail or a head), and tiny trees."""
+    import numpy as np
+    say("launch edges", mode)
+    for n in (8191, 8192, 8193, 8194, 16385):
+        succ, w = o.gen_list(n, n, True)
+        er, ek = o.list_closed_form(n, n, w)
+        root, rank = ctx.rank_list(dev(succ), dev(w), launch=mode)
+        assert (host(root) == er).all() and (host(rank) == ek).all(), n
+    n = 50000
+    single = np.arange(n, dtype=np.uint64)
+    root, rank = ctx.rank_list(dev(single), None, launch=mode)
+    assert (host(root) == single).all() and (host(rank) == 0).all()
+    pairs = np.arange(n, dtype=np.uint64)
+    pairs[0::2] += 1                                   # 2i -> 2i+1, 2i+1 tail
+    w = np.zeros(n, np.int64)
+    w[0::2] = 7
+    er, ek = o.sequential_walk(pairs, w)
+    root, rank = ctx.rank_list(dev(pairs), dev(w), launch=mode, ruler_spacing=4)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
+    for nv in (1, 2, 3, 4097, 4098):
+        parent = o.gen_tree(nv, nv)
+        er, ed = o.tree_closed_form(nv, nv)
+        root, depth = ctx.root_tree(dev(parent), launch=mode)
+        assert (host(root) == er).all() and (host(depth) == ed).all(), nv
