@@ -28,8 +28,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 # HBM bytes per element moved by the level-0 chase, from rocprofv3 FETCH_SIZE
 # (x2.0, calibrated on the pack kernel's known 2 GiB read) and WRITE_SIZE (x1.0)
-# at n = 2^27: 127.8 B read + 33.6 B written.
-CHASE_TRAFFIC_BYTES_PER_ELEMENT = 127.8 + 33.6
+# at n = 2^27: 8,376,237 KiB -> 127.8 B read and 4,253,444 KiB -> 32.5 B written.
+CHASE_TRAFFIC_BYTES_PER_ELEMENT = 127.8 + 32.5
 BYTES_PER_ELEMENT = 32  # succ 8 in + weight 8 in + root 8 out + rank 8 out
 CHASE_BYTES_PER_ELEMENT = 32  # chase: read (succ, w) 16 B + write (ruler, dist) 16 B per element
 
