@@ -315,6 +315,16 @@ typedef enum kc_graph_kind {
  * generator); a self loop's head moves to the next vertex. */
 kc_status kc_gen_graph(void *ctx, int32_t kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t *src_out,
                        uint64_t *dst_out, void *stream);
+/* Random geometric graph in the unit square, the role of KaGen's rgg2d in the
+ * reference's Euler-tour suites (experiments/submission/
+ * sparse-ruling-set-eulertour.suite.yaml:19-28): n seeded points with 31-bit
+ * fixed-point coordinates and an undirected edge (i, j), i < j, for every pair
+ * within radius r = sqrt(2m / (pi n^2)), so that about m edges are expected
+ * (fewer near the border).  With src_out or dst_out NULL only *m_out (the
+ * edge count) is computed; otherwise cap must be at least that count and the
+ * edges are written grouped by i ascending (identical to the CPU generator). */
+kc_status kc_gen_rgg2d(void *ctx, uint64_t n, uint64_t m, uint64_t seed, uint64_t *src_out, uint64_t *dst_out,
+                       uint64_t cap, uint64_t *m_out, void *stream);
 /* CSR of the undirected graph: xadj_out (n+1 entries), adj_out (2m entries,
  * both directions of every edge; the order within a vertex is unspecified). */
 kc_status kc_build_csr(void *ctx, const uint64_t *src, const uint64_t *dst, uint64_t n, uint64_t m,

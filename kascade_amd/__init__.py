@@ -136,6 +136,7 @@ _SIGS = {
     "kc_compute_stats": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _U64, ctypes.POINTER(Stats), _P, _P, _P, _P,
                                         _U64, _P]),
     "kc_gen_graph": (ctypes.c_int, [_P, ctypes.c_int32, _U64, _U64, _U64, _P, _P, _P]),
+    "kc_gen_rgg2d": (ctypes.c_int, [_P, _U64, _U64, _U64, _P, _P, _U64, ctypes.POINTER(_U64), _P]),
     "kc_build_csr": (ctypes.c_int, [_P, _P, _P, _U64, _U64, _P, _P, _P]),
     "kc_bfs_forest": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P]),
     "kc_trace_enable": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -370,6 +371,17 @@ class Context:
         self._check(self._lib.kc_gen_graph(self.handle, self.GRAPH_KINDS[kind], n, m, seed, _ptr(src, np.uint64),
                                            _ptr(dst, np.uint64), stream))
         return src, dst
+
+    def gen_rgg2d(self, n, m, seed, stream=None):
+        """Random geometric graph with about m edges on n points (device tensors)."""
+        import torch
+        cnt = ctypes.c_uint64()
+        self._check(self._lib.kc_gen_rgg2d(self.handle, n, m, seed, None, None, 0, ctypes.byref(cnt), stream))
+        src = torch.empty(max(cnt.value, 1), dtype=torch.uint64, device="cuda")
+        dst = torch.empty(max(cnt.value, 1), dtype=torch.uint64, device="cuda")
+        self._check(self._lib.kc_gen_rgg2d(self.handle, n, m, seed, _ptr(src, np.uint64), _ptr(dst, np.uint64),
+                                           cnt.value, ctypes.byref(cnt), stream))
+        return src[:cnt.value], dst[:cnt.value]
 
     def build_csr(self, src, dst, n, stream=None):
         m = _numel(src)

@@ -756,6 +756,25 @@ kc_status kc_gen_graph(void* vctx, int32_t kind, uint64_t n, uint64_t m, uint64_
   });
 }
 
+kc_status kc_gen_rgg2d(void* vctx, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src_out, uint64_t* dst_out,
+                       uint64_t cap, uint64_t* m_out, void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (!m_out) throw KcError{KC_EINVAL, "null edge count"};
+    hipStream_t st = pick(ctx, stream);
+    if (!src_out || !dst_out) {
+      *m_out = kc::gen_rgg2d_device(ctx, n, m, seed, nullptr, nullptr, 0, st);
+      return;
+    }
+    Scratch scratch(ctx);
+    Staged<uint64_t> s(ctx, scratch, src_out, cap, false, true, st);
+    Staged<uint64_t> d(ctx, scratch, dst_out, cap, false, true, st);
+    *m_out = kc::gen_rgg2d_device(ctx, n, m, seed, s.dev(), d.dev(), cap, st);
+    s.finish();
+    d.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
 kc_status kc_build_csr(void* vctx, const uint64_t* src, const uint64_t* dst, uint64_t n, uint64_t m,
                        uint64_t* xadj_out, uint64_t* adj_out, void* stream) {
   return guarded(vctx, [&](Context* ctx) {

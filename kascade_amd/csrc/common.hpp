@@ -307,6 +307,11 @@ void validate_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t s
 // graph inputs (graph.hip)
 void gen_graph_device(Context* ctx, int kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src, uint64_t* dst,
                       hipStream_t st);
+// Random geometric graph: returns the edge count; writes the edges when src
+// and dst are given (at most cap).  rgg2d_radius: the radius in 2^-31 units.
+uint64_t gen_rgg2d_device(Context* ctx, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src, uint64_t* dst,
+                          uint64_t cap, hipStream_t st);
+uint64_t rgg2d_radius(uint64_t n, uint64_t m);
 void build_csr_device(Context* ctx, const uint64_t* src, const uint64_t* dst, uint64_t n, uint64_t m,
                       uint64_t* xadj, uint64_t* adj, hipStream_t st);
 void bfs_forest_device(Context* ctx, const uint64_t* xadj, const uint64_t* adj, uint64_t n, uint64_t* parent,

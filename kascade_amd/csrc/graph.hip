@@ -10,6 +10,9 @@
 // visiting order.  On the device: connected components by min-label hooking
 // with pointer jumping, then one level-synchronous BFS from all component
 // minima at once.
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
 #include "device.hpp"
 
@@ -187,7 +190,131 @@ bfs_level_kernel(const uint64_t* __restrict__ xadj, const uint64_t* __restrict__
   }
 }
 
+// ---- random geometric graph (the role of KaGen's rgg2d) ---------------------
+// Point i: 31-bit fixed-point coordinates from the seed (the same arithmetic as
+// orc_gen_rgg2d).  Cells of a g x g grid are w >= r wide, so every pair within
+// r lies in the same or an adjacent cell.
+__device__ __forceinline__ uint32_t rgg_x(uint64_t seed, uint64_t i) {
+  return static_cast<uint32_t>(splitmix64(seed ^ (2 * i + 1) * 0xA0761D6478BD642Full) >> 33);
+}
+__device__ __forceinline__ uint32_t rgg_y(uint64_t seed, uint64_t i) {
+  return static_cast<uint32_t>(splitmix64(seed ^ (2 * i + 2) * 0xE7037ED1A0B428DBull) >> 33);
+}
+
+__global__ void __launch_bounds__(kThreads)
+rgg_cells_kernel(uint64_t n, uint64_t seed, uint64_t w, uint64_t g, uint32_t* __restrict__ cell,
+                 uint32_t* __restrict__ idx, unsigned long long* __restrict__ count) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+    const uint64_t c = (rgg_y(seed, i) / w) * g + rgg_x(seed, i) / w;
+    cell[i] = static_cast<uint32_t>(c);
+    idx[i] = static_cast<uint32_t>(i);
+    atomicAdd(&count[c], 1ull);
+  }
+}
+
+// Neighbours j > i within r of point i, visited in the order of the oracle:
+// neighbour cells row by row, points of a cell by ascending id (the stable
+// sort by cell keeps that order).  kWrite: write the edges at out_off, else
+// count them.
+template <bool kWrite>
+__global__ void __launch_bounds__(kThreads)
+rgg_edges_kernel(uint64_t n, uint64_t seed, uint64_t w, uint64_t g, uint64_t r2,
+                 const uint64_t* __restrict__ start, const uint32_t* __restrict__ sorted_idx,
+                 uint64_t* __restrict__ cnt, const uint64_t* __restrict__ out_off, uint64_t* __restrict__ src,
+                 uint64_t* __restrict__ dst) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+    const uint64_t xi = rgg_x(seed, i), yi = rgg_y(seed, i);
+    const int64_t cx = static_cast<int64_t>(xi / w), cy = static_cast<int64_t>(yi / w);
+    uint64_t k = kWrite ? out_off[i] : 0;
+    for (int64_t dy = -1; dy <= 1; dy++) {
+      const int64_t ay = cy + dy;
+      if (ay < 0 || ay >= static_cast<int64_t>(g)) continue;
+      for (int64_t dx = -1; dx <= 1; dx++) {
+        const int64_t ax = cx + dx;
+        if (ax < 0 || ax >= static_cast<int64_t>(g)) continue;
+        const uint64_t c = static_cast<uint64_t>(ay) * g + static_cast<uint64_t>(ax);
+        for (uint64_t p = start[c]; p < start[c + 1]; p++) {
+          const uint64_t j = sorted_idx[p];
+          if (j <= i) continue;
+          const uint64_t xj = rgg_x(seed, j), yj = rgg_y(seed, j);
+          const uint64_t ddx = xi > xj ? xi - xj : xj - xi;
+          const uint64_t ddy = yi > yj ? yi - yj : yj - yi;
+          if (ddx * ddx + ddy * ddy > r2) continue;
+          if (kWrite) {
+            src[k] = i;
+            dst[k] = j;
+          }
+          k++;
+        }
+      }
+    }
+    if (!kWrite) cnt[i] = k;
+  }
+}
+
+// The end of the last cell (after the exclusive scan of the cell counts).
+__global__ void rgg_cells_end_kernel(uint64_t* __restrict__ start, uint64_t cells, uint64_t n) {
+  if (threadIdx.x == 0) start[cells] = n;
+}
+
+__global__ void rgg_total_kernel(uint64_t n, const uint64_t* __restrict__ off, const uint64_t* __restrict__ cnt,
+                                 uint64_t* __restrict__ total) {
+  if (threadIdx.x == 0) total[0] = n ? off[n - 1] + cnt[n - 1] : 0;
+}
+
 }  // namespace
+
+uint64_t rgg2d_radius(uint64_t n, uint64_t m) {
+  if (n < 2 || m == 0) return 0;
+  const double r = std::sqrt(2.0 * static_cast<double>(m) / (3.141592653589793 * static_cast<double>(n) *
+                                                               static_cast<double>(n)));
+  if (r >= 1.0) return uint64_t{1} << 31;
+  return static_cast<uint64_t>(r * 2147483648.0);
+}
+
+uint64_t gen_rgg2d_device(Context* ctx, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src, uint64_t* dst,
+                          uint64_t cap, hipStream_t st) {
+  if (n >= (uint64_t{1} << 32)) throw KcError{KC_EINVAL, "rgg2d supports n < 2^32"};
+  const uint64_t r_fp = rgg2d_radius(n, m);
+  if (r_fp == 0) return 0;
+  uint64_t g = (uint64_t{1} << 31) / r_fp;
+  uint64_t side = 1;
+  while (side * side < n) side++;
+  g = std::max<uint64_t>(1, std::min<uint64_t>({g, side, 65535}));
+  const uint64_t w = ((uint64_t{1} << 31) + g - 1) / g;
+  const uint64_t cells = g * g;
+  Scratch scratch(ctx);
+  uint32_t* cell = scratch.get<uint32_t>(n);
+  uint32_t* idx = scratch.get<uint32_t>(n);
+  uint64_t* count = scratch.get<uint64_t>(cells);
+  uint64_t* start = scratch.get<uint64_t>(cells + 1);
+  uint64_t* cnt = scratch.get<uint64_t>(n);
+  uint64_t* off = scratch.get<uint64_t>(n);
+  uint64_t* total = scratch.get<uint64_t>(1);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(ctx, cells)), dim3(kThreads), 0, st, count, cells, uint64_t{0});
+  hipLaunchKernelGGL(rgg_cells_kernel, dim3(grid_for(ctx, n)), dim3(kThreads), 0, st, n, seed, w, g, cell, idx,
+                     reinterpret_cast<unsigned long long*>(count));
+  unsigned bits = 1;
+  while (bits < 32 && (uint64_t{1} << bits) < cells) bits++;
+  radix_sort_pairs_u32(ctx, cell, idx, n, bits, st);  // stable: ascending ids within a cell
+  exclusive_scan_u64(ctx, count, start, cells, st);
+  hipLaunchKernelGGL(rgg_cells_end_kernel, dim3(1), dim3(64), 0, st, start, cells, n);
+  hipLaunchKernelGGL(rgg_edges_kernel<false>, dim3(grid_for(ctx, n)), dim3(kThreads), 0, st, n, seed, w, g,
+                     r_fp * r_fp, start, idx, cnt, nullptr, nullptr, nullptr);
+  exclusive_scan_u64(ctx, cnt, off, n, st);
+  hipLaunchKernelGGL(rgg_total_kernel, dim3(1), dim3(64), 0, st, n, off, cnt, total);
+  uint64_t edges = 0;
+  KC_HIP(hipMemcpyAsync(&edges, total, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  KC_HIP(hipStreamSynchronize(st));
+  if (!src || !dst) return edges;
+  if (cap < edges) throw KcError{KC_EINVAL, "rgg2d: the edge arrays are smaller than the edge count"};
+  hipLaunchKernelGGL(rgg_edges_kernel<true>, dim3(grid_for(ctx, n)), dim3(kThreads), 0, st, n, seed, w, g,
+                     r_fp * r_fp, start, idx, cnt, off, src, dst);
+  KC_HIP(hipGetLastError());
+  return edges;
+}
 
 void gen_graph_device(Context* ctx, int kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t* src, uint64_t* dst,
                       hipStream_t st) {

@@ -144,7 +144,7 @@ struct Json {
 struct Config {
   std::string output_path = "stdout";
   uint64_t iterations = 1;
-  std::string input = "list";  // list | list-local | forest | tree | gnm | rmat
+  std::string input = "list";  // list | list-local | forest | tree | gnm | rmat | rgg2d
   std::string input_processing = "bfs";  // graph inputs: bfs | eulertour
   uint64_t n = uint64_t{1} << 20;
   uint64_t m = uint64_t{1} << 22;  // graph inputs: edges
@@ -166,7 +166,7 @@ void usage() {
   std::printf(
       "usage: kascade_runner [options]\n"
       "  --algorithm PointerDoubling|SparseRulingSet|EulerTour   (default SparseRulingSet)\n"
-      "  --input list|list-local|forest|tree|gnm|rmat            (default list)\n"
+      "  --input list|list-local|forest|tree|gnm|rmat|rgg2d      (default list)\n"
       "  --input-processing bfs|eulertour  (graph inputs: BFS forest, or its Euler tour)\n"
       "  --n N  --m M  --seed S  --lists L  --weighted\n"
       "  --iterations K                                          (default 1)\n"
@@ -216,7 +216,7 @@ Config parse(int argc, char** argv) {
   if (c.algorithm != "PointerDoubling" && c.algorithm != "SparseRulingSet" && c.algorithm != "EulerTour")
     die("invalid algorithm " + c.algorithm);
   if (c.input != "list" && c.input != "list-local" && c.input != "forest" && c.input != "tree" &&
-      c.input != "gnm" && c.input != "rmat")
+      c.input != "gnm" && c.input != "rmat" && c.input != "rgg2d")
     die("invalid input " + c.input);
   if (c.input_processing != "bfs" && c.input_processing != "eulertour")
     die("invalid input processing " + c.input_processing);
@@ -266,15 +266,20 @@ void generate(const Config& c, const Device& d, std::vector<uint64_t>& succ, std
   if (c.input == "tree") {
     d.check(kc_gen_tree(d.ctx, n, c.seed, ds, nullptr), "kc_gen_tree");
     hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
-  } else if (c.input == "gnm" || c.input == "rmat") {
+  } else if (c.input == "gnm" || c.input == "rmat" || c.input == "rgg2d") {
     // generate_input with "bfs" / "eulertour" processing (generation.cpp:74-90)
-    const int kind = c.input == "gnm" ? KC_GRAPH_GNM : KC_GRAPH_RMAT;
-    uint64_t* src = dalloc<uint64_t>(c.m);
-    uint64_t* dst = dalloc<uint64_t>(c.m);
+    uint64_t m = c.m;
+    if (c.input == "rgg2d") d.check(kc_gen_rgg2d(d.ctx, n, c.m, c.seed, nullptr, nullptr, 0, &m, nullptr), "kc_gen_rgg2d");
+    uint64_t* src = dalloc<uint64_t>(m ? m : 1);
+    uint64_t* dst = dalloc<uint64_t>(m ? m : 1);
     uint64_t* xadj = dalloc<uint64_t>(n + 1);
-    uint64_t* adj = dalloc<uint64_t>(2 * c.m);
-    d.check(kc_gen_graph(d.ctx, kind, n, c.m, c.seed, src, dst, nullptr), "kc_gen_graph");
-    d.check(kc_build_csr(d.ctx, src, dst, n, c.m, xadj, adj, nullptr), "kc_build_csr");
+    uint64_t* adj = dalloc<uint64_t>(2 * m ? 2 * m : 1);
+    if (c.input == "rgg2d")
+      d.check(kc_gen_rgg2d(d.ctx, n, c.m, c.seed, src, dst, m, &m, nullptr), "kc_gen_rgg2d");
+    else
+      d.check(kc_gen_graph(d.ctx, c.input == "gnm" ? KC_GRAPH_GNM : KC_GRAPH_RMAT, n, m, c.seed, src, dst, nullptr),
+              "kc_gen_graph");
+    d.check(kc_build_csr(d.ctx, src, dst, n, m, xadj, adj, nullptr), "kc_build_csr");
     d.check(kc_bfs_forest(d.ctx, xadj, adj, n, ds, nullptr), "kc_bfs_forest");
     (void)hipFree(src);
     (void)hipFree(dst);
@@ -323,7 +328,7 @@ void generate(const Config& c, const Device& d, std::vector<uint64_t>& succ, std
     hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
     if (c.weighted) hip_check(hipMemcpy(w.data(), dw, n * 8, hipMemcpyDeviceToHost), "copy");
   }
-  if (!c.weighted || c.input == "tree" || c.input == "gnm" || c.input == "rmat")
+  if (!c.weighted || c.input == "tree" || c.input == "gnm" || c.input == "rmat" || c.input == "rgg2d")
     // unit weights, as set_initial_ranking_state
     for (uint64_t i = 0; i < succ.size(); i++) w[i] = succ[i] == i ? 0 : 1;
   (void)hipFree(ds);
@@ -599,7 +604,7 @@ int main(int argc, char** argv) {
   cfg["iterations"] = Json::uinteger(c.iterations);
   {
     std::ostringstream g;
-    const bool graph = c.input == "gnm" || c.input == "rmat";
+    const bool graph = c.input == "gnm" || c.input == "rmat" || c.input == "rgg2d";
     g << c.input << ";n=" << c.n << ";seed=" << c.seed << ";weighted=" << (c.weighted ? 1 : 0);
     if (c.input == "forest") g << ";lists=" << c.lists;
     if (graph) g << ";m=" << c.m;

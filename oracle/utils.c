@@ -9,6 +9,7 @@
  * One address space stands for all MPI ranks: the request / reply exchanges of
  * the reference become direct array accesses, which changes no result.
  */
+#include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -189,6 +190,86 @@ void orc_gen_graph(int kind, uint64_t n, uint64_t m, uint64_t seed, uint64_t *sr
     src[e] = a;
     dst[e] = b;
   }
+}
+
+/* Random geometric graph in the unit square (the role of KaGen's rgg2d
+ * generator in the reference's submission suites,
+ * experiments/submission/sparse-ruling-set-eulertour.suite.yaml:19-28; not
+ * reference code: KaGen is unavailable, and the device generator in
+ * kascade_amd/csrc/graph.hip does the same arithmetic).  Point i has 31-bit
+ * fixed-point coordinates from the seed; an undirected edge joins i < j when
+ * (xi-xj)^2 + (yi-yj)^2 <= r^2 in those units, with r the radius r_fp the
+ * caller derives from the expected edge count (orc_rgg2d_radius).  Points are
+ * bucketed on a g x g grid of cells at least r wide; edges are emitted grouped
+ * by i ascending, then by neighbour cell (rows, then columns, -1..1), then by
+ * j ascending.  Writes at most cap edges and returns their number. */
+uint64_t orc_rgg2d_radius(uint64_t n, uint64_t m) {
+  if (n < 2 || m == 0) return 0;
+  double r = sqrt(2.0 * (double)m / (3.141592653589793 * (double)n * (double)n));
+  if (r >= 1.0) return UINT64_C(1) << 31;
+  return (uint64_t)(r * 2147483648.0);
+}
+
+uint64_t orc_rgg2d_cells(uint64_t n, uint64_t r_fp) {
+  uint64_t g = (UINT64_C(1) << 31) / r_fp;
+  uint64_t s = 1;
+  while (s * s < n) s++;
+  if (g > s) g = s;
+  if (g > 65535) g = 65535;
+  return g ? g : 1;
+}
+
+uint64_t orc_gen_rgg2d(uint64_t n, uint64_t m, uint64_t seed, uint64_t *src, uint64_t *dst, uint64_t cap) {
+  const uint64_t r_fp = orc_rgg2d_radius(n, m);
+  if (r_fp == 0) return 0;
+  const uint64_t g = orc_rgg2d_cells(n, r_fp);
+  const uint64_t w = ((UINT64_C(1) << 31) + g - 1) / g;
+  const uint64_t r2 = r_fp * r_fp;
+  uint32_t *x = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint32_t *y = (uint32_t *)malloc(n * sizeof(uint32_t));
+  uint64_t *cell = (uint64_t *)malloc(n * sizeof(uint64_t));
+  uint64_t *start = (uint64_t *)calloc(g * g + 1, sizeof(uint64_t));
+  uint64_t *order = (uint64_t *)malloc(n * sizeof(uint64_t));
+  for (uint64_t i = 0; i < n; i++) {
+    x[i] = (uint32_t)(orc_splitmix64(seed ^ (2 * i + 1) * UINT64_C(0xA0761D6478BD642F)) >> 33);
+    y[i] = (uint32_t)(orc_splitmix64(seed ^ (2 * i + 2) * UINT64_C(0xE7037ED1A0B428DB)) >> 33);
+    cell[i] = (y[i] / w) * g + x[i] / w;
+    start[cell[i] + 1]++;
+  }
+  for (uint64_t c = 0; c < g * g; c++) start[c + 1] += start[c];
+  uint64_t *fill = (uint64_t *)malloc((g * g + 1) * sizeof(uint64_t));
+  memcpy(fill, start, (g * g + 1) * sizeof(uint64_t));
+  for (uint64_t i = 0; i < n; i++) order[fill[cell[i]]++] = i; /* stable: ascending i per cell */
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; i++) {
+    const int64_t cx = (int64_t)(x[i] / w), cy = (int64_t)(y[i] / w);
+    for (int64_t dy = -1; dy <= 1; dy++) {
+      for (int64_t dx = -1; dx <= 1; dx++) {
+        const int64_t ax = cx + dx, ay = cy + dy;
+        if (ax < 0 || ay < 0 || ax >= (int64_t)g || ay >= (int64_t)g) continue;
+        const uint64_t c = (uint64_t)ay * g + (uint64_t)ax;
+        for (uint64_t p = start[c]; p < start[c + 1]; p++) {
+          const uint64_t j = order[p];
+          if (j <= i) continue;
+          const uint64_t ddx = x[i] > x[j] ? x[i] - x[j] : x[j] - x[i];
+          const uint64_t ddy = y[i] > y[j] ? y[i] - y[j] : y[j] - y[i];
+          if (ddx * ddx + ddy * ddy > r2) continue;
+          if (k < cap) {
+            src[k] = i;
+            dst[k] = j;
+          }
+          k++;
+        }
+      }
+    }
+  }
+  free(x);
+  free(y);
+  free(cell);
+  free(start);
+  free(fill);
+  free(order);
+  return k;
 }
 
 /* generate_bfs_tree, bench/input/bfs_forest_generation.cpp:52-76, on one rank:

@@ -53,6 +53,8 @@ def lib():
                                               c.c_void_p, c.c_void_p, c.c_void_p]),
             "orc_gen_graph": (None, [c.c_int, c.c_uint64, c.c_uint64, c.c_uint64, u64p, u64p]),
             "orc_bfs_forest": (None, [u64p, u64p, c.c_uint64, u64p]),
+            "orc_rgg2d_radius": (c.c_uint64, [c.c_uint64, c.c_uint64]),
+            "orc_gen_rgg2d": (c.c_uint64, [c.c_uint64, c.c_uint64, c.c_uint64, u64p, u64p, c.c_uint64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -238,6 +240,19 @@ def gen_graph(kind, n, m, seed):
     dst = np.empty(max(m, 1), np.uint64)
     lib().orc_gen_graph(GRAPH_KINDS[kind], n, m, seed, _p64(src), _p64(dst))
     return src[:m], dst[:m]
+
+
+def gen_rgg2d(n, m, seed):
+    """The oracle's random geometric graph: (src, dst) in its emission order."""
+    cnt = lib().orc_gen_rgg2d(n, m, seed, None, None, 0)
+    src = np.empty(max(cnt, 1), np.uint64)
+    dst = np.empty(max(cnt, 1), np.uint64)
+    lib().orc_gen_rgg2d(n, m, seed, _p64(src), _p64(dst), cnt)
+    return src[:cnt], dst[:cnt]
+
+
+def rgg2d_radius(n, m):
+    return lib().orc_rgg2d_radius(n, m)
 
 
 def csr(src, dst, n):

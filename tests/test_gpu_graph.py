@@ -72,3 +72,47 @@ def test_bfs_forest_rejects_asymmetric_adjacency(ctx):
     with pytest.raises(kascade_amd.KascadeError) as e:
         ctx.bfs_forest(dev(xadj), dev(adj))
     assert e.value.status == kascade_amd.KC_EINVAL
+
+
+@pytest.mark.parametrize("n,m", [(0, 0), (1, 3), (2, 1), (500, 1500), (1 << 16, 1 << 18), (1 << 19, 1 << 22)])
+def test_rgg2d_matches_oracle(ctx, n, m):
+    """The device random geometric graph equals the oracle's edge for edge and
+    in order (the oracle is pinned by an all-pairs check); its BFS forest
+    gives the oracle's roots and depths (the submission suites' rgg2d +
+    eulertour pipeline at its per-rank size, 2^19 vertices and 2^22 edges)."""
+    say("rgg2d", n, m)
+    seed = 11 + n
+    src, dst = ctx.gen_rgg2d(n, m, seed)
+    es, ed = o.gen_rgg2d(n, m, seed)
+    assert (host(src) == es).all() and (host(dst) == ed).all()
+    if n < 2:
+        return
+    xadj, adj = ctx.build_csr(src, dst, n)
+    ex, ea = o.csr(es, ed, n)
+    parent = ctx.bfs_forest(xadj, adj)
+    er, edp = forest_root_depth(o.bfs_forest(ex, ea))
+    r, d = forest_root_depth(host(parent))
+    assert (r == er).all() and (d == edp).all()
+    if n >= 1 << 16:
+        root, depth = ctx.root_tree(parent)
+        assert (host(root) == er).all() and (host(depth) == edp).all()
+
+
+def test_rgg2d_host_arrays_and_capacity(ctx):
+    import ctypes
+    import kascade_amd
+    say("rgg2d host")
+    n, m = 3000, 9000
+    es, ed = o.gen_rgg2d(n, m, 4)
+    cnt = ctypes.c_uint64()
+    lib = kascade_amd.load()
+    assert lib.kc_gen_rgg2d(ctx.handle, n, m, 4, None, None, 0, ctypes.byref(cnt), None) == kascade_amd.KC_OK
+    assert cnt.value == len(es)
+    src = np.empty(len(es), np.uint64)
+    dst = np.empty(len(es), np.uint64)
+    assert lib.kc_gen_rgg2d(ctx.handle, n, m, 4, src.ctypes.data, dst.ctypes.data, len(es), ctypes.byref(cnt),
+                            None) == kascade_amd.KC_OK
+    assert (src == es).all() and (dst == ed).all()
+    small = np.empty(10, np.uint64)
+    assert lib.kc_gen_rgg2d(ctx.handle, n, m, 4, small.ctypes.data, small.ctypes.data, 10, ctypes.byref(cnt),
+                            None) == kascade_amd.KC_EINVAL

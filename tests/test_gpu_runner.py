@@ -112,3 +112,8 @@ def test_runner_graph_inputs(tmp_path):
                  "--input-processing", "eulertour")
     assert r.returncode == 0 and "Rank array matches reference." in r.stderr
     assert rep["stats"]["input_distribution"]["total_size"] == 200000
+    # the submission suites' rgg2d input with eulertour processing
+    r, rep = run(tmp_path, "--algorithm", "SparseRulingSet", "--input", "rgg2d", "--n", "65536", "--m", "262144",
+                 "--input-processing", "eulertour", "--statistics", "basic")
+    assert r.returncode == 0 and "Rank array matches reference." in r.stderr
+    assert rep["stats"]["input_distribution"]["total_size"] == 2 * 65536

@@ -412,3 +412,47 @@ def test_rmat_is_skewed():
     g_src, g_dst = o.gen_graph("gnm", n, m, 3)
     gdeg = np.bincount(np.concatenate([g_src, g_dst]).astype(np.int64), minlength=n)
     assert gdeg.max() < 4 * gdeg.mean()
+
+
+def _rgg2d_points(n, seed):
+    """The generator's definition of point i (31-bit fixed point)."""
+    m64 = (1 << 64) - 1
+
+    def sm(x):
+        x = (x + 0x9E3779B97F4A7C15) & m64
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & m64
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & m64
+        return x ^ (x >> 31)
+
+    xs = [sm(seed ^ (((2 * i + 1) * 0xA0761D6478BD642F) & m64)) >> 33 for i in range(n)]
+    ys = [sm(seed ^ (((2 * i + 2) * 0xE7037ED1A0B428DB) & m64)) >> 33 for i in range(n)]
+    return np.array(xs, np.int64), np.array(ys, np.int64)
+
+
+@pytest.mark.parametrize("n,m", [(0, 0), (1, 5), (2, 1), (50, 40), (700, 3000), (1500, 100000)])
+def test_rgg2d_grid_matches_brute_force(n, m):
+    """The grid neighbour search finds exactly the pairs within the radius
+    (all-pairs check on the same points), each once with i < j, grouped by i."""
+    seed = 17 + n
+    src, dst = o.gen_rgg2d(n, m, seed)
+    r = o.rgg2d_radius(n, m)
+    exp = set()
+    if n >= 2 and r > 0:
+        x, y = _rgg2d_points(n, seed)
+        dx = x[:, None] - x[None, :]
+        dy = y[:, None] - y[None, :]
+        close = (dx * dx + dy * dy) <= r * r
+        iu, ju = np.nonzero(np.triu(close, 1))
+        exp = set(zip(iu.tolist(), ju.tolist()))
+    got = list(zip(src.tolist(), dst.tolist()))
+    assert len(got) == len(set(got)) == len(exp)
+    assert set(got) == exp
+    assert all(a < b for a, b in got)
+    assert list(src) == sorted(src)
+
+
+def test_rgg2d_edge_count_near_target():
+    n, m = 4000, 20000
+    src, _ = o.gen_rgg2d(n, m, 3)
+    # pairs near the border lose part of their disc: expect a little under m
+    assert 0.85 * m < len(src) <= 1.02 * m
