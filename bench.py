@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-check", action="store_true", help="skip the bit-exact check of the last step")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--no-c1", action="store_true", help="skip the C1 section (2^20 unit-weight list vs CPU)")
+    ap.add_argument("--no-unit-list", action="store_true",
+                    help="skip the 2^20 unit-weight list section (GPU beside the CPU path on 4 ranks)")
     ap.add_argument("--serve-jobs", type=int, default=300)
     ap.add_argument("--serve-slots", type=int, default=3)
     return ap.parse_args()
@@ -118,8 +119,8 @@ def cpu_baseline(n_sample, seed):
             "bit_exact_vs_closed_form": ok and ok4 and ok_pd and ok_seq}
 
 
-def c1_bench(ctx, seed, reps=20):
-    """C1: one random list of 2^20 elements, unit weights.  GPU sparse ruling
+def unit_list_bench(ctx, seed, reps=20):
+    """One random list of 2^20 elements, unit weights.  GPU sparse ruling
     set and pointer doubling beside the oracle's restatements of the reference's
     sync pointer doubling and sparse ruling set on 4 emulated MPI ranks; all
     outputs compared."""
@@ -157,7 +158,7 @@ def c1_bench(ctx, seed, reps=20):
 
 
 def serve_bench(ctx, jobs, seed, slots, mode="all"):
-    """Mixed serving stream (60 % lists as C2, 40 % trees as C3, sizes
+    """Mixed serving stream (60 % weighted random lists, 40 % random rooted trees, sizes
     log-uniform in [2^10, 2^24], device-resident arrays).
 
     Baseline: one job per launch, one job at a time (submit, rank, wait, next).
@@ -268,7 +269,7 @@ def timed(ctx, fn, steps, warmup):
 
 
 def tree_main(args, ctx):
-    """Random rooted tree (C3): parent of generation index i uniform in [0, i),
+    """Random rooted tree: parent of generation index i uniform in [0, i),
     relabelled by a seeded permutation; Euler-tour rooting."""
     import torch
     n = args.n or (1 << 26)
@@ -281,7 +282,7 @@ def tree_main(args, ctx):
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "int64",
            "data": "synthetic: random recursive tree, seeded relabelling",
-           "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting (C3)", "n": n, "seed": args.seed,
+           "config": {"workload": f"random rooted tree n={n}, Euler-tour rooting", "n": n, "seed": args.seed,
                       "tour_build": args.tour_build},
            "effective_gbps": 24 * n / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
@@ -315,7 +316,7 @@ def tree_cpu_baseline(n_sample, seed):
 
 def segmented_main(args, ctx):
     """20,000 lists, sizes log-uniform in [2^8, 2^20], 30 % locality-biased,
-    u32 local ids, unit weights, ranked in one segmented call (C4)."""
+    u32 local ids, unit weights, ranked in one segmented call."""
     import numpy as np
     import torch
     rng = np.random.default_rng(args.seed)
@@ -336,7 +337,7 @@ def segmented_main(args, ctx):
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
            "data": "synthetic: seeded random permutations, 30% locality-biased (blocks of 10)",
-           "config": {"workload": f"{args.segments} lists log-uniform in [2^8, 2^20] (C4)",
+           "config": {"workload": f"{args.segments} lists log-uniform in [2^8, 2^20], one segmented call",
                       "segments": args.segments, "total_elements": total, "seed": args.seed},
            "effective_gbps": (16 * total + 8 * args.segments) / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
@@ -416,7 +417,7 @@ def main():
         "dtype": "int64",
         "data": "synthetic: seeded random permutation list, int64 weights uniform in [1,100], tail weight 0",
         "config": {"workload": f"single random list n={n}, int64 weights, {args.algorithm} "
-                               f"(C2: uint64 successors, weighted ranks bit-exact)",
+                               f"(uint64 successors, weighted ranks bit-exact)",
                    "n": n, "algorithm": args.algorithm, "seed": args.seed},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
@@ -450,8 +451,8 @@ def main():
     if traffic is not None:
         out["roofline"]["amplification"] = traffic / alg_bytes
         out["roofline"]["traffic_gbps"] = traffic / (dom_ms * 1e-3) / 1e9
-    if not args.no_c1:
-        out["c1"] = c1_bench(ctx, args.seed)
+    if not args.no_unit_list:
+        out["unit_list_2e20"] = unit_list_bench(ctx, args.seed)
     if not args.no_serve:
         out["serve"] = serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots)
     if not args.no_cpu_baseline:

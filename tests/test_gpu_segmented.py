@@ -70,7 +70,7 @@ def test_segmented_small(ctx, algo, weighted):
 
 
 def test_segmented_log_uniform(ctx):
-    """A reduced C4 batch: 2000 lists, sizes log-uniform in [2^8, 2^16], 30 %
+    """A reduced segmented batch: 2000 lists, sizes log-uniform in [2^8, 2^16], 30 %
     locality-biased."""
     say("seg log-uniform")
     rng = np.random.default_rng(11)
