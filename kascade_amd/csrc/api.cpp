@@ -440,7 +440,8 @@ uint64_t kc_context_scratch_bytes(void* vctx) {
 
 kc_status kc_context_trim(void* vctx) {
   return guarded(vctx, [&](Context* ctx) {
-    KC_HIP(hipStreamSynchronize(ctx->stream));
+    // the scratch may have served calls on caller streams too
+    KC_HIP(hipDeviceSynchronize());
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     ctx->graph_exec = nullptr;
     ctx->graph_key.clear();

@@ -844,7 +844,8 @@ void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
   Scratch scratch(ctx);
   Rec* other = scratch.get<Rec>(n);
   const uint32_t rounds = log2_floor(n) + 2;
-  uint32_t* active = scratch.get<uint32_t>(rounds);
+  // one flag per round, zeroed as (rounds + 1) / 2 u64 words
+  uint32_t* active = scratch.get<uint32_t>(rounds + 1);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, reinterpret_cast<unsigned long long*>(active),
                      static_cast<int>((rounds + 1) / 2));
   const unsigned blocks = blocks_for(n, kThreads);
