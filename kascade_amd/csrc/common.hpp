@@ -271,6 +271,11 @@ struct JobOut {
   uint32_t nlists = 0;
   uint32_t ntrees = 0;
   uint64_t nl = 0;
+  // Segmented batch: roots are written as u32 ids local to their segment
+  // (segment s owns [seg_off[s], seg_off[s + 1])), ranks at the element.
+  uint32_t* seg_root = nullptr;
+  const uint64_t* seg_off = nullptr;
+  uint64_t nseg = 0;
 };
 
 // Euler tours inside a ranking: elements [lo, n) are interleaved tour pairs

@@ -60,6 +60,17 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long* ctr, bool mi
 
 __device__ __forceinline__ void raise_err(uint32_t* err, uint32_t bits) { atomicOr(err, bits); }
 
+// Segment of global element i: the largest s with seg_off[s] <= i (an empty
+// segment shares its offset with the next one, and the search takes the last).
+__device__ __forceinline__ uint64_t segment_of(const uint64_t* __restrict__ seg_off, uint64_t nseg, uint64_t i) {
+  uint64_t lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) / 2;
+    if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Streaming accesses (read or written once, coalesced): non-temporal, so they
 // do not evict the gather tables the same kernel keeps in L2.
 __device__ __forceinline__ Rec load_stream(const Rec* p) {

@@ -40,10 +40,16 @@ enum Slot : int {
   kSlotCount = 8,
 };
 
-// Writes element e's final (root, rank): through the job table of a mixed
-// batch, or at the OutMap position (non-temporal: written once, never read).
+// Writes element e's final (root, rank): as a segment-local u32 root (root_id
+// is then already local), through the job table of a mixed batch, or at the
+// OutMap position (non-temporal: written once, never read).
 __device__ __forceinline__ void emit(const JobOut& jo, OutMap om, uint64_t n_out, uint64_t* __restrict__ root,
                                      int64_t* __restrict__ rank, uint64_t e, uint64_t root_id, int64_t r) {
+  if (jo.seg_root) {
+    store_stream(&jo.seg_root[e], static_cast<uint32_t>(root_id));
+    store_stream(&rank[e], r);
+    return;
+  }
   if (jo.lists == nullptr && jo.trees == nullptr) {
     if (!out_selected(om, e, n_out)) return;
     store_stream(&root[out_pos(om, e)], root_id >> om.shift);
@@ -273,6 +279,22 @@ map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uin
   T8[i] = t.w;
 }
 
+// Segmented batches: the tails of the ruler sublist (self loops; the roots of
+// every ruler) get their segment-local id in place of the global one, so the
+// roots that map_roots_kernel hands to propagation are already local.  Only
+// root ids of the sublist are ever looked up in sub_orig.
+__global__ void __launch_bounds__(kThreads)
+seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig, uint64_t m,
+                       const uint64_t* __restrict__ m_dev, const uint64_t* __restrict__ seg_off, uint64_t nseg,
+                       const uint32_t* __restrict__ err) {
+  if (*err) return;
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (m_dev && *m_dev < m) m = *m_dev;
+  if (j >= m || sub[j].s != j) return;
+  const uint64_t g = sub_orig[j];
+  sub_orig[j] = g - seg_off[segment_of(seg_off, nseg, g)];
+}
+
 // Rank propagation (ruler_propagation.hpp:148-257, pull mode): every claimed
 // element reads its ruler's (root, rank) and subtracts its distance from the
 // ruler.  Unclaimed elements (list prefixes no ruler reaches, the analogue of
@@ -353,7 +375,8 @@ __global__ void __launch_bounds__(kThreads)
 unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
                        const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T,
                        Rec* __restrict__ urec, Rec* __restrict__ ext,
-                       const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err) {
+                       const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err,
+                       const uint64_t* __restrict__ seg_off, uint64_t nseg) {
   if (*err) return;
   uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (nu_dev && *nu_dev < nu) nu = *nu_dev;
@@ -362,8 +385,9 @@ unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__
   const Rec r0 = u_orig[k];
   const uint64_t s = r0.s;
   if (s == v) {
+    // a tail no ruler reached: its own root (segment-local in a segmented batch)
     urec[k] = Rec{k, 0};
-    ext[k] = Rec{v, 0};
+    ext[k] = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
     return;
   }
   const Rec ms = rec[s];
@@ -717,6 +741,9 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   uint64_t extra = 0;
   read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
   const uint64_t m = nR + nF + extra;
+  if (jo.seg_off)
+    hipLaunchKernelGGL(seg_local_tails_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, sub, sub_orig,
+                       m, nullptr, jo.seg_off, jo.nseg, ctx->d_err);
   size_t trace_at = 0;
   if (ctx->trace_on) {
     // the counter block was copied to the pinned mirror by read_counters
@@ -778,7 +805,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* ext = scratch.get<Rec>(nu);
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err);
+                     rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err, jo.seg_off, jo.nseg);
   // every unclaimed prefix belongs to a different list of this level
   ctx->trace_subproblem = 1;
   rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, std::min(nu, max_tails), tune,
@@ -892,6 +919,9 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   }
   uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
   hipLaunchKernelGGL(level_size_kernel, dim3(1), dim3(64), 0, st, n_dev, n, logK, nF, ctr + kDsExtra, m_dev);
+  if (jo.seg_off)
+    hipLaunchKernelGGL(seg_local_tails_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, sub, sub_orig,
+                       cap, m_dev, jo.seg_off, jo.nseg, ctx->d_err);
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
   ds_rank(ctx, sub, cap, m_dev, cap, &T[0].s, &T[0].w, OutMap{2, 0}, lists, tune,
           cap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1, st, nullptr);
@@ -918,7 +948,7 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   Rec* ures = scratch.get<Rec>(ucap);
   const unsigned long long* nu_dev = ctr + kDsUnclaimed;
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
-                     u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err);
+                     u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err, jo.seg_off, jo.nseg);
   ds_rank(ctx, urec, ucap, reinterpret_cast<const uint64_t*>(nu_dev), ucap, &ures[0].s, &ures[0].w, OutMap{2, 0},
           std::min(ucap, max_tails), tune, ucap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
           level + 1, st, nullptr);
@@ -1032,6 +1062,9 @@ std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_
   k.push_back(jo.nlists);
   k.push_back(jo.ntrees);
   k.push_back(jo.nl);
+  k.push_back(reinterpret_cast<uint64_t>(jo.seg_root));
+  k.push_back(reinterpret_cast<uint64_t>(jo.seg_off));
+  k.push_back(jo.nseg);
   ctx->arena.signature(k);
   return k;
 }

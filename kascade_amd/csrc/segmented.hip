@@ -27,16 +27,6 @@ __device__ __forceinline__ uint64_t find_segment(const uint64_t* __restrict__ se
   return lo;
 }
 
-__device__ uint64_t upper_segment(const uint64_t* __restrict__ seg_off, uint64_t nseg, uint64_t i) {
-  // largest s with seg_off[s] <= i (empty segments skipped by taking the last)
-  uint64_t lo = 0, hi = nseg - 1;
-  while (lo < hi) {
-    uint64_t mid = (lo + hi + 1) / 2;
-    if (seg_off[mid] <= i) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
 // Each block covers kPer x kThreads consecutive elements: the two searches
 // for the block's segment range (dependent loads over all nseg offsets) are
 // paid once per 4096 elements, and each thread then moves forward from the
@@ -46,8 +36,8 @@ constexpr unsigned kPer = 16;
 __device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ seg_off, uint64_t nseg,
                                             uint64_t total, uint64_t first, uint64_t* s_range) {
   const uint64_t lastv = std::min<uint64_t>(first + uint64_t{kPer} * kThreads, total) - 1;
-  if (threadIdx.x == 0) s_range[0] = upper_segment(seg_off, nseg, first);
-  if (threadIdx.x == 1) s_range[1] = upper_segment(seg_off, nseg, lastv);
+  if (threadIdx.x == 0) s_range[0] = segment_of(seg_off, nseg, first);
+  if (threadIdx.x == 1) s_range[1] = segment_of(seg_off, nseg, lastv);
   __syncthreads();
 }
 
@@ -126,9 +116,17 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  // With the ruling set at the top level the ranking writes segment-local
+  // roots directly; otherwise global roots go to a scratch array and a last
+  // pass converts them.
+  const bool local_roots = top_level_is_ruling_set(total, tune, algo);
   Scratch scratch(ctx);
   Rec* rec = scratch.get<Rec>(total);
-  uint64_t* root_global = scratch.get<uint64_t>(total);
+  uint64_t* root_global = local_roots ? nullptr : scratch.get<uint64_t>(total);
+  JobOut jo;
+  jo.seg_root = root;
+  jo.seg_off = seg_off;
+  jo.nseg = nseg;
   for (;;) {
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
     hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st, succ,
@@ -136,10 +134,13 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
                        reinterpret_cast<unsigned long long*>(ctx->d_counters));
     uint64_t tails = 0;
     read_counters(ctx, st, 0, 1, &tails);
-    if (rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st)) break;
+    if (rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st, 0, nullptr,
+                     local_roots ? &jo : nullptr))
+      break;
     tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
   }
   check_err(ctx, st);
+  if (local_roots) return;
   hipLaunchKernelGGL(seg_root_local_kernel, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st,
                      root_global, seg_off, nseg, total, root);
 }
