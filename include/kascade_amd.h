@@ -144,6 +144,13 @@ uint32_t kc_trace_read(void *ctx, kc_trace_level *out, uint32_t max_entries);
  * chunk, one-workgroup threshold) and returns the number of loaded rows. */
 kc_status kc_tuning_set(void *ctx, const uint32_t *rows, uint32_t nrows);
 uint32_t kc_tuning_get(void *ctx, uint64_t n, uint32_t *out5);
+/* The same per workload kind (kc_job_kind): KC_JOB_LIST is the table above;
+ * KC_JOB_TREE is tree rooting, whose classes count vertices (the tour has 2n
+ * records) and whose rows the table file prefixes with "tree".  Mixed
+ * batches use the list table.  No reference counterpart (the reference picks
+ * ruler counts by formula, ruler_selection.hpp:18-79). */
+kc_status kc_tuning_set_kind(void *ctx, int32_t kind, const uint32_t *rows, uint32_t nrows);
+uint32_t kc_tuning_get_kind(void *ctx, int32_t kind, uint64_t n, uint32_t *out5);
 
 /* ---- list ranking ------------------------------------------------------- */
 

@@ -108,6 +108,8 @@ _SIGS = {
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint64)]),
     "kc_tuning_set": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]),
     "kc_tuning_get": (ctypes.c_uint32, [_P, _U64, ctypes.POINTER(ctypes.c_uint32)]),
+    "kc_tuning_set_kind": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32]),
+    "kc_tuning_get_kind": (ctypes.c_uint32, [_P, ctypes.c_int32, _U64, ctypes.POINTER(ctypes.c_uint32)]),
     "kc_set_initial_ranking_state": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
     "kc_rank_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, ctypes.POINTER(RankCfg), _P]),
     "kc_pointer_doubling": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
@@ -199,6 +201,7 @@ def _numel(a):
 
 
 LAUNCHES = {"auto": 0, "device": 1, "host": 2}
+JOB_KINDS = {"list": 0, "tree": 1}  # kc_job_kind
 
 
 def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
@@ -531,15 +534,16 @@ class Context:
         return r.value, ms.value
 
     # ---- tuning ---------------------------------------------------------------
-    def tuning_set(self, rows):
-        """rows: iterable of (log2n, spacing_top, spacing_deep, chase_blocks_per_cu, chase_chunk)."""
+    def tuning_set(self, rows, kind="list"):
+        """rows: iterable of (log2n, spacing_top, spacing_deep, chase_blocks_per_cu, chase_chunk);
+        kind "list" (classes count elements) or "tree" (classes count vertices)."""
         flat = [int(x) for r in rows for x in r]
         arr = (ctypes.c_uint32 * max(1, len(flat)))(*flat)
-        self._check(self._lib.kc_tuning_set(self.handle, arr, len(flat) // 5))
+        self._check(self._lib.kc_tuning_set_kind(self.handle, JOB_KINDS[kind], arr, len(flat) // 5))
 
-    def tuning_get(self, n):
+    def tuning_get(self, n, kind="list"):
         out = (ctypes.c_uint32 * 5)()
-        rows = self._lib.kc_tuning_get(self.handle, n, out)
+        rows = self._lib.kc_tuning_get_kind(self.handle, JOB_KINDS[kind], n, out)
         return {"spacing_top": out[0], "spacing_deep": out[1], "chase_blocks_per_cu": out[2],
                 "chase_chunk": out[3], "small_n": out[4], "table_rows": rows}
 
@@ -595,7 +599,7 @@ class Server:
         root_out = np.empty(n, np.uint64) if root_out is None else root_out
         rank_out = np.empty(n, np.int64) if rank_out is None else rank_out
         j = Job()
-        j.kind = {"list": 0, "tree": 1}[kind] if isinstance(kind, str) else int(kind)
+        j.kind = JOB_KINDS[kind] if isinstance(kind, str) else int(kind)
         j.n = n
         j.succ = _ptr(succ, np.uint64)
         j.weight = _ptr(weight, np.int64)

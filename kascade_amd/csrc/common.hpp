@@ -115,7 +115,8 @@ constexpr int kErrSlot = kCounterSlots - 1;
 struct TuneRow;
 struct Context {
   Profiler prof;
-  std::vector<TuneRow> tuning;  // sorted by log2n; empty = built-in table
+  std::vector<TuneRow> tuning;       // lists; sorted by log2n; empty = built-in table
+  std::vector<TuneRow> tuning_tree;  // tree rooting, by vertices; the same
   int device = 0;
   uint64_t budget = 0;
   hipStream_t stream = nullptr;
@@ -221,8 +222,11 @@ struct TuneRow {
   uint32_t chase_blocks_per_cu;
   uint32_t chase_chunk;
 };
-RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg);
-// Loads a tuning table file (one row per line, '#' comments); false if absent.
+// The knobs for ranking n list elements, or (tree) for rooting a tree of n
+// vertices, whose tour has 2n records: tours have their own table.
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bool tree = false);
+// Loads a tuning table file (one row per line, '#' comments; rows prefixed
+// with "tree" are tree-rooting rows); false if absent.
 bool load_tuning_file(Context* ctx, const std::string& path);
 
 // ---- internal entry points (device pointers only) -----------------------

@@ -203,7 +203,7 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   uint64_t* heads = scratch.get<uint64_t>(n);
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
-  RankTuning tune = tuning_for(ctx, 2 * n, rc);
+  RankTuning tune = tuning_for(ctx, n, rc, /*tree=*/true);
   int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
   if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
     throw KcError{KC_EINVAL, "invalid algorithm"};

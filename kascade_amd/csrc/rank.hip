@@ -985,17 +985,26 @@ const TuneRow kBuiltinTuning[] = {
     {27, 128, 8, 2, 256},
 };
 
-RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg) {
+// Tree rooting, by vertices (the tour has twice as many records).
+const TuneRow kBuiltinTreeTuning[] = {
+    {10, 8, 32, 2, 64}, {12, 8, 8, 4, 64},  {14, 8, 8, 4, 64},
+    {16, 8, 8, 8, 64},  {18, 8, 16, 8, 64}, {20, 8, 8, 1, 64},
+    {22, 8, 8, 2, 256}, {24, 32, 16, 2, 64}, {26, 128, 16, 2, 64},
+};
+
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bool tree) {
   RankTuning t{};
   t.max_levels = 8;
   t.small_n = kSmallMax;
   uint32_t lg = 0;
   while (lg < 63 && (uint64_t{2} << lg) <= n) lg++;
-  const TuneRow* rows = kBuiltinTuning;
-  size_t nrows = sizeof(kBuiltinTuning) / sizeof(kBuiltinTuning[0]);
-  if (ctx && !ctx->tuning.empty()) {
-    rows = ctx->tuning.data();
-    nrows = ctx->tuning.size();
+  const TuneRow* rows = tree ? kBuiltinTreeTuning : kBuiltinTuning;
+  size_t nrows = tree ? sizeof(kBuiltinTreeTuning) / sizeof(kBuiltinTreeTuning[0])
+                      : sizeof(kBuiltinTuning) / sizeof(kBuiltinTuning[0]);
+  const std::vector<TuneRow>* loaded = ctx ? (tree ? &ctx->tuning_tree : &ctx->tuning) : nullptr;
+  if (loaded && !loaded->empty()) {
+    rows = loaded->data();
+    nrows = loaded->size();
   }
   // the row of the largest size class not above log2(n), else the smallest
   const TuneRow* pick = &rows[0];

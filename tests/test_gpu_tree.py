@@ -180,3 +180,43 @@ def test_tree_cycle_beside_trees(ctx, build, cycle):
         with pytest.raises(kascade_amd.KascadeError) as e:
             ctx.root_tree(dev(out), tour_build=build, ruler_spacing=spacing)
         assert e.value.status == kascade_amd.KC_ECYCLE, (build, cycle, spacing)
+
+
+def test_tree_tuning_table(ctx):
+    """Tree rooting has its own tuning table (its classes count vertices): it
+    changes the knobs of tree rooting only, never a result."""
+    say("tree tuning")
+    list_before = ctx.tuning_get(1 << 14)
+    try:
+        ctx.tuning_set([(10, 4, 4, 1, 64), (16, 256, 8, 2, 1024)], kind="tree")
+        assert ctx.tuning_get(1 << 12, kind="tree")["spacing_top"] == 4
+        assert ctx.tuning_get(1 << 17, kind="tree")["spacing_top"] == 256
+        assert ctx.tuning_get(1 << 14) == list_before
+        for n in [6000, 100003]:
+            parent = o.gen_tree(n, 31 + n)
+            root, depth = ctx.root_tree(dev(parent))
+            er, ed = o.tree_closed_form(n, 31 + n)
+            assert (host(root) == er).all() and (host(depth) == ed).all()
+    finally:
+        ctx.tuning_set([], kind="tree")
+    assert ctx.tuning_get(1 << 26, kind="tree")["table_rows"] == 0
+
+
+def test_tuning_file_kinds(tmp_path, monkeypatch):
+    """A table file holds list rows and "tree" rows; malformed rows are skipped."""
+    import kascade_amd
+    say("tuning file kinds")
+    f = tmp_path / "tuning.txt"
+    f.write_text("# comment\n10 8 8 2 64\n20 64 8 2 64   # 1.0\n"
+                 "tree 10 16 8 1 64\ntree 20 32 16 4 256\nnot a row\ntree 12 3 8 1 64\n")
+    monkeypatch.setenv("KASCADE_AMD_TUNING", str(f))
+    c = kascade_amd.Context(0)
+    try:
+        lst = c.tuning_get(1 << 21)
+        assert (lst["spacing_top"], lst["table_rows"]) == (64, 2)
+        t = c.tuning_get(1 << 21, kind="tree")
+        assert (t["spacing_top"], t["spacing_deep"], t["chase_blocks_per_cu"], t["chase_chunk"],
+                t["table_rows"]) == (32, 16, 4, 256, 2)
+        assert c.tuning_get(1 << 11, kind="tree")["spacing_top"] == 16
+    finally:
+        c.close()
