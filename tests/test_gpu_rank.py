@@ -69,6 +69,27 @@ def test_rank_random_list(ctx, algo, n):
     assert (host(rank) == ek).all()
 
 
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("kind", ["signed", "zero", "wide"])
+@pytest.mark.parametrize("n,nlists", [(100003, 1), (1 << 20, 1), (200000, 5000)])
+def test_rank_weight_ranges(ctx, algo, kind, n, nlists):
+    """Weights are any int64 (rank_t): signed, all zero, and wide (|w| up to
+    2^40, |rank| up to about 2^60); tails carry weight 0."""
+    say("weights", algo, kind, n, nlists)
+    rng = np.random.default_rng(n + nlists)
+    succ, _ = random_forest_lists(n, nlists, n + 7, weighted=False)
+    if kind == "signed":
+        w = rng.integers(-1000, 1001, size=n, dtype=np.int64)
+    elif kind == "zero":
+        w = np.zeros(n, np.int64)
+    else:
+        w = rng.integers(-(1 << 40), (1 << 40) + 1, size=n, dtype=np.int64)
+    w[succ == np.arange(n, dtype=np.uint64)] = 0
+    root, rank = ctx.rank_list(dev(succ), dev(w), algorithm=algo)
+    er, ek = o.sequential_walk(succ, w)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
 @pytest.mark.parametrize("spacing,deep", [(2, 2), (4, 16), (64, 4), (256, 32), (4096, 8)])
 def test_rank_spacings(ctx, spacing, deep):
     n = 200003
