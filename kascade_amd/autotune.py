@@ -14,9 +14,9 @@ records), and picks, by coordinate descent over the knobs
 the configuration with the lowest median time over `reps` calls.  Every
 candidate's output must equal the first configuration's output bit for bit
 (outputs are unique, so any difference is a bug and aborts the sweep).  The
-table is written as text (tree rows prefixed with "tree"; the rows of the
-other kind already in the file are kept); contexts created afterwards load it
-from next to the library.
+table is written as text (tree rows prefixed with "tree"; rows already in the
+file for the other kind or for classes not tuned in this run are kept);
+contexts created afterwards load it from next to the library.
 """
 import argparse
 import os
@@ -26,8 +26,8 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 KNOBS = {
-    "spacing_top": [8, 16, 32, 64, 128, 256, 512],
-    "spacing_deep": [8, 16, 32],
+    "spacing_top": [4, 8, 16, 32, 64, 128, 256, 512],
+    "spacing_deep": [2, 4, 8, 16, 32],
     "chase_blocks_per_cu": [1, 2, 4, 8],
     "chase_chunk": [64, 256, 1024],
 }
@@ -86,19 +86,22 @@ def tune_class(ctx, c, reps, kind="list", seed=12345):
     return best, best_t
 
 
-def kept_rows(path, kind):
-    """Data lines of the other kind already in the table file."""
-    keep = []
+def kept_rows(path, kind, classes):
+    """Data lines already in the table file that this run does not replace:
+    (list rows, tree rows), each as (class, line)."""
+    keep = ([], [])
     if not os.path.exists(path):
         return keep
     with open(path) as f:
         for line in f:
-            data = line.split("#", 1)[0].strip()
-            if not data:
+            words = line.split("#", 1)[0].split()
+            if not words:
                 continue
-            is_tree = data.startswith("tree")
-            if is_tree != (kind == "tree"):
-                keep.append(line.rstrip("\n"))
+            is_tree = words[0] == "tree"
+            c = int(words[1] if is_tree else words[0])
+            if is_tree == (kind == "tree") and c in classes:
+                continue
+            keep[1 if is_tree else 0].append((c, line.rstrip("\n")))
     return keep
 
 
@@ -122,15 +125,15 @@ def main():
         rows.append((c, best, t))
         n = max(1 << c, MIN_TUNED_N[args.kind])
         print(f"{args.kind} 2^{c}: {best} {t * 1e3:.3f} ms ({n / t:.3e} {unit}/s)", flush=True)
-    keep = kept_rows(args.out, args.kind)
+    lists, trees = kept_rows(args.out, args.kind, {c for c, _, _ in rows})
     prefix = "tree " if args.kind == "tree" else ""
-    mine = [f"{prefix}{c} {b['spacing_top']} {b['spacing_deep']} {b['chase_blocks_per_cu']} "
-            f"{b['chase_chunk']}   # {t * 1e3:.3f}" for c, b, t in rows]
-    lists, trees = (mine, keep) if args.kind == "list" else (keep, mine)
+    mine = [(c, f"{prefix}{c} {b['spacing_top']} {b['spacing_deep']} {b['chase_blocks_per_cu']} "
+                f"{b['chase_chunk']}   # {t * 1e3:.3f}") for c, b, t in rows]
+    (trees if args.kind == "tree" else lists).extend(mine)
     with open(args.out, "w") as f:
         f.write("# kascade_amd tuning table (written by kascade_amd/autotune.py on the GPU)\n")
         f.write("# [tree] log2n spacing_top spacing_deep chase_blocks_per_cu chase_chunk   # median ms\n")
-        for line in lists + trees:
+        for _, line in sorted(lists) + sorted(trees):
             f.write(line + "\n")
     print(f"wrote {args.out}", flush=True)
 

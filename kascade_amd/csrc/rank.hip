@@ -979,17 +979,17 @@ void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t
 // Built-in table (the autotuner's output for MI355X at the time of the last
 // sweep; a tuning_table.txt next to the library overrides it).
 const TuneRow kBuiltinTuning[] = {
-    {10, 8, 32, 4, 64}, {12, 8, 8, 2, 64}, {14, 8, 32, 4, 64},
-    {16, 16, 32, 8, 64}, {18, 8, 8, 8, 64}, {20, 16, 16, 4, 64},
+    {10, 4, 8, 4, 64}, {12, 4, 2, 8, 64}, {14, 4, 32, 2, 64},
+    {16, 16, 16, 8, 64}, {18, 4, 4, 4, 64}, {20, 16, 4, 1, 64},
     {22, 16, 8, 1, 64}, {24, 32, 8, 2, 64}, {26, 64, 16, 2, 64},
     {27, 128, 8, 2, 256},
 };
 
 // Tree rooting, by vertices (the tour has twice as many records).
 const TuneRow kBuiltinTreeTuning[] = {
-    {10, 8, 32, 2, 64}, {12, 8, 8, 4, 64},  {14, 8, 8, 4, 64},
-    {16, 8, 8, 8, 64},  {18, 8, 16, 8, 64}, {20, 8, 8, 1, 64},
-    {22, 8, 8, 2, 256}, {24, 32, 16, 2, 64}, {26, 128, 16, 2, 64},
+    {10, 4, 16, 8, 64}, {12, 4, 32, 4, 64}, {14, 8, 32, 4, 64},
+    {16, 4, 8, 1, 64}, {18, 8, 4, 1, 64}, {20, 16, 4, 1, 64},
+    {22, 32, 8, 2, 64}, {24, 32, 8, 4, 64}, {26, 128, 16, 2, 64},
 };
 
 RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bool tree) {
