@@ -171,3 +171,45 @@ def test_scratch_reuse_across_contexts(ctx):
     # within one context the pool is steady after its first pass over the jobs
     assert held[1] == held[2] and held[4] == held[5] and held[7] == held[8], held
     kascade_amd.release_cached_scratch(0)
+
+
+def test_server_concurrent_submitters(ctx):
+    """Four host threads submit and wait at once (the server is shared, as a
+    serving front end would share it); every result matches the oracle."""
+    import threading
+    import torch
+    import kascade_amd
+    say("server concurrent submitters")
+    jobs = make_jobs(80, 21)
+    # device arrays are made here: the submitting threads only submit and wait
+    arrays = []
+    for kind, succ, w in jobs:
+        arrays.append((dev(succ), dev(w) if w is not None else None,
+                       torch.empty(len(succ), dtype=torch.uint64, device="cuda"),
+                       torch.empty(len(succ), dtype=torch.int64, device="cuda")))
+    torch.cuda.synchronize()
+    sv = kascade_amd.Server(ctx, num_slots=3)
+    errors = []
+
+    def submitter(part):
+        try:
+            mine = []
+            for k in range(part, len(jobs), 4):
+                s, ww, ro, ra = arrays[k]
+                mine.append(sv.submit(jobs[k][0], s, ww, ro, ra)[0])
+            for t in mine:
+                sv.wait(t)
+        except Exception as e:  # reported on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=submitter, args=(p,)) for p in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not errors and not any(th.is_alive() for th in threads)
+    assert sv.stats()["jobs_completed"] == len(jobs)
+    for (kind, succ, w), (_, _, ro, ra) in zip(jobs, arrays):
+        cr, ck = o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ)
+        assert (host(ro) == cr).all() and (host(ra) == ck).all()
+    sv.close()
