@@ -352,7 +352,42 @@ def segmented_main(args, ctx):
             er, ek = o.sequential_walk(es, ew)
             ok &= bool((root[a:b].cpu().numpy() == er).all() and (rank[a:b].cpu().numpy() == ek).all())
         out["bit_exact_sampled_segments"] = ok
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = segmented_cpu_baseline(off, seeds, local, args.cpu_sample)
     print(json.dumps(out), flush=True)
+
+
+def segmented_cpu_baseline(off, seeds, local, max_elements):
+    """The oracle's restatement of the reference's sparse ruling set
+    (sparse_ruling_set.cpp:335-393, submission configuration) on emulated MPI
+    ranks, ranking the batch's first segments (at most max_elements elements
+    in all) as one forest with global ids, as the reference would rank the
+    whole batch; compared with the sequential walk."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as o
+    k = 0
+    while k < len(seeds) and int(off[k + 1]) <= max_elements:
+        k += 1
+    k = max(k, 1)
+    total = int(off[k])
+    succ = np.empty(total, np.uint64)
+    w = np.empty(total, np.int64)
+    for s in range(k):
+        a, b = int(off[s]), int(off[s + 1])
+        ls, lw = o.gen_list(b - a, int(seeds[s]), False, local=bool(local[s]))
+        succ[a:b] = ls + np.uint64(a)
+        w[a:b] = lw
+    ranks = host_cores()
+    t0 = time.perf_counter()
+    r, rk = o.sparse_ruling_set(succ, w, ranks)
+    dt = time.perf_counter() - t0
+    er, ek = o.sequential_walk(succ, w)
+    return {"value": total / dt, "unit": "elements/s", "cores": ranks, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"the batch's first {k} segments ({total} elements) as one forest: sparse ruling set "
+                      f"(ultimate-sanders f=2, spawn, post-invert, local contraction, 2 rounds + PD) on {ranks} "
+                      f"emulated ranks, {dt:.2f} s",
+            "bit_exact_vs_sequential_walk": bool((r == er).all() and (rk == ek).all())}
 
 
 def main():
