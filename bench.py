@@ -298,20 +298,34 @@ def tree_main(args, ctx):
 
 
 def tree_cpu_baseline(n_sample, seed):
-    """The oracle's restatement of the reference's Euler-tour rooting
-    (eulertour.cpp:33-127, 195-278; one address space, one core) on a bounded
-    sample of the same tree generator."""
+    """The oracle's restatement of the reference's rank_via_euler_tour
+    (eulertour.cpp:337-377) on all host cores: tour build and linking on one
+    thread per core, ranking by the sparse ruling set on as many emulated ranks
+    (experiments/eulertour-sparse-ruling-set.suite.yaml:86-92), on a bounded
+    sample of the same tree generator.  Beside it: the one-core restatement
+    (hash-table linking, GatherChase ranking) on a quarter of the sample."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as o
+    ranks = host_cores()
     parent = o.gen_tree(n_sample, seed)
     er, ed = o.tree_closed_form(n_sample, seed)
     t0 = time.perf_counter()
-    r, d = o.euler_tour_rooting(parent)
+    r, d = o.euler_tour_rooting_mt(parent, ranks)
     dt = time.perf_counter() - t0
-    return {"value": n_sample / dt, "unit": "vertices/s", "cores": 1, "kind": "port", "cpu_model": cpu_model(),
-            "sample": f"random rooted tree n={n_sample} of the same generator, Euler-tour rooting "
-                      f"(tour build, linking, ranking by walk, map back) on one core, {dt:.2f} s",
-            "bit_exact_vs_closed_form": bool((r == er).all() and (d == ed).all())}
+    n1 = max(1, n_sample // 4)
+    p1 = o.gen_tree(n1, seed)
+    e1r, e1d = o.tree_closed_form(n1, seed)
+    t0 = time.perf_counter()
+    r1, d1 = o.euler_tour_rooting(p1)
+    dt1 = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "vertices/s", "cores": ranks, "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"random rooted tree n={n_sample} of the same generator, Euler-tour rooting on {ranks} threads "
+                      f"(parallel tour build and linking, sparse ruling set on {ranks} emulated ranks, map back), "
+                      f"{dt:.2f} s",
+            "one_core_vertices_per_s": n1 / dt1,
+            "one_core_sample": f"n={n1}, hash-table linking and sequential walk, {dt1:.2f} s",
+            "bit_exact_vs_closed_form": bool((r == er).all() and (d == ed).all() and (r1 == e1r).all()
+                                             and (d1 == e1d).all())}
 
 
 def segmented_main(args, ctx):

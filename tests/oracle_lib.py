@@ -45,6 +45,7 @@ def lib():
             "orc_pack": (c.c_int, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_unpack": (None, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_euler_tour_rooting": (c.c_int, [u64p, i64p, c.c_uint64]),
+            "orc_euler_tour_rooting_mt": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int]),
             "orc_sparse_ruling_set": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int, c.c_int]),
             "orc_is_list": (c.c_int, [u64p, c.c_uint64]),
             "orc_leaf_info": (c.c_int64, [u64p, c.c_uint64, c.c_void_p]),
@@ -153,6 +154,16 @@ def euler_tour_rooting(parent):
     s = np.array(parent, dtype=np.uint64, copy=True)
     r = np.empty(len(s), np.int64)
     rc = lib().orc_euler_tour_rooting(_p64(s), _p64(r), len(s))
+    assert rc == 0
+    return s, r
+
+
+def euler_tour_rooting_mt(parent, p=4):
+    """rank_via_euler_tour on p threads: parallel tour build, ranking by the
+    sparse ruling set on p emulated ranks."""
+    s = np.array(parent, dtype=np.uint64, copy=True)
+    r = np.empty(len(s), np.int64)
+    rc = lib().orc_euler_tour_rooting_mt(_p64(s), _p64(r), len(s), p)
     assert rc == 0
     return s, r
 

@@ -227,6 +227,40 @@ def test_euler_tour_forest_and_paths():
     assert (er == br).all() and (ed == bd).all()
 
 
+@pytest.mark.parametrize("p", [1, 3, 8])
+@pytest.mark.parametrize("n", [1, 2, 3, 10, 257, 5000, 100003])
+def test_euler_tour_rooting_threads(p, n):
+    """rank_via_euler_tour on p threads (parallel build, ranking by the
+    sparse ruling set on p emulated ranks) gives the BFS roots and depths."""
+    parent = o.gen_tree(n, 9 + n)
+    tr, td = o.tree_closed_form(n, 9 + n)
+    er, ed = o.euler_tour_rooting_mt(parent, p)
+    assert (er == tr).all() and (ed == td).all()
+    if n <= 5000:
+        br, bd = bfs_depth(parent)
+        assert (er == br).all() and (ed == bd).all()
+
+
+@pytest.mark.parametrize("p", [1, 4])
+def test_euler_tour_rooting_threads_shapes(p):
+    """Forest, star, deep path and isolated roots, relabelled."""
+    rng = np.random.default_rng(p)
+    n = 6000
+    parent = np.arange(n, dtype=np.uint64)
+    for v in range(1, 2000):
+        parent[v] = rng.integers(0, v)
+    for v in range(2001, 3000):
+        parent[v] = 2000
+    for v in range(3001, 5000):
+        parent[v] = v - 1
+    sigma = rng.permutation(n)
+    relab = np.empty(n, np.uint64)
+    relab[sigma] = sigma[parent]
+    br, bd = bfs_depth(relab)
+    er, ed = o.euler_tour_rooting_mt(relab, p)
+    assert (er == br).all() and (ed == bd).all()
+
+
 @pytest.mark.parametrize("p", [1, 2, 4, 7])
 @pytest.mark.parametrize("n,nlists", [(1, 1), (10, 1), (100, 3), (5000, 40), (20000, 1), (30000, 3000)])
 def test_sparse_ruling_set_ranks(p, n, nlists):
