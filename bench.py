@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=1 << 24)
     ap.add_argument("--no-check", action="store_true", help="skip the bit-exact check of the last step")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-host-buffers", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) rates of the headline ranking")
     ap.add_argument("--no-unit-list", action="store_true",
                     help="skip the 2^20 unit-weight list section (GPU beside the CPU path on 4 ranks)")
     ap.add_argument("--serve-jobs", type=int, default=300)
@@ -117,6 +119,30 @@ def cpu_baseline(n_sample, seed):
             "pd_elements_per_s": rate(dt_pd),
             "sequential_walk_1_core_elements_per_s": rate(dt_seq),
             "bit_exact_vs_closed_form": ok and ok4 and ok_pd and ok_seq}
+
+
+def host_buffer_rates(ctx, succ, w, cfg, reps=2):
+    """The same ranking with every array in host memory (pinned, then
+    pageable): the library stages inputs and outputs over PCIe, 32 B per
+    element.  Reported beside the headline, never as it."""
+    import torch
+    n = succ.numel()
+    out = {}
+    for kind in ("pinned", "pageable"):
+        pin = kind == "pinned"
+        hs, hw = succ.cpu(), w.cpu()
+        hr = torch.empty(n, dtype=torch.uint64, pin_memory=pin)
+        hk = torch.empty(n, dtype=torch.int64, pin_memory=pin)
+        if pin:
+            hs, hw = hs.pin_memory(), hw.pin_memory()
+        ctx.rank_list(hs, hw, hr, hk, **cfg)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.rank_list(hs, hw, hr, hk, **cfg)
+        dt = (time.perf_counter() - t0) / reps
+        out[kind] = {"ms": dt * 1e3, "elements_per_s": n / dt, "pcie_gbps": 32 * n / dt / 1e9}
+        del hs, hw, hr, hk
+    return out
 
 
 def unit_list_bench(ctx, seed, reps=20):
@@ -483,6 +509,8 @@ def main():
     ctx.rank_list(succ, w, root, rank, **cfg)
     out["ruler_trace"] = ctx.trace_read()
     ctx.trace(False)
+    if not args.no_host_buffers:
+        out["host_buffers"] = host_buffer_rates(ctx, succ, w, cfg)
     del succ, w, root, rank
     torch.cuda.empty_cache()
     # random-gather ceiling: dst[i] = src[idx[i]], idx a random permutation of
