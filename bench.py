@@ -39,10 +39,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["list", "tree", "segmented", "serve"], default="list",
+    ap.add_argument("--workload", choices=["list", "tree", "segmented", "serve", "unit-list"], default="list",
                     help="list: one random weighted list (the headline); tree: random rooted tree "
                          "(Euler tour); segmented: 20,000 lists of log-uniform size in one batch; "
-                         "serve: the mixed serving stream alone")
+                         "serve: the mixed serving stream alone; unit-list: the 2^20 unit-weight list "
+                         "section alone")
     ap.add_argument("--serve-mode", choices=["all", "batched", "serial"], default="all",
                     help="with --workload serve: every mode, or only the coalescing server (batched) "
                          "or only the one-job-per-launch baseline (serial), each at capacity")
@@ -443,6 +444,10 @@ def main():
         return tree_main(args, ctx)
     if args.workload == "segmented":
         return segmented_main(args, ctx)
+    if args.workload == "unit-list":
+        print(json.dumps({"metric": "elements ranked/s @ n=2^20, unit weights", "n_gpus": 1,
+                          "unit_list_2e20": unit_list_bench(ctx, args.seed)}), flush=True)
+        return
     if args.workload == "serve":
         out = {"metric": "serving capacity (jobs/s) and p50/p99 latency", "n_gpus": 1,
                "serve": serve_bench(ctx, args.serve_jobs, args.seed, args.serve_slots, args.serve_mode)}

@@ -430,32 +430,12 @@ constexpr uint32_t kSmallMax = 8192;
 constexpr unsigned kSmallThreads = 1024;
 constexpr uint32_t kSmallFlag = 0x80000000u;
 
-__global__ void __launch_bounds__(kSmallThreads)
-small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint64_t* __restrict__ root,
-                  int64_t* __restrict__ rank, OutMap om, uint32_t* err, const uint64_t* __restrict__ n_dev) {
-  if (*err) return;
-  if (n_dev && *n_dev < n) {  // device-sized: every element is an output
-    n = static_cast<uint32_t>(*n_dev);
-    n_out = n;
-  }
-  __shared__ uint32_t s_succ[kSmallMax];
-  __shared__ int64_t s_rank[kSmallMax];
+// Jacobi doubling over s_succ / s_rank (n <= kSmallMax entries; a tail carries
+// kSmallFlag); every thread of the workgroup calls it.  Afterwards s_succ[i]
+// is i's tail (flagged) and s_rank[i] its distance to it.  More than 16
+// rounds mean a cycle without a tail.
+__device__ void lds_doubling(uint32_t* s_succ, int64_t* s_rank, uint32_t n, uint32_t* err) {
   constexpr int kPer = kSmallMax / kSmallThreads;
-  for (uint32_t i = threadIdx.x; i < n; i += kSmallThreads) {
-    const Rec r = rec[i];
-    uint32_t s = static_cast<uint32_t>(r.s & kIdMask);
-    if ((r.s & (kRootFlag | kAuxFlag)) || r.s >= n) {
-      raise_err(err, kErrRange);
-      s = i;
-    }
-    if (s == i) {
-      if (r.w != 0) raise_err(err, kErrCycle);  // a self-loop carrying weight closes a cycle
-      s |= kSmallFlag;
-    }
-    s_succ[i] = s;
-    s_rank[i] = r.w;
-  }
-  __syncthreads();
   uint32_t ns[kPer];
   int64_t nr[kPer];
   int rounds = 0;
@@ -492,10 +472,83 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
       break;
     }
   }
+}
+
+__global__ void __launch_bounds__(kSmallThreads)
+small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint64_t* __restrict__ root,
+                  int64_t* __restrict__ rank, OutMap om, uint32_t* err, const uint64_t* __restrict__ n_dev) {
+  if (*err) return;
+  if (n_dev && *n_dev < n) {  // device-sized: every element is an output
+    n = static_cast<uint32_t>(*n_dev);
+    n_out = n;
+  }
+  __shared__ uint32_t s_succ[kSmallMax];
+  __shared__ int64_t s_rank[kSmallMax];
+  for (uint32_t i = threadIdx.x; i < n; i += kSmallThreads) {
+    const Rec r = rec[i];
+    uint32_t s = static_cast<uint32_t>(r.s & kIdMask);
+    if ((r.s & (kRootFlag | kAuxFlag)) || r.s >= n) {
+      raise_err(err, kErrRange);
+      s = i;
+    }
+    if (s == i) {
+      if (r.w != 0) raise_err(err, kErrCycle);  // a self-loop carrying weight closes a cycle
+      s |= kSmallFlag;
+    }
+    s_succ[i] = s;
+    s_rank[i] = r.w;
+  }
+  __syncthreads();
+  lds_doubling(s_succ, s_rank, n, err);
   for (uint32_t i = threadIdx.x; i < n; i += kSmallThreads) {
     if (!out_selected(om, i, n_out)) continue;
     root[out_pos(om, i)] = (s_succ[i] & ~kSmallFlag) >> om.shift;
     rank[out_pos(om, i)] = s_rank[i];
+  }
+}
+
+// The unclaimed sub-problem of a level in one workgroup, when it has at most
+// kSmallMax elements: unclaimed_build_kernel's links, the LDS solver and
+// unclaimed_fix_kernel's outputs in one launch instead of three.  ext (nu
+// records of global scratch) is written and read back by this workgroup only.
+__global__ void __launch_bounds__(kSmallThreads)
+unclaimed_small_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
+                       const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T, Rec* ext,
+                       uint64_t n_out, uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
+                       const unsigned long long* __restrict__ nu_dev, uint32_t* err, JobOut jo,
+                       const uint64_t* __restrict__ seg_off, uint64_t nseg) {
+  if (*err) return;  // the level failed (or overflowed): its records are not valid
+  if (nu_dev && *nu_dev < nu) nu = *nu_dev;
+  __shared__ uint32_t s_succ[kSmallMax];
+  __shared__ int64_t s_rank[kSmallMax];
+  const uint32_t m = static_cast<uint32_t>(nu);
+  for (uint32_t k = threadIdx.x; k < m; k += kSmallThreads) {
+    const uint64_t v = u_ids[k];
+    const Rec r0 = u_orig[k];
+    uint32_t link = k;
+    int64_t w = 0;
+    Rec e{0, 0};
+    if (r0.s == v) {
+      e = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
+    } else {
+      const Rec ms = rec[r0.s];
+      if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
+        link = static_cast<uint32_t>(ms.s & kIdMask);
+        w = r0.w;
+      } else if (ms.s & kRootFlag) {
+        const Rec t = T[ms.s & kIdMask];
+        e = Rec{t.s, r0.w + t.w - ms.w};
+      }
+    }
+    ext[k] = e;
+    s_succ[k] = link == k ? (k | kSmallFlag) : link;
+    s_rank[k] = w;
+  }
+  __syncthreads();
+  lds_doubling(s_succ, s_rank, m, err);
+  for (uint32_t k = threadIdx.x; k < m; k += kSmallThreads) {
+    const Rec e = ext[s_succ[k] & ~kSmallFlag];
+    emit(jo, om, n_out, root, rank, u_ids[k], e.s, s_rank[k] + e.w);
   }
 }
 
@@ -801,8 +854,15 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     u_orig = orig2;
   }
   Phase ph(ctx, "unclaimed", st);
-  Rec* urec = scratch.get<Rec>(nu);
   Rec* ext = scratch.get<Rec>(nu);
+  if (nu <= kSmallMax && nu <= tune.small_n && !ctx->trace_on) {
+    // small enough for one workgroup (the per-level trace keeps the
+    // sub-problem as its own ranking)
+    hipLaunchKernelGGL(unclaimed_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, nu, T,
+                       ext, n_out, root, rank, om, nullptr, ctx->d_err, jo, jo.seg_off, jo.nseg);
+    return;
+  }
+  Rec* urec = scratch.get<Rec>(nu);
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err, jo.seg_off, jo.nseg);
@@ -943,10 +1003,15 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   }
   if (ucap == 0) return;  // tours only: nothing is left unreached
   Phase ph(ctx, "unclaimed", st);
-  Rec* urec = scratch.get<Rec>(ucap);
   Rec* ext = scratch.get<Rec>(ucap);
-  Rec* ures = scratch.get<Rec>(ucap);
   const unsigned long long* nu_dev = ctr + kDsUnclaimed;
+  if (ucap <= kSmallMax && ucap <= tune.small_n) {
+    hipLaunchKernelGGL(unclaimed_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, ucap, T,
+                       ext, n_out, root, rank, om, nu_dev, ctx->d_err, jo, jo.seg_off, jo.nseg);
+    return;
+  }
+  Rec* urec = scratch.get<Rec>(ucap);
+  Rec* ures = scratch.get<Rec>(ucap);
   hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
                      u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err, jo.seg_off, jo.nseg);
   ds_rank(ctx, urec, ucap, reinterpret_cast<const uint64_t*>(nu_dev), ucap, &ures[0].s, &ures[0].w, OutMap{2, 0},
