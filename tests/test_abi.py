@@ -47,3 +47,23 @@ def test_cpp_header_compiles():
         r = subprocess.run(["g++", "-std=c++20", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), cpp],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_c_header_is_plain_c():
+    """The C ABI header compiles as C11 with warnings as errors (no C++ or
+    torch types in the signatures), and a C program links against the library
+    with every declared entry point referenced."""
+    import subprocess
+    import tempfile
+    names = declared_functions()
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        refs = "\n".join(f"  (void)&{n};" for n in names)
+        open(c, "w").write('#include "kascade_amd.h"\n#include <stddef.h>\n'
+                           f"int main(void) {{\n{refs}\n  return 0;\n}}\n")
+        exe = os.path.join(d, "t")
+        r = subprocess.run(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-pedantic",
+                            "-I", os.path.join(ROOT, "include"), c, "-o", exe,
+                            "-L", os.path.dirname(kascade_amd.lib_path()), "-lkascade_amd"],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
