@@ -374,9 +374,10 @@ kc_status kc_probe_record_access(void *ctx, uint64_t n, int pattern, uint32_t re
 /* ---- serving ------------------------------------------------------------ */
 typedef enum kc_job_kind { KC_JOB_LIST = 0, KC_JOB_TREE = 1 } kc_job_kind;
 
-/* A job's arrays live in host memory owned by the caller until the ticket
- * completes.  LIST: succ/weight in, root/rank out.  TREE: succ = parent in,
- * root/rank(depth) out, weight ignored. */
+/* A job's arrays (host or device memory) are owned by the caller and must
+ * stay valid until the ticket completes; device arrays are ranked in place,
+ * host arrays are staged.  LIST: succ/weight in, root/rank out.  TREE: succ =
+ * parent in, root/rank(depth) out, weight ignored. */
 typedef struct kc_job {
   int32_t kind; /* kc_job_kind */
   uint32_t reserved;

@@ -576,7 +576,7 @@ def _alloc_pair(n, device, weights=True):
 
 
 class Server:
-    """Continuous-batching request server (kc_server_*); jobs use host arrays."""
+    """Continuous-batching request server (kc_server_*); a job's arrays may be host or device memory."""
 
     def __init__(self, ctx, mem_budget_bytes=200 << 30, max_batch_jobs=0, num_slots=0,
                  max_batch_elements=0, one_job_per_launch=False, tour_build="auto"):
