@@ -28,7 +28,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 KNOBS = {
     "spacing_top": [4, 8, 16, 32, 64, 128, 256, 512],
     "spacing_deep": [2, 4, 8, 16, 32],
-    "chase_blocks_per_cu": [1, 2, 4, 8],
+    "chase_blocks_per_cu": [1, 2, 3, 4, 8],
     "chase_chunk": [64, 256, 1024],
 }
 START = {"spacing_top": 32, "spacing_deep": 16, "chase_blocks_per_cu": 4, "chase_chunk": 256}
