@@ -93,3 +93,30 @@ def test_segmented_host_arrays_and_errors(ctx):
     with pytest.raises(kascade_amd.KascadeError) as e:
         ctx.rank_lists_segmented(bad, off, None)
     assert e.value.status == kascade_amd.KC_EINVAL
+
+
+@pytest.mark.parametrize("algo", ["srs", "pd"])
+def test_segmented_many_tiny_lists(ctx, algo):
+    """200,000 segments of 0 to 20 elements: most lists hold no ruler, so
+    nearly every element goes through the unreached-prefix sub-problem."""
+    say("seg many tiny", algo)
+    rng = np.random.default_rng(23)
+    sizes = rng.integers(0, 21, size=200000).astype(np.uint64)
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    total = int(off[-1])
+    succ = np.empty(total, np.uint32)
+    w = rng.integers(-5, 50, size=total).astype(np.int64)
+    gsucc = np.empty(total, np.uint64)
+    for s in np.flatnonzero(sizes):
+        a, b = int(off[s]), int(off[s + 1])
+        perm = rng.permutation(b - a)
+        loc = np.arange(b - a, dtype=np.uint64)
+        loc[perm[:-1]] = perm[1:]
+        succ[a:b] = loc.astype(np.uint32)
+        gsucc[a:b] = loc + np.uint64(a)
+        w[a + perm[-1]] = 0
+    root, rank = ctx.rank_lists_segmented(dev(succ), dev(off), dev(w), algorithm=algo)
+    er, ek = o.sequential_walk(gsucc, w)
+    base = np.repeat(off[:-1], sizes.astype(np.int64))
+    assert (host(root).astype(np.uint64) == er - base).all() and (host(rank) == ek).all()
