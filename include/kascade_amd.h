@@ -13,9 +13,21 @@
  * Pointers may be device memory (hipMalloc / torch tensors) or host memory;
  * host arrays are staged through device buffers by the library.  All entry
  * points are synchronous with respect to the host: they enqueue on `stream`
- * (NULL = the context's stream) and return when the results are written, like
- * the reference's collective calls.  A context is externally synchronised,
- * like a communicator: one host thread at a time.
+ * and return when the results are written, like the reference's collective
+ * calls.  A context is externally synchronised, like a communicator: one host
+ * thread at a time.
+ *
+ * Stream ordering.  `stream` is a hipStream_t passed as void*:
+ *   - a stream the caller created: the call runs on it, after the work
+ *     already queued there;
+ *   - NULL: the context's stream, which is non-blocking and NOT ordered with
+ *     any other stream, HIP's null stream included: device inputs written on
+ *     other streams must be complete (the caller synchronises that stream)
+ *     before the call.
+ * Because every call returns only when its results are written, work queued
+ * after it on any stream sees them.  Server jobs (kc_submit) run on the
+ * server's own streams: their device arrays must be complete before
+ * kc_submit and must not be touched until the ticket completes.
  */
 #ifndef KASCADE_AMD_H
 #define KASCADE_AMD_H

@@ -196,6 +196,28 @@ def _ptr(a, dtype):
     return a.data_ptr()
 
 
+def _st(stream):
+    """The stream argument of a call.  None becomes torch's current stream
+    when torch has initialised the GPU, so that the library's work is ordered
+    after the torch kernels that produced its inputs (and, the call being
+    synchronous, before the ones queued after it).  torch's default stream is
+    HIP's null stream, which the C ABI does not take (NULL selects the
+    context's stream): its queued work is waited for on the host, and the
+    call runs on the context's stream.  Without torch on the GPU, None selects
+    the context's stream."""
+    if stream is not None:
+        return stream
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is None or not torch.cuda.is_initialized():
+        return None
+    cur = torch.cuda.current_stream()
+    if cur.cuda_stream == 0:
+        cur.synchronize()
+        return None
+    return cur.cuda_stream
+
+
 def _numel(a):
     return a.size if isinstance(a, np.ndarray) else a.numel()
 
@@ -270,27 +292,27 @@ class Context:
         c = _cfg(**cfg)
         self._check(self._lib.kc_rank_list(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64), n,
                                            _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
-                                           ctypes.byref(c), stream))
+                                           ctypes.byref(c), _st(stream)))
         return root_out, rank_out
 
     def pointer_doubling(self, succ_array, rank_array, stream=None, **cfg):
         """In place: succ -> root, rank (weights) -> rank (pointer_doubling.hpp:18-23)."""
         c = _cfg(**cfg)
         self._check(self._lib.kc_pointer_doubling(self.handle, ctypes.byref(c), _ptr(succ_array, np.uint64),
-                                                  _ptr(rank_array, np.int64), _numel(succ_array), stream))
+                                                  _ptr(rank_array, np.int64), _numel(succ_array), _st(stream)))
 
     def sparse_ruling_set(self, succ_array, rank_array, stream=None, **cfg):
         """In place, as kascade::sparse_ruling_set (sparse_ruling_set.hpp:13-18)."""
         c = _cfg(**cfg)
         self._check(self._lib.kc_sparse_ruling_set(self.handle, ctypes.byref(c), _ptr(succ_array, np.uint64),
-                                                   _ptr(rank_array, np.int64), _numel(succ_array), stream))
+                                                   _ptr(rank_array, np.int64), _numel(succ_array), _st(stream)))
 
     def set_initial_ranking_state(self, succ, root_out=None, rank_out=None, stream=None):
         root_out = _like(succ, np.uint64) if root_out is None else root_out
         rank_out = _like(succ, np.int64) if rank_out is None else rank_out
         self._check(self._lib.kc_set_initial_ranking_state(self.handle, _ptr(succ, np.uint64), _numel(succ),
                                                            _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
-                                                           stream))
+                                                           _st(stream)))
         return root_out, rank_out
 
     # ---- per-level trace (RulerTrace) --------------------------------------------
@@ -312,7 +334,7 @@ class Context:
     def is_list(self, succ, stream=None):
         """kascade::is_list: no element has two predecessors."""
         out = ctypes.c_int32()
-        self._check(self._lib.kc_is_list(self.handle, _ptr(succ, np.uint64), _numel(succ), ctypes.byref(out), stream))
+        self._check(self._lib.kc_is_list(self.handle, _ptr(succ, np.uint64), _numel(succ), ctypes.byref(out), _st(stream)))
         return bool(out.value)
 
     def leaf_info(self, succ, stream=None):
@@ -320,7 +342,7 @@ class Context:
         flags = _like(succ, np.uint8)
         cnt = ctypes.c_uint64()
         self._check(self._lib.kc_leaf_info(self.handle, _ptr(succ, np.uint64), _numel(succ), _ptr(flags, np.uint8),
-                                           ctypes.byref(cnt), stream))
+                                           ctypes.byref(cnt), _st(stream)))
         return flags, cnt.value
 
     def reverse_list(self, succ, dist_to_succ=None, stream=None):
@@ -332,7 +354,7 @@ class Context:
         nr = ctypes.c_uint64()
         self._check(self._lib.kc_reverse_list(self.handle, _ptr(succ, np.uint64), _ptr(dist_to_succ, np.int64), n,
                                               _ptr(pred, np.uint64), _ptr(dpred, np.int64), _ptr(roots, np.uint64),
-                                              n, ctypes.byref(nr), stream))
+                                              n, ctypes.byref(nr), _st(stream)))
         return pred, dpred, roots[:nr.value]
 
     def compute_stats(self, root, rank, level="basic", stream=None):
@@ -348,7 +370,7 @@ class Context:
             tm, tsum = np.empty(cap, np.int64), np.empty(cap, np.int64)
         self._check(self._lib.kc_compute_stats(self.handle, lv, _ptr(root, np.uint64), _ptr(rank, np.int64), n,
                                                ctypes.byref(st), _ptr(tr, np.uint64), _ptr(ts, np.uint64),
-                                               _ptr(tm, np.int64), _ptr(tsum, np.int64), cap, stream))
+                                               _ptr(tm, np.int64), _ptr(tsum, np.int64), cap, _st(stream)))
         out = {}
         if lv >= 2:
             out["basic_stats"] = {"num_trees": st.num_trees, "max_rank": st.max_rank, "avg_rank": st.avg_rank}
@@ -372,18 +394,18 @@ class Context:
         src = torch.empty(m, dtype=torch.uint64, device="cuda")
         dst = torch.empty(m, dtype=torch.uint64, device="cuda")
         self._check(self._lib.kc_gen_graph(self.handle, self.GRAPH_KINDS[kind], n, m, seed, _ptr(src, np.uint64),
-                                           _ptr(dst, np.uint64), stream))
+                                           _ptr(dst, np.uint64), _st(stream)))
         return src, dst
 
     def gen_rgg2d(self, n, m, seed, stream=None):
         """Random geometric graph with about m edges on n points (device tensors)."""
         import torch
         cnt = ctypes.c_uint64()
-        self._check(self._lib.kc_gen_rgg2d(self.handle, n, m, seed, None, None, 0, ctypes.byref(cnt), stream))
+        self._check(self._lib.kc_gen_rgg2d(self.handle, n, m, seed, None, None, 0, ctypes.byref(cnt), _st(stream)))
         src = torch.empty(max(cnt.value, 1), dtype=torch.uint64, device="cuda")
         dst = torch.empty(max(cnt.value, 1), dtype=torch.uint64, device="cuda")
         self._check(self._lib.kc_gen_rgg2d(self.handle, n, m, seed, _ptr(src, np.uint64), _ptr(dst, np.uint64),
-                                           cnt.value, ctypes.byref(cnt), stream))
+                                           cnt.value, ctypes.byref(cnt), _st(stream)))
         return src[:cnt.value], dst[:cnt.value]
 
     def build_csr(self, src, dst, n, stream=None):
@@ -391,7 +413,7 @@ class Context:
         xadj = _like(src, np.uint64, n + 1)
         adj = _like(src, np.uint64, 2 * m)
         self._check(self._lib.kc_build_csr(self.handle, _ptr(src, np.uint64), _ptr(dst, np.uint64), n, m,
-                                           _ptr(xadj, np.uint64), _ptr(adj, np.uint64), stream))
+                                           _ptr(xadj, np.uint64), _ptr(adj, np.uint64), _st(stream)))
         return xadj, adj
 
     def bfs_forest(self, xadj, adj, stream=None):
@@ -399,7 +421,7 @@ class Context:
         n = _numel(xadj) - 1
         parent = _like(xadj, np.uint64, n)
         self._check(self._lib.kc_bfs_forest(self.handle, _ptr(xadj, np.uint64), _ptr(adj, np.uint64), n,
-                                            _ptr(parent, np.uint64), stream))
+                                            _ptr(parent, np.uint64), _st(stream)))
         return parent
 
     def pd_rounds(self, succ, weight, rounds, stream=None):
@@ -407,7 +429,7 @@ class Context:
         ro = _like(succ, np.int64)
         self._check(self._lib.kc_pointer_doubling_rounds(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64),
                                                          _numel(succ), rounds, _ptr(so, np.uint64),
-                                                         _ptr(ro, np.int64), stream))
+                                                         _ptr(ro, np.int64), _st(stream)))
         return so, ro
 
     def pack(self, succ, rank, active, stream=None):
@@ -416,13 +438,13 @@ class Context:
         pr = _like(active, np.int64, na)
         self._check(self._lib.kc_pack(self.handle, _ptr(succ, np.uint64), _ptr(rank, np.int64), _numel(succ),
                                       _ptr(active, np.uint64), na, _ptr(ps, np.uint64), _ptr(pr, np.int64),
-                                      stream))
+                                      _st(stream)))
         return ps, pr
 
     def unpack(self, psucc, prank, active, succ, rank, stream=None):
         self._check(self._lib.kc_unpack(self.handle, _ptr(psucc, np.uint64), _ptr(prank, np.int64),
                                         _ptr(active, np.uint64), _numel(active), _ptr(succ, np.uint64),
-                                        _ptr(rank, np.int64), _numel(succ), stream))
+                                        _ptr(rank, np.int64), _numel(succ), _st(stream)))
 
     def rank_lists_segmented(self, succ_local, seg_off, weight=None, root_out=None, rank_out=None,
                              stream=None, **cfg):
@@ -433,7 +455,7 @@ class Context:
         self._check(self._lib.kc_rank_lists_segmented(self.handle, _ptr(succ_local, np.uint32),
                                                        _ptr(seg_off, np.uint64), _numel(seg_off) - 1,
                                                        _ptr(weight, np.int64), _ptr(root_out, np.uint32),
-                                                       _ptr(rank_out, np.int64), ctypes.byref(c), stream))
+                                                       _ptr(rank_out, np.int64), ctypes.byref(c), _st(stream)))
         return root_out, rank_out
 
     # ---- trees --------------------------------------------------------------
@@ -445,7 +467,7 @@ class Context:
         t.tour_build = TOUR_BUILDS[tour_build]
         self._check(self._lib.kc_root_tree(self.handle, _ptr(parent, np.uint64), _numel(parent),
                                            _ptr(root_out, np.uint64), _ptr(depth_out, np.int64),
-                                           ctypes.byref(t), stream))
+                                           ctypes.byref(t), _st(stream)))
         return root_out, depth_out
 
     def rank_via_euler_tour(self, succ_array, rank_array, stream=None, tour_build="auto", **cfg):
@@ -454,26 +476,26 @@ class Context:
         t.rank = _cfg(**cfg)
         t.tour_build = TOUR_BUILDS[tour_build]
         self._check(self._lib.kc_rank_via_euler_tour(self.handle, ctypes.byref(t), _ptr(succ_array, np.uint64),
-                                                     _ptr(rank_array, np.int64), _numel(succ_array), stream))
+                                                     _ptr(rank_array, np.int64), _numel(succ_array), _st(stream)))
 
     def compute_euler_tour(self, parent, stream=None):
         n = _numel(parent)
         s = _like(parent, np.uint64, 2 * n)
         w = _like(parent, np.int64, 2 * n)
         self._check(self._lib.kc_compute_euler_tour(self.handle, _ptr(parent, np.uint64), n,
-                                                    _ptr(s, np.uint64), _ptr(w, np.int64), stream))
+                                                    _ptr(s, np.uint64), _ptr(w, np.int64), _st(stream)))
         return s, w
 
     # ---- synthetic inputs --------------------------------------------------------
     def gen_list(self, n, seed, weighted=True, local=False, device="cuda", stream=None):
         succ, w = _alloc_pair(n, device)
         self._check(self._lib.kc_gen_list(self.handle, n, seed, int(weighted), int(local),
-                                          _ptr(succ, np.uint64), _ptr(w, np.int64), stream))
+                                          _ptr(succ, np.uint64), _ptr(w, np.int64), _st(stream)))
         return succ, w
 
     def gen_tree(self, n, seed, device="cuda", stream=None):
         parent, _ = _alloc_pair(n, device, weights=False)
-        self._check(self._lib.kc_gen_tree(self.handle, n, seed, _ptr(parent, np.uint64), stream))
+        self._check(self._lib.kc_gen_tree(self.handle, n, seed, _ptr(parent, np.uint64), _st(stream)))
         return parent
 
     def gen_lists_segmented(self, seg_off, seeds, local_flags=None, weighted=True, device="cuda",
@@ -490,7 +512,7 @@ class Context:
                                                      _ptr(seeds, np.uint64),
                                                      _ptr(local_flags, np.uint8) if local_flags is not None else None,
                                                      int(weighted), _ptr(succ, np.uint32), _ptr(w, np.int64),
-                                                     stream))
+                                                     _st(stream)))
         return succ, w
 
     # ---- phase timing -----------------------------------------------------------
@@ -598,6 +620,11 @@ class Server:
         n = _numel(succ)
         root_out = np.empty(n, np.uint64) if root_out is None else root_out
         rank_out = np.empty(n, np.int64) if rank_out is None else rank_out
+        # the server's slots run on streams of their own: device arrays must
+        # be complete before the job is handed over
+        if any(getattr(a, "is_cuda", False) for a in (succ, weight, root_out, rank_out)):
+            import torch
+            torch.cuda.current_stream().synchronize()
         j = Job()
         j.kind = JOB_KINDS[kind] if isinstance(kind, str) else int(kind)
         j.n = n
@@ -616,9 +643,9 @@ class Server:
         st = self._lib.kc_poll(self.handle, ticket)
         if st == KC_EAGAIN:
             return False
+        self._keep.pop(ticket, None)
         if st != KC_OK:
             raise KascadeError(st, f"job {ticket} failed")
-        self._keep.pop(ticket, None)
         return True
 
     def wait(self, ticket):

@@ -56,7 +56,8 @@ struct Server {
   std::condition_variable cv_work;
   std::condition_variable cv_done;
   std::deque<Ticket*> queue;
-  std::unordered_map<uint64_t, std::unique_ptr<Ticket>> tickets;
+  // shared: a waiter keeps its ticket alive while another thread erases it
+  std::unordered_map<uint64_t, std::shared_ptr<Ticket>> tickets;
   uint64_t next_id = 1;
   uint64_t bytes_in_use = 0;
   uint64_t peak_bytes = 0;
@@ -262,7 +263,7 @@ kc_status kc_submit(void* vsv, const kc_job* job, uint64_t* ticket_out) {
   if (!sv || !job || !ticket_out) return KC_EINVAL;
   if (job->kind != KC_JOB_LIST && job->kind != KC_JOB_TREE) return KC_EINVAL;
   if (job->n && (!job->succ || !job->root_out || !job->rank_out)) return KC_EINVAL;
-  auto t = std::make_unique<Ticket>();
+  auto t = std::make_shared<Ticket>();
   t->job = *job;
   t->bytes = job_bytes(*job);
   if (t->bytes > sv->cfg.mem_budget_bytes) return KC_EBUDGET;
@@ -301,11 +302,11 @@ kc_status kc_wait(void* vsv, uint64_t ticket) {
   std::unique_lock<std::mutex> lk(sv->mu);
   auto it = sv->tickets.find(ticket);
   if (it == sv->tickets.end()) return KC_ENOTFOUND;
-  Ticket* t = it->second.get();
+  std::shared_ptr<Ticket> t = it->second;
   sv->cv_done.wait(lk, [&] { return t->state == 2; });
-  kc_status s = t->status;
+  // another waiter or kc_poll may have collected the ticket meanwhile
   sv->tickets.erase(ticket);
-  return s;
+  return t->status;
 }
 
 kc_status kc_server_stats_get(void* vsv, kc_server_stats* out) {
