@@ -184,6 +184,30 @@ kc_status kc_rank_list(void *ctx, const uint64_t *succ, const int64_t *weight, u
                        uint64_t *root_out, int64_t *rank_out, const kc_rank_cfg *cfg,
                        void *stream);
 
+/* kc_rank_list without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch is a static device array of
+ * KC_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends; the call itself calls no hipMalloc or hipFree and
+ * captures no HIP graph (cfg->launch AUTO runs as KC_LAUNCH_DEVICE).
+ *   - n must be at most KC_NOALLOC_MAX_N (else KC_EINVAL); with the built-in
+ *     tuning every list or forest of that size fits.  A caller-fixed ruler
+ *     spacing or an adversarial input that needs more scratch gets
+ *     KC_ENOMEM before any kernel writes past the array.
+ *   - succ, weight (may be NULL), root_out and rank_out must be device
+ *     memory (else KC_EINVAL): host arrays would need staging buffers.
+ *   - Every call on one GPU shares that GPU's array, whatever the context:
+ *     calls that overlap on one GPU (two host threads, or two streams of
+ *     one caller) race and must be serialised by the caller.  The array
+ *     keeps the last call's contents; every call clears what it needs.
+ * Results are identical to kc_rank_list's. */
+#define KC_NOALLOC_MAX_N ((uint64_t)1 << 20)
+#define KC_NOALLOC_SCRATCH_BYTES ((size_t)192 << 20)
+kc_status kc_rank_list_noalloc(void *ctx, const uint64_t *succ, const int64_t *weight, uint64_t n,
+                               uint64_t *root_out, int64_t *rank_out, const kc_rank_cfg *cfg,
+                               void *stream);
+
 /* In-place forms with the reference signatures: succ_array in = successors,
  * out = roots; rank_array in = distance to successor (0 on roots), out =
  * distance to root.  kascade::pointer_doubling (pointer_doubling.hpp:18-23). */

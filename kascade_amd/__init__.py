@@ -112,6 +112,7 @@ _SIGS = {
     "kc_tuning_get_kind": (ctypes.c_uint32, [_P, ctypes.c_int32, _U64, ctypes.POINTER(ctypes.c_uint32)]),
     "kc_set_initial_ranking_state": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
     "kc_rank_list": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, ctypes.POINTER(RankCfg), _P]),
+    "kc_rank_list_noalloc": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, ctypes.POINTER(RankCfg), _P]),
     "kc_pointer_doubling": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
     "kc_sparse_ruling_set": (ctypes.c_int, [_P, ctypes.POINTER(RankCfg), _P, _P, _U64, _P]),
     "kc_pointer_doubling_rounds": (ctypes.c_int, [_P, _P, _P, _U64, ctypes.c_uint32, _P, _P, _P]),
@@ -293,6 +294,18 @@ class Context:
         self._check(self._lib.kc_rank_list(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64), n,
                                            _ptr(root_out, np.uint64), _ptr(rank_out, np.int64),
                                            ctypes.byref(c), _st(stream)))
+        return root_out, rank_out
+
+    NOALLOC_MAX_N = 1 << 20  # KC_NOALLOC_MAX_N
+
+    def rank_list_noalloc(self, succ, weight, root_out, rank_out, stream=None, **cfg):
+        """kc_rank_list_noalloc: rank_list with no device allocation (scratch
+        from the library's static array); device tensors only, caller-provided
+        outputs, n at most NOALLOC_MAX_N."""
+        c = _cfg(**cfg)
+        self._check(self._lib.kc_rank_list_noalloc(self.handle, _ptr(succ, np.uint64), _ptr(weight, np.int64),
+                                                   _numel(succ), _ptr(root_out, np.uint64),
+                                                   _ptr(rank_out, np.int64), ctypes.byref(c), _st(stream)))
         return root_out, rank_out
 
     def pointer_doubling(self, succ_array, rank_array, stream=None, **cfg):

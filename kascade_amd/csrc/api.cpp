@@ -52,7 +52,16 @@ size_t round_chunk(size_t b) { return (b + kMinChunk - 1) / kMinChunk * kMinChun
 
 Arena::~Arena() { trim(); }
 
+void Arena::use_fixed(char* base, size_t size) {
+  chunks_.assign(1, Chunk{base, size});
+  blocks_.clear();
+  blocks_.reserve(1024);  // a ranking's scratch stack stays far below this
+  top_ = live_ = 0;
+  fixed_ = true;
+}
+
 bool Arena::new_chunk(size_t want) {
+  if (fixed_) return false;
   want = std::max(want, kMinChunk);
   char* p = nullptr;
   size_t size = 0;
@@ -119,7 +128,7 @@ void Arena::release(size_t mark) {
 }
 
 void Arena::recycle() {
-  if (top_ != 0) return;  // never hand out live blocks
+  if (top_ != 0 || fixed_) return;  // never hand out live blocks or memory it does not own
   ChunkCache& c = chunk_cache(device);
   {
     std::lock_guard<std::mutex> lk(c.mu);
@@ -140,7 +149,7 @@ void Arena::recycle() {
 }
 
 void Arena::trim() {
-  if (top_ != 0) return;  // never free live blocks
+  if (top_ != 0 || fixed_) return;  // never free live blocks or memory it does not own
   for (auto& c : chunks_) (void)hipFree(c.base);
   chunks_.clear();
   blocks_.clear();
@@ -414,6 +423,10 @@ kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void** ctx_ou
     return KC_EDEVICE;
   }
   ctx->d_err = reinterpret_cast<uint32_t*>(ctx->d_counters + kc::kErrSlot);
+  char* nb = nullptr;
+  size_t nbytes = 0;
+  if (kc::noalloc_scratch_init(device) && kc::noalloc_scratch(device, &nb, &nbytes))
+    ctx->noalloc_arena.use_fixed(nb, nbytes);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
   kc::load_default_tuning(ctx);
@@ -489,6 +502,45 @@ kc_status kc_rank_list(void* vctx, const uint64_t* succ, const int64_t* weight, 
     kc::rank_list_device(ctx, s.dev(), w.dev(), n, ro.dev(), ra.dev(), cfg, st);
     ro.finish();
     ra.finish();
+    KC_HIP(hipStreamSynchronize(st));
+  });
+}
+
+kc_status kc_rank_list_noalloc(void* vctx, const uint64_t* succ, const int64_t* weight, uint64_t n,
+                               uint64_t* root_out, int64_t* rank_out, const kc_rank_cfg* cfg,
+                               void* stream) {
+  return guarded(vctx, [&](Context* ctx) {
+    if (n > KC_NOALLOC_MAX_N)
+      throw KcError{KC_EINVAL, "kc_rank_list_noalloc ranks at most 2^20 elements; use kc_rank_list"};
+    if (n && (!succ || !root_out || !rank_out)) throw KcError{KC_EINVAL, "null array"};
+    if (!kc::is_device_ptr(succ) || !kc::is_device_ptr(weight) || !kc::is_device_ptr(root_out) ||
+        !kc::is_device_ptr(rank_out))
+      throw KcError{KC_EINVAL, "kc_rank_list_noalloc needs device arrays (host arrays would be staged)"};
+    if (!ctx->noalloc_arena.fixed())
+      throw KcError{KC_EDEVICE, "the static scratch array's device address is not available"};
+    hipStream_t st = pick(ctx, stream);
+    kc_rank_cfg c{};
+    if (cfg) c = *cfg;
+    if (c.launch == KC_LAUNCH_AUTO) c.launch = KC_LAUNCH_DEVICE;  // a graph capture may allocate
+    // every allocation of the ranking comes from the static array; phase
+    // timing (which creates events) is off for the call
+    struct Restore {
+      Context* ctx;
+      bool prof;
+      ~Restore() {
+        ctx->active_arena = &ctx->arena;
+        ctx->prof.on = prof;
+      }
+    } restore{ctx, ctx->prof.on};
+    ctx->noalloc_arena.release(0);
+    ctx->active_arena = &ctx->noalloc_arena;
+    ctx->prof.on = false;
+    try {
+      kc::rank_list_device(ctx, succ, weight, n, root_out, rank_out, &c, st);
+    } catch (const std::bad_alloc&) {
+      (void)hipStreamSynchronize(st);  // kernels queued before the failed allocation still use the array
+      throw KcError{KC_ENOMEM, "input needs more scratch than the static array of kc_rank_list_noalloc holds"};
+    }
     KC_HIP(hipStreamSynchronize(st));
   });
 }

@@ -55,6 +55,11 @@ class Arena {
   // Appends what decides the addresses of the next allocations (chunks and
   // the stack position), so equal signatures hand out equal addresses.
   void signature(std::vector<uint64_t>& out) const;
+  // Serves every allocation from [base, base + size), memory the arena does
+  // not own: it never allocates, frees or caches a chunk, and an allocation
+  // that does not fit fails (alloc returns nullptr).
+  void use_fixed(char* base, size_t size);
+  bool fixed() const { return fixed_; }
 
  private:
   struct Chunk {
@@ -73,10 +78,17 @@ class Arena {
   size_t top_ = 0;   // number of live blocks
   size_t live_ = 0;  // bytes in live blocks
   size_t peak_ = 0;  // most bytes live at once, over the arena's life
+  bool fixed_ = false;
   std::atomic<uint64_t> held_{0};  // readable from other threads (server statistics)
 };
 // Frees the chunks cached for `device` (kc_release_cached_scratch).
 void release_chunk_cache(int device);
+// The static scratch array of kc_rank_list_noalloc (noalloc.hip).
+// noalloc_scratch_init looks up this device's copy once per device (the
+// device must be current); noalloc_scratch returns it.  Both return false
+// when the address is not available.
+bool noalloc_scratch_init(int device);
+bool noalloc_scratch(int device, char** base, size_t* bytes);
 
 // Per-phase device timing with HIP events on the launching stream (the
 // library's counterpart of the reference's kamping phase timers).
@@ -121,6 +133,12 @@ struct Context {
   uint64_t budget = 0;
   hipStream_t stream = nullptr;
   Arena arena;
+  // The static scratch of kc_rank_list_noalloc (fixed over this device's
+  // copy of the library's static array; unset when its address was not
+  // available), and the arena Scratch allocates from: `arena` except inside
+  // kc_rank_list_noalloc.
+  Arena noalloc_arena;
+  Arena* active_arena = &arena;
   uint32_t* d_err = nullptr;       // error word (inside the counter block)
   uint64_t* d_counters = nullptr;  // kCounterSlots device counters
   uint64_t* h_pinned = nullptr;    // pinned host mirror of counters
@@ -143,17 +161,17 @@ struct Context {
 // Scoped stack allocation from the arena.
 class Scratch {
  public:
-  explicit Scratch(Context* ctx) : ctx_(ctx), mark_(ctx->arena.mark()) {}
-  ~Scratch() { ctx_->arena.release(mark_); }
+  explicit Scratch(Context* ctx) : arena_(ctx->active_arena), mark_(arena_->mark()) {}
+  ~Scratch() { arena_->release(mark_); }
   template <typename T>
   T* get(uint64_t count) {
-    void* p = ctx_->arena.alloc(count * sizeof(T) + 256);
+    void* p = arena_->alloc(count * sizeof(T) + 256);
     if (!p) throw std::bad_alloc();
     return static_cast<T*>(p);
   }
 
  private:
-  Context* ctx_;
+  Arena* arena_;
   size_t mark_;
 };
 
