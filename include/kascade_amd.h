@@ -273,15 +273,21 @@ kc_status kc_root_tree(void *ctx, const uint64_t *parent, uint64_t n, uint64_t *
  * roots; rank_array out = depth (input values ignored, eulertour.cpp:45-53). */
 kc_status kc_rank_via_euler_tour(void *ctx, const kc_tree_cfg *cfg, uint64_t *succ_array,
                                  int64_t *rank_array, uint64_t n, void *stream);
-/* The tour successor array alone (kascade::compute_euler_tour,
- * eulertour.cpp:379-392), in this library's element numbering: element 2v is
- * the upward edge (v, parent v) or, for a root, the tour's terminal dummy;
- * element 2v + 1 is the downward edge (parent v, v) (an isolated self loop for
- * a root).  tour_succ_out has 2n entries, tour_weight_out (may be NULL) the
- * +1/-1/0 weights.  Children are grouped by a stable sort, so the tour is the
- * same on every call. */
+/* The tour successor array alone, in the reference's element order
+ * (kascade::compute_euler_tour, eulertour.cpp:379-392, as one PE builds it):
+ * one element per directed tree edge, grouped by source vertex in ascending
+ * order and, within a vertex, in the order of make_graph's CSR
+ * (successor_utils.cpp:236-260): children above the vertex in descending
+ * order, its parent, children below it in descending order; then one dummy
+ * per root that has children, in vertex order, then one self-looping slot
+ * per childless root.  succ[(u_prev, v)] = (v, u_next) in that cyclic order;
+ * the wrap to a root's first neighbour goes to its dummy
+ * (compute_euler_tour_impl, eulertour.cpp:33-127).  *tour_len_out =
+ * 2(n - roots) + roots; tour_succ_out (and tour_weight_out, may be NULL:
+ * +1 upward, -1 downward, 0 dummies) need capacity 2n. */
 kc_status kc_compute_euler_tour(void *ctx, const uint64_t *parent, uint64_t n,
-                                uint64_t *tour_succ_out, int64_t *tour_weight_out, void *stream);
+                                uint64_t *tour_succ_out, int64_t *tour_weight_out,
+                                uint64_t *tour_len_out, void *stream);
 
 /* ---- list utilities and statistics --------------------------------------- */
 /* *is_list_out = 1 iff no element has two predecessors (self loops ignored).

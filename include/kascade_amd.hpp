@@ -14,8 +14,11 @@
 // cycles) are reported the same way instead of being undefined.
 #pragma once
 
+#include <algorithm>
 #include <any>
+#include <cstddef>
 #include <cstdint>
+#include <type_traits>
 #include <optional>
 #include <span>
 #include <stdexcept>
@@ -36,7 +39,15 @@ constexpr idx_t root_flag_mask = idx_t(1) << 63;
 [[nodiscard]] constexpr auto has_root_flag(idx_t v) noexcept -> bool { return (v & root_flag_mask) != 0; }
 }  // namespace bits
 
-// lib/kascade/configuration.hpp:9-18
+// ---- configuration (lib/kascade/configuration.hpp) --------------------------
+// Every enum, struct, member and default of the reference, so that code which
+// fills these structs compiles unchanged.  Members that only shape MPI
+// communication (grid communicator, aggregation, owner caching, request
+// schemes, the Briefkasten queue, ruler-count strategies sized for p ranks)
+// are accepted and have no effect on one GPU; what each member does here is
+// stated beside it.  Members marked "GPU:" are this library's additions.
+
+// configuration.hpp:9-18
 enum class Algorithm : std::uint8_t {
   GatherChase,
   PointerDoubling,
@@ -48,41 +59,132 @@ enum class Algorithm : std::uint8_t {
   invalid,
 };
 
-// lib/kascade/configuration.hpp:39-47.  The MPI-routing knobs are accepted and
-// ignored (there is no network on one GPU).
+enum class AggregationLevel : std::uint8_t { none, local, all, invalid };  // :20
+
+struct AsyncPointerChasingConfig {  // :22-24 (async doubling = sync doubling here)
+  bool use_caching = false;
+};
+
+enum class RMASyncMode : std::uint8_t { passive_target, fenced, invalid };  // :26
+
+struct RMAPointerChasingConfig {  // :28-31 (RMA doubling = sync doubling here)
+  RMASyncMode sync_mode = RMASyncMode::passive_target;
+  std::size_t batch_size = 1;
+};
+
+enum class GridCommunicatorMode : std::uint8_t { topology_aware, balanced, invalid };  // :33-37
+
+// :39-47.  Every member is an MPI-routing knob: accepted, no effect.
 struct PointerDoublingConfig {
   bool use_local_preprocessing = true;
+  GridCommunicatorMode grid_communicator_mode = GridCommunicatorMode::topology_aware;
   bool use_grid_communication = false;
   bool use_local_first_request_scheme = true;
   bool cache_succ_owners = true;
+  AggregationLevel aggregation_level = AggregationLevel::none;
   double fallback_allgather_size_ratio = 0.0;
 };
 
+// :49-57
 enum class RulerSelectionStrategy : std::uint8_t {
   dehne, heuristic, sanders, ultimate, ultimate_sanders, limit_rounds, invalid
 };
+enum class RulerPropagationMode : std::uint8_t { pull, push, invalid };  // :58
 
-// lib/kascade/configuration.hpp:65-95.  On the GPU, rulers are one per block of
-// `ruler_spacing` ids (0 = the autotuned spacing for the list size) and the
-// recursion depth is `sparse_ruling_set_rounds` (0 = until the one-workgroup
-// base case).  The ruler-count strategies of the reference size an MPI run and
-// are not used.
+struct BriefkastenConfig {  // :60-63 (async message queue; no effect)
+  std::size_t local_threshold = 2048;
+  std::size_t poll_skip_threshold = 1000;
+};
+
+// :65-95.
+//   sparse_ruling_set_rounds: ruling-set levels before the base algorithm
+//     ranks the ruler sublist, as in sparse_ruling_set.cpp:381-392; 0 runs
+//     the base algorithm alone (:376-380).
+//   base_algorithm: PointerDoubling, AsyncPointerDoubling and
+//     RMAPointerDoubling run sync pointer doubling; GatherChase (a gather to
+//     one rank and a sequential walk) runs the ruling set's own recursion,
+//     whose results are identical; other values throw, as :374 does.
+//   ruler_selection and the *_factor members size the ruler set for p MPI
+//     ranks; on the GPU the ruler spacing comes from the autotuned table
+//     (or ruler_spacing below), so they have no effect, and neither do
+//     current_sparse_ruling_set_round, round_limit, ruler_propagation_mode
+//     (always pull, the reference's default), sync / sync_locality_aware /
+//     spawn / no_precompute_rulers / post_invert(_detect_leaves) (the chase
+//     engine is this library's own), use_local_contraction, cache_owners,
+//     root_gather_threshold, the aggregation and locality switches, the
+//     Briefkasten and grid settings.
 struct SparseRulingSetConfig {
   RulerSelectionStrategy ruler_selection = RulerSelectionStrategy::sanders;
   Algorithm base_algorithm = Algorithm::PointerDoubling;
   std::any base_algorithm_config;
-  std::size_t sparse_ruling_set_rounds = 0;
+  std::size_t current_sparse_ruling_set_round = 0;
+  std::size_t sparse_ruling_set_rounds = 1;
+  RulerPropagationMode ruler_propagation_mode = RulerPropagationMode::pull;
+  bool use_aggregation_in_ruler_propagation = false;
+  bool use_local_contraction = false;
+  bool reverse_list_locality_aware = true;
+  bool use_locality_aware_in_ruler_propagation = true;
+  double dehne_factor = 1.0;
+  double heuristic_factor = 0.01;
+  double sanders_factor = 1.0;
+  double ultimate_factor = 0.1;
+  double ultimate_sanders_factor = 2.0;
+  std::size_t round_limit = 100;
+  bool cache_owners = false;
+  std::size_t root_gather_threshold = 100;
+  bool sync = true;
+  bool sync_locality_aware = false;
+  bool spawn = false;
+  bool no_precompute_rulers = false;
+  bool post_invert = false;
+  bool post_invert_detect_leaves = false;
+  BriefkastenConfig briefkasten{};
+  GridCommunicatorMode grid_communicator_mode = GridCommunicatorMode::topology_aware;
+  bool use_grid_communication = false;
+  bool use_local_first_request_scheme = false;
+  // GPU: elements per ruler at the top level and below it (powers of two;
+  // 0 = the autotuned spacing for the size)
   std::uint32_t ruler_spacing = 0;
   std::uint32_t ruler_spacing_deep = 0;
-  bool validate_list = false;  // the reference's level-100 is_list KASSERT, as a real check
+  // GPU: reject in-degree > 1 before ranking (the reference's is_list KASSERT)
+  bool validate_list = false;
 };
 
-// lib/kascade/configuration.hpp:97-103
+// :97-103.  algorithm ranks the tour (eulertour.cpp:282-321); algo_config
+// may hold the SparseRulingSetConfig whose levels it uses.
+// use_high_degree_handling splits high-degree vertices to balance MPI
+// messages: no effect, like the grid settings.
 struct EulerTourConfig {
   Algorithm algorithm = Algorithm::SparseRulingSet;
-  bool use_high_degree_handling = false;  // a communication-volume optimisation; ignored
+  bool use_high_degree_handling = false;
   std::any algo_config;
-  bool reproducible_tour = false;  // group children by radix sort instead of atomic linking
+  GridCommunicatorMode grid_communicator_mode = GridCommunicatorMode::topology_aware;
+  bool use_grid_communication = false;
+  // GPU: group children by radix sort (the same tour on every call) instead
+  // of atomic linking; the results are identical either way
+  bool reproducible_tour = false;
+};
+
+// The reference's block Distribution (lib/kascade/distribution.hpp:16-36)
+// on one GPU: a single block holding every element.  Returned by
+// set_initial_ranking_state and accepted (with a communicator and a grid
+// communicator of any type) by the entry points' reference-shaped overloads.
+class Distribution {
+ public:
+  Distribution() = default;
+  explicit Distribution(std::size_t n) : n_(n) {}
+  template <typename Comm>
+  Distribution(std::size_t n, Comm const& /*comm*/) : n_(n) {}
+  auto get_local_size(int /*rank*/ = 0) const -> std::size_t { return n_; }
+  auto get_global_size() const -> std::size_t { return n_; }
+  auto get_owner(idx_t /*idx*/) const -> int { return 0; }
+  auto get_global_idx(std::size_t local, int /*rank*/ = 0) const -> idx_t { return local; }
+  auto get_local_idx(idx_t global, int /*rank*/ = 0) const -> std::size_t { return global; }
+  auto index_range_begin(int /*rank*/ = 0) const -> idx_t { return 0; }
+  auto index_range_end(int /*rank*/ = 0) const -> idx_t { return n_; }
+
+ private:
+  std::size_t n_ = 0;
 };
 
 class Error : public std::runtime_error {
@@ -126,6 +228,28 @@ inline kc_rank_cfg rank_cfg(int algorithm, std::uint32_t spacing = 0, std::uint3
   c.validate = validate ? 1u : 0u;
   return c;
 }
+// sparse_ruling_set_rounds and base_algorithm as sparse_ruling_set.cpp:343-392
+// read them (see SparseRulingSetConfig).
+inline kc_rank_cfg srs_cfg(SparseRulingSetConfig const& config) {
+  switch (config.base_algorithm) {
+    case Algorithm::GatherChase:
+    case Algorithm::PointerDoubling:
+    case Algorithm::AsyncPointerDoubling:
+    case Algorithm::RMAPointerDoubling:
+      break;
+    default:
+      throw Error(KC_EINVAL, "Invalid base algorithm selected for sparse ruling set");  // sparse_ruling_set.cpp:374
+  }
+  if (config.ruler_selection == RulerSelectionStrategy::invalid)
+    throw Error(KC_EINVAL, "Invalid ruler selection strategy");  // ruler_selection.hpp:74-75
+  const bool gather = config.base_algorithm == Algorithm::GatherChase;
+  if (config.sparse_ruling_set_rounds == 0)  // the base algorithm alone
+    return rank_cfg(gather ? KC_ALGO_SPARSE_RULING_SET : KC_ALGO_POINTER_DOUBLING, config.ruler_spacing,
+                    config.ruler_spacing_deep, 0, config.validate_list);
+  const std::size_t levels = std::min<std::size_t>(config.sparse_ruling_set_rounds, 64);
+  return rank_cfg(KC_ALGO_SPARSE_RULING_SET, config.ruler_spacing, config.ruler_spacing_deep,
+                  gather ? 0u : static_cast<std::uint32_t>(levels), config.validate_list);
+}
 inline int algorithm_code(Algorithm a) {
   switch (a) {
     case Algorithm::PointerDoubling:
@@ -142,12 +266,22 @@ inline int algorithm_code(Algorithm a) {
 }  // namespace detail
 
 // kascade::set_initial_ranking_state (list_ranking.hpp:47-50)
-inline void set_initial_ranking_state(std::span<const idx_t> succ_array, std::span<idx_t> root_array,
-                                      std::span<rank_t> rank_array, Context& ctx = Context::default_context()) {
+inline auto set_initial_ranking_state(std::span<const idx_t> succ_array, std::span<idx_t> root_array,
+                                      std::span<rank_t> rank_array, Context& ctx = Context::default_context())
+    -> Distribution {
   if (root_array.size() != succ_array.size() || rank_array.size() != succ_array.size())
     throw Error(KC_EINVAL, "array sizes differ");
   ctx.check(kc_set_initial_ranking_state(ctx.handle(), succ_array.data(), succ_array.size(),
                                          root_array.data(), rank_array.data(), nullptr));
+  return Distribution(succ_array.size());
+}
+// The reference's signature (list_ranking.hpp:47-50): the communicator, of
+// any type, is ignored; the default context ranks.
+template <typename Comm>
+  requires(!std::is_same_v<std::remove_cvref_t<Comm>, Context>)
+auto set_initial_ranking_state(std::span<const idx_t> succ_array, std::span<idx_t> root_array,
+                               std::span<rank_t> rank_array, Comm const& /*comm*/) -> Distribution {
+  return set_initial_ranking_state(succ_array, root_array, rank_array, Context::default_context());
 }
 
 // kascade::pointer_doubling (pointer_doubling.hpp:18-23): in place,
@@ -164,16 +298,9 @@ inline void pointer_doubling(PointerDoublingConfig /*config*/, std::span<idx_t> 
 inline void sparse_ruling_set(SparseRulingSetConfig const& config, std::span<idx_t> succ_array,
                               std::span<rank_t> rank_array, Context& ctx = Context::default_context()) {
   if (rank_array.size() != succ_array.size()) throw Error(KC_EINVAL, "array sizes differ");
-  if (config.base_algorithm == Algorithm::EulerTour || config.base_algorithm == Algorithm::MPLR ||
-      config.base_algorithm == Algorithm::invalid)
-    throw Error(KC_EINVAL, "Invalid base algorithm selected for sparse ruling set");  // sparse_ruling_set.cpp:374
-  if (config.ruler_selection == RulerSelectionStrategy::invalid)
-    throw Error(KC_EINVAL, "Invalid ruler selection strategy");  // ruler_selection.hpp:74-75
-  kc_rank_cfg c = detail::rank_cfg(KC_ALGO_SPARSE_RULING_SET, config.ruler_spacing, config.ruler_spacing_deep,
-                                   static_cast<std::uint32_t>(config.sparse_ruling_set_rounds),
-                                   config.validate_list);
-  ctx.check(kc_sparse_ruling_set(ctx.handle(), &c, succ_array.data(), rank_array.data(), succ_array.size(),
-                                 nullptr));
+  kc_rank_cfg c = detail::srs_cfg(config);
+  ctx.check(kc_rank_list(ctx.handle(), succ_array.data(), rank_array.data(), succ_array.size(), succ_array.data(),
+                         rank_array.data(), &c, nullptr));
 }
 
 // kascade::rank_via_euler_tour (eulertour.hpp:22-27), in place after
@@ -184,9 +311,117 @@ inline void rank_via_euler_tour(EulerTourConfig const& config, std::span<idx_t> 
   if (config.algorithm == Algorithm::EulerTour) throw Error(KC_EINVAL, "Cannot apply Euler Tour recursively.");
   kc_tree_cfg t{};
   t.rank = detail::rank_cfg(detail::algorithm_code(config.algorithm));
+  if (config.algorithm == Algorithm::SparseRulingSet)
+    if (auto const* srs = std::any_cast<SparseRulingSetConfig>(&config.algo_config)) t.rank = detail::srs_cfg(*srs);
   t.tour_build = config.reproducible_tour ? KC_TOUR_SORTED : KC_TOUR_LINKED;
   ctx.check(kc_rank_via_euler_tour(ctx.handle(), &t, succ_array.data(), rank_array.data(), succ_array.size(),
                                    nullptr));
+}
+
+// The reference's six-argument signatures (pointer_doubling.hpp:18-23,
+// sparse_ruling_set.hpp:13-18, eulertour.hpp:22-27).  One GPU ranks the
+// whole array, so the call must come from a single process holding all of
+// it: a communicator with a size() above 1 throws (gather to one rank first,
+// as GatherRank does, list_ranking.cpp:29-58).  The Distribution and the
+// optional grid communicator, of any type, are otherwise ignored, so call
+// sites compile unchanged.
+namespace detail {
+template <typename Comm>
+void require_one_process(Comm const& comm) {
+  if constexpr (requires { comm.size(); }) {
+    if (comm.size() != 1)
+      throw Error(KC_EINVAL, "the whole array must be on one process: gather it to one rank first");
+  }
+}
+}  // namespace detail
+template <typename Dist, typename Comm, typename Grid>
+void pointer_doubling(PointerDoublingConfig config, std::span<idx_t> succ_array, std::span<rank_t> rank_array,
+                      Dist const& /*dist*/, Comm const& comm, Grid const& /*grid_comm*/) {
+  detail::require_one_process(comm);
+  pointer_doubling(config, succ_array, rank_array);
+}
+template <typename Dist, typename Comm, typename Grid>
+void sparse_ruling_set(SparseRulingSetConfig const& config, std::span<idx_t> succ_array,
+                       std::span<rank_t> rank_array, Dist const& /*dist*/, Comm const& comm,
+                       Grid const& /*grid_comm*/) {
+  detail::require_one_process(comm);
+  sparse_ruling_set(config, succ_array, rank_array);
+}
+template <typename Dist, typename Comm, typename Grid>
+void rank_via_euler_tour(EulerTourConfig const& config, std::span<idx_t> succ_array, std::span<rank_t> rank_array,
+                         Dist const& /*dist*/, Comm const& comm, Grid const& /*grid_comm*/) {
+  detail::require_one_process(comm);
+  rank_via_euler_tour(config, succ_array, rank_array);
+}
+
+// Conversions from the reference's own configuration types (any types with
+// the members of configuration.hpp:39-103; enums convert by value, their
+// enumerators being in the same order), so a harness built against both
+// libraries passes its kascade::*Config objects straight through.
+namespace detail {
+template <typename To, typename From>
+To enum_cast(From f) {
+  return static_cast<To>(static_cast<std::underlying_type_t<From>>(f));
+}
+}  // namespace detail
+template <typename Ref>
+  requires requires(Ref const& r) { r.fallback_allgather_size_ratio; r.cache_succ_owners; }
+auto from_reference(Ref const& r) -> PointerDoublingConfig {
+  PointerDoublingConfig c;
+  c.use_local_preprocessing = r.use_local_preprocessing;
+  c.grid_communicator_mode = detail::enum_cast<GridCommunicatorMode>(r.grid_communicator_mode);
+  c.use_grid_communication = r.use_grid_communication;
+  c.use_local_first_request_scheme = r.use_local_first_request_scheme;
+  c.cache_succ_owners = r.cache_succ_owners;
+  c.aggregation_level = detail::enum_cast<AggregationLevel>(r.aggregation_level);
+  c.fallback_allgather_size_ratio = r.fallback_allgather_size_ratio;
+  return c;
+}
+template <typename Ref>
+  requires requires(Ref const& r) { r.sparse_ruling_set_rounds; r.ruler_selection; }
+auto from_reference(Ref const& r) -> SparseRulingSetConfig {
+  SparseRulingSetConfig c;
+  c.ruler_selection = detail::enum_cast<RulerSelectionStrategy>(r.ruler_selection);
+  c.base_algorithm = detail::enum_cast<Algorithm>(r.base_algorithm);
+  c.current_sparse_ruling_set_round = r.current_sparse_ruling_set_round;
+  c.sparse_ruling_set_rounds = r.sparse_ruling_set_rounds;
+  c.ruler_propagation_mode = detail::enum_cast<RulerPropagationMode>(r.ruler_propagation_mode);
+  c.use_aggregation_in_ruler_propagation = r.use_aggregation_in_ruler_propagation;
+  c.use_local_contraction = r.use_local_contraction;
+  c.reverse_list_locality_aware = r.reverse_list_locality_aware;
+  c.use_locality_aware_in_ruler_propagation = r.use_locality_aware_in_ruler_propagation;
+  c.dehne_factor = r.dehne_factor;
+  c.heuristic_factor = r.heuristic_factor;
+  c.sanders_factor = r.sanders_factor;
+  c.ultimate_factor = r.ultimate_factor;
+  c.ultimate_sanders_factor = r.ultimate_sanders_factor;
+  c.round_limit = r.round_limit;
+  c.cache_owners = r.cache_owners;
+  c.root_gather_threshold = r.root_gather_threshold;
+  c.sync = r.sync;
+  c.sync_locality_aware = r.sync_locality_aware;
+  c.spawn = r.spawn;
+  c.no_precompute_rulers = r.no_precompute_rulers;
+  c.post_invert = r.post_invert;
+  c.post_invert_detect_leaves = r.post_invert_detect_leaves;
+  c.briefkasten.local_threshold = r.briefkasten.local_threshold;
+  c.briefkasten.poll_skip_threshold = r.briefkasten.poll_skip_threshold;
+  c.grid_communicator_mode = detail::enum_cast<GridCommunicatorMode>(r.grid_communicator_mode);
+  c.use_grid_communication = r.use_grid_communication;
+  c.use_local_first_request_scheme = r.use_local_first_request_scheme;
+  return c;
+}
+// RefSrs: the reference's SparseRulingSetConfig type, which algo_config may hold.
+template <typename RefSrs, typename Ref>
+  requires requires(Ref const& r) { r.use_high_degree_handling; r.algo_config; }
+auto from_reference(Ref const& r) -> EulerTourConfig {
+  EulerTourConfig c;
+  c.algorithm = detail::enum_cast<Algorithm>(r.algorithm);
+  c.use_high_degree_handling = r.use_high_degree_handling;
+  if (auto const* srs = std::any_cast<RefSrs>(&r.algo_config)) c.algo_config = from_reference(*srs);
+  c.grid_communicator_mode = detail::enum_cast<GridCommunicatorMode>(r.grid_communicator_mode);
+  c.use_grid_communication = r.use_grid_communication;
+  return c;
 }
 
 // kascade::is_list (successor_utils.hpp): no element has two predecessors.
@@ -290,14 +525,16 @@ inline auto compute_stats(StatsLevel level, std::span<const idx_t> root_array, s
   return out;
 }
 
-// kascade::compute_euler_tour (eulertour.hpp:32-34): the tour successor array
-// in this library's numbering (element 2v = upward edge of v or a root's
-// dummy, element 2v + 1 = downward edge into v).
+// kascade::compute_euler_tour (eulertour.hpp:32-34, eulertour.cpp:379-392):
+// the tour successor array in the reference's element order and length
+// (2(n - roots) + roots; see kc_compute_euler_tour).
 inline auto compute_euler_tour(std::span<const idx_t> parent_array, Context& ctx = Context::default_context())
     -> std::vector<idx_t> {
   std::vector<idx_t> tour(2 * parent_array.size());
+  std::uint64_t len = 0;
   ctx.check(kc_compute_euler_tour(ctx.handle(), parent_array.data(), parent_array.size(), tour.data(), nullptr,
-                                  nullptr));
+                                  &len, nullptr));
+  tour.resize(len);
   return tour;
 }
 
@@ -368,7 +605,14 @@ class SparseRulingSet : public AlgorithmBase {  // algorithm_impl.hpp:161-201
       : AlgorithmBase(ctx), config_(std::move(c)) {}
   void run() override {
     init();
-    sparse_ruling_set(config_, root_array_, rank_array_, *ctx_);
+    SparseRulingSetConfig c = config_;
+    if (c.base_algorithm == Algorithm::SparseRulingSet) {
+      // the legacy base algorithm (algorithm_impl.hpp:166-171): at least two
+      // ruling-set levels over a pointer-doubling base
+      c.sparse_ruling_set_rounds = std::max(std::size_t{2}, c.sparse_ruling_set_rounds);
+      c.base_algorithm = Algorithm::PointerDoubling;
+    }
+    sparse_ruling_set(c, root_array_, rank_array_, *ctx_);
   }
 
  private:

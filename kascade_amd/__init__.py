@@ -121,7 +121,7 @@ _SIGS = {
     "kc_rank_lists_segmented": (ctypes.c_int, [_P, _P, _P, _U64, _P, _P, _P, ctypes.POINTER(RankCfg), _P]),
     "kc_root_tree": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.POINTER(TreeCfg), _P]),
     "kc_rank_via_euler_tour": (ctypes.c_int, [_P, ctypes.POINTER(TreeCfg), _P, _P, _U64, _P]),
-    "kc_compute_euler_tour": (ctypes.c_int, [_P, _P, _U64, _P, _P, _P]),
+    "kc_compute_euler_tour": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.POINTER(_U64), _P]),
     "kc_gen_list": (ctypes.c_int, [_P, _U64, _U64, ctypes.c_int, ctypes.c_int, _P, _P, _P]),
     "kc_gen_tree": (ctypes.c_int, [_P, _U64, _U64, _P, _P]),
     "kc_gen_lists_segmented": (ctypes.c_int, [_P, _P, _U64, _P, _P, ctypes.c_int, _P, _P, _P]),
@@ -492,12 +492,16 @@ class Context:
                                                      _ptr(rank_array, np.int64), _numel(succ_array), _st(stream)))
 
     def compute_euler_tour(self, parent, stream=None):
+        """kascade::compute_euler_tour in the reference's element order:
+        (tour successors, tour weights), 2(n - roots) + roots entries."""
         n = _numel(parent)
         s = _like(parent, np.uint64, 2 * n)
         w = _like(parent, np.int64, 2 * n)
+        ln = ctypes.c_uint64()
         self._check(self._lib.kc_compute_euler_tour(self.handle, _ptr(parent, np.uint64), n,
-                                                    _ptr(s, np.uint64), _ptr(w, np.int64), _st(stream)))
-        return s, w
+                                                    _ptr(s, np.uint64), _ptr(w, np.int64), ctypes.byref(ln),
+                                                    _st(stream)))
+        return s[:ln.value], w[:ln.value]
 
     # ---- synthetic inputs --------------------------------------------------------
     def gen_list(self, n, seed, weighted=True, local=False, device="cuda", stream=None):

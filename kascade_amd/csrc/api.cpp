@@ -625,8 +625,8 @@ kc_status kc_rank_via_euler_tour(void* vctx, const kc_tree_cfg* cfg, uint64_t* s
   return kc_root_tree(vctx, succ_array, n, succ_array, rank_array, cfg, stream);
 }
 
-kc_status kc_compute_euler_tour(void* vctx, const uint64_t* parent, uint64_t n,
-                                uint64_t* tour_succ_out, int64_t* tour_weight_out, void* stream) {
+kc_status kc_compute_euler_tour(void* vctx, const uint64_t* parent, uint64_t n, uint64_t* tour_succ_out,
+                                int64_t* tour_weight_out, uint64_t* tour_len_out, void* stream) {
   return guarded(vctx, [&](Context* ctx) {
     if (n && (!parent || !tour_succ_out)) throw KcError{KC_EINVAL, "null array"};
     hipStream_t st = pick(ctx, stream);
@@ -634,16 +634,11 @@ kc_status kc_compute_euler_tour(void* vctx, const uint64_t* parent, uint64_t n,
     Staged<const uint64_t> p(ctx, scratch, parent, n, true, false, st);
     Staged<uint64_t> so(ctx, scratch, tour_succ_out, 2 * n, false, true, st);
     Staged<int64_t> wo(ctx, scratch, tour_weight_out, 2 * n, false, true, st);
-    kc::Rec* tour = scratch.get<kc::Rec>(2 * n + 1);
-    kc::euler_tour_device(ctx, p.dev(), n, tour, st);
-    KC_HIP(hipMemcpy2DAsync(so.dev(), sizeof(uint64_t), &tour[0].s, sizeof(kc::Rec), sizeof(uint64_t),
-                            2 * n, hipMemcpyDeviceToDevice, st));
-    if (wo.dev())
-      KC_HIP(hipMemcpy2DAsync(wo.dev(), sizeof(int64_t), &tour[0].w, sizeof(kc::Rec), sizeof(int64_t),
-                              2 * n, hipMemcpyDeviceToDevice, st));
+    const uint64_t len = kc::compute_euler_tour_device(ctx, p.dev(), n, so.dev(), wo.dev(), st);
     so.finish();
     wo.finish();
     KC_HIP(hipStreamSynchronize(st));
+    if (tour_len_out) *tour_len_out = len;
   });
 }
 

@@ -355,8 +355,11 @@ void compute_stats_device(Context* ctx, const uint64_t* root, const int64_t* ran
                           int64_t* tree_sum, uint64_t tree_cap, hipStream_t st);
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
                       int64_t* depth, const kc_tree_cfg* cfg, hipStream_t st);
-void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
-                       hipStream_t st);
+// compute_euler_tour in the reference's element order: tour_succ (and
+// tour_w, may be null) receive 2(n - roots) + roots entries; returns that
+// length.  The arrays need capacity 2n.
+uint64_t compute_euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* tour_succ,
+                                   int64_t* tour_w, hipStream_t st);
 void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* seg_off,
                            uint64_t nseg, uint64_t total, const int64_t* w, uint32_t* root,
                            int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st);
@@ -374,7 +377,7 @@ void radix_sort_pairs_u32(Context* ctx, uint32_t* keys, uint32_t* vals, uint64_t
                           unsigned key_bits, hipStream_t st);
 void radix_sort_pairs_u64(Context* ctx, uint64_t* keys, uint64_t* vals, uint64_t n,
                           unsigned key_bits, hipStream_t st);
-// Exclusive scan of u64 (returns total on the host).
+// Exclusive scan of u64 on the device (enqueued on `st`; returns 0).
 uint64_t exclusive_scan_u64(Context* ctx, const uint64_t* in, uint64_t* out, uint64_t n,
                             hipStream_t st);
 

@@ -186,13 +186,155 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
   return counts[0];
 }
 
-void euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour,
-                       hipStream_t st) {
-  if (n == 0) return;
+// ---------------------------------------------------------------------------
+// compute_euler_tour (eulertour.cpp:379-392) in the reference's own element
+// order, so that a tour handed over as an input list is the same list.  On
+// one PE the reference builds the CSR of reverse_rooted_tree / make_graph
+// (successor_utils.cpp:236-300): the edges of vertex v are emitted in the
+// order of the loop index that creates them (child c at index c, the parent
+// at index v) and filled back to front, so v's neighbours are its children
+// above v in descending order, then its parent, then its children below v in
+// descending order.  Edge (v, nb_v[i]) is element off(v) + i, off = exclusive
+// scan of the degrees; compute_euler_tour_impl (eulertour.cpp:76-111) links
+// succ[(x, y)] = (y, the neighbour after x in y's cyclic order), except that
+// the wrap to a root's first neighbour goes to the root's dummy, element
+// E + (roots with children before it), E = 2(n - roots); dummies and the
+// slots of childless roots are self loops (:116).  Weights (:45-53): +1 on an
+// upward edge, -1 on a downward edge, 0 on the dummies.
+// Here the neighbour positions follow from each vertex's child count k, the
+// number L of children below it, and a child's rank m among its siblings
+// (ascending ids, from a stable radix sort of (parent, child)):
+//   pos of child c_m in p's list = k-1-m if m >= L, else k-m with a parent
+//   slot (p not a root) or k-1-m without; the parent sits at k - L.
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void __launch_bounds__(kThreads)
+ref_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __restrict__ keys,
+                uint32_t* __restrict__ vals, uint32_t* err, unsigned long long* roots) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool is_root = false;
+  if (v < n) {
+    uint64_t p = parent[v];
+    if (p >= n) {
+      raise_err(err, kErrRange);
+      p = v;
+    }
+    is_root = (p == v);
+    keys[v] = is_root ? static_cast<uint32_t>(n) : static_cast<uint32_t>(p);
+    vals[v] = static_cast<uint32_t>(v);
+  }
+  wave_append(roots, is_root);
+}
+
+// Over the sorted pairs: first / one-past-last position of each parent's
+// children, and one past the last child below the parent (0 = none).
+__global__ void __launch_bounds__(kThreads)
+ref_groups_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals, uint64_t n,
+                  uint64_t* __restrict__ start, uint64_t* __restrict__ end, uint64_t* __restrict__ lpos) {
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t p = keys[j];
+  if (p == n) return;  // roots sort last
+  const bool first = (j == 0) || keys[j - 1] != p;
+  const bool last = (j + 1 == n) || keys[j + 1] != p;
+  if (first) start[p] = j;
+  if (last) end[p] = j + 1;
+  const uint32_t c = vals[j];
+  if (c < p && (last || vals[j + 1] > p)) lpos[p] = j + 1;
+}
+
+// deg(v) = children + (v not a root); droot(v) = 1 for a root with children.
+__global__ void __launch_bounds__(kThreads)
+ref_degree_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint64_t* __restrict__ start,
+                  const uint64_t* __restrict__ end, uint64_t* __restrict__ deg, uint64_t* __restrict__ droot) {
+  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (v >= n) return;
+  const uint64_t k = end[v] ? end[v] - start[v] : 0;
+  const bool root = parent[v] == v || parent[v] >= n;
+  deg[v] = k + (root ? 0 : 1);
+  droot[v] = (root && k > 0) ? 1 : 0;
+}
+
+// Both elements of every tree edge (c, p): (c, p) = element off(c) + k_c - L_c
+// and (p, c) = element off(p) + pos_p(c).
+__global__ void __launch_bounds__(kThreads)
+ref_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                const uint64_t* __restrict__ parent, uint64_t n, const uint64_t* __restrict__ start,
+                const uint64_t* __restrict__ end, const uint64_t* __restrict__ lpos,
+                const uint64_t* __restrict__ off, const uint64_t* __restrict__ droot, uint64_t E,
+                uint64_t* __restrict__ tsucc, int64_t* __restrict__ tw) {
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= n) return;
+  const uint64_t p = keys[j];
+  if (p == n) return;
+  const uint64_t c = vals[j];
+  const uint64_t m = j - start[p];
+  const uint64_t kp = end[p] - start[p];
+  const uint64_t lp = lpos[p] ? lpos[p] - start[p] : 0;
+  const bool proot = parent[p] == p;
+  const uint64_t kc = end[c] ? end[c] - start[c] : 0;
+  const uint64_t lc = lpos[c] ? lpos[c] - start[c] : 0;
+  const uint64_t degp = kp + (proot ? 0 : 1);
+  const uint64_t degc = kc + 1;
+  const uint64_t pos_pc = m >= lp ? kp - 1 - m : kp - m - (proot ? 1 : 0);
+  // (c, p) -> (p, the neighbour after c in p's list), or p's dummy on the wrap
+  const uint64_t a = off[c] + (kc - lc);
+  const uint64_t ip = (pos_pc + 1) % degp;
+  tsucc[a] = (proot && ip == 0) ? E + droot[p] : off[p] + ip;
+  // (p, c) -> (c, the neighbour after p in c's list)
+  const uint64_t b = off[p] + pos_pc;
+  tsucc[b] = off[c] + (kc - lc + 1) % degc;
+  if (tw) {
+    tw[a] = 1;
+    tw[b] = -1;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+ref_dummies_kernel(uint64_t E, uint64_t roots, uint64_t* __restrict__ tsucc, int64_t* __restrict__ tw) {
+  uint64_t t = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (t >= roots) return;
+  tsucc[E + t] = E + t;
+  if (tw) tw[E + t] = 0;
+}
+
+}  // namespace
+
+uint64_t compute_euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* tour_succ,
+                                   int64_t* tour_w, hipStream_t st) {
+  if (n == 0) return 0;
+  if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "compute_euler_tour supports n < 2^32 - 1"};
   Scratch scratch(ctx);
-  uint64_t* heads = scratch.get<uint64_t>(n);
-  uint64_t nheads = 0;
-  build_tour(ctx, parent, n, tour, true, heads, &nheads, st);
+  uint32_t* keys = scratch.get<uint32_t>(n);
+  uint32_t* vals = scratch.get<uint32_t>(n);
+  uint64_t* start = scratch.get<uint64_t>(n);
+  uint64_t* end = scratch.get<uint64_t>(n);
+  uint64_t* lpos = scratch.get<uint64_t>(n);
+  uint64_t* deg = scratch.get<uint64_t>(n);
+  uint64_t* droot = scratch.get<uint64_t>(n);
+  uint64_t* off = scratch.get<uint64_t>(n);
+  uint64_t* droot_off = scratch.get<uint64_t>(n);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  const unsigned blocks = blocks_for(n, kThreads);
+  KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+  KC_HIP(hipMemsetAsync(end, 0, n * sizeof(uint64_t), st));
+  KC_HIP(hipMemsetAsync(lpos, 0, n * sizeof(uint64_t), st));
+  hipLaunchKernelGGL(ref_keys_kernel, dim3(blocks), dim3(kThreads), 0, st, parent, n, keys, vals, ctx->d_err, ctr);
+  uint64_t roots = 0;
+  read_counters(ctx, st, 0, 1, &roots);  // also raises an out-of-range parent
+  radix_sort_pairs_u32(ctx, keys, vals, n, bits_for(n), st);
+  hipLaunchKernelGGL(ref_groups_kernel, dim3(blocks), dim3(kThreads), 0, st, keys, vals, n, start, end, lpos);
+  hipLaunchKernelGGL(ref_degree_kernel, dim3(blocks), dim3(kThreads), 0, st, parent, n, start, end, deg, droot);
+  exclusive_scan_u64(ctx, deg, off, n, st);
+  exclusive_scan_u64(ctx, droot, droot_off, n, st);
+  const uint64_t E = 2 * (n - roots);
+  hipLaunchKernelGGL(ref_link_kernel, dim3(blocks), dim3(kThreads), 0, st, keys, vals, parent, n, start, end, lpos,
+                     off, droot_off, E, tour_succ, tour_w);
+  hipLaunchKernelGGL(ref_dummies_kernel, dim3(blocks_for(roots, kThreads)), dim3(kThreads), 0, st, E, roots,
+                     tour_succ, tour_w);
+  check_err(ctx, st);
+  return E + roots;
 }
 
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,

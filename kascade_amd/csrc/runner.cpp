@@ -288,10 +288,11 @@ void generate(const Config& c, const Device& d, std::vector<uint64_t>& succ, std
     if (c.input_processing == "eulertour") {
       // compute_euler_tour (eulertour.cpp:379-392): the tour is the input list
       uint64_t* tour = dalloc<uint64_t>(2 * n);
-      d.check(kc_compute_euler_tour(d.ctx, ds, n, tour, nullptr, nullptr), "kc_compute_euler_tour");
-      succ.assign(2 * n, 0);
-      w.assign(2 * n, 0);
-      hip_check(hipMemcpy(succ.data(), tour, 2 * n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
+      uint64_t len = 0;
+      d.check(kc_compute_euler_tour(d.ctx, ds, n, tour, nullptr, &len, nullptr), "kc_compute_euler_tour");
+      succ.assign(len, 0);
+      w.assign(len, 0);
+      hip_check(hipMemcpy(succ.data(), tour, len * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");
       (void)hipFree(tour);
     } else {
       hip_check(hipMemcpy(succ.data(), ds, n * sizeof(uint64_t), hipMemcpyDeviceToHost), "copy");

@@ -510,13 +510,18 @@ static int et_cmp(const void *a, const void *b) {
   if (x->dst != y->dst) return x->dst < y->dst ? -1 : 1;
   return 0;
 }
-int orc_euler_tour_rooting(uint64_t *succ, int64_t *rank, uint64_t n) {
-  /* set_initial_ranking_state (list_ranking.cpp:60-80) */
+/* The reference's tour of a parent array on one PE (compute_euler_tour,
+ * eulertour.cpp:379-392, via reverse_rooted_tree successor_utils.cpp:265-300,
+ * make_graph :236-260 and compute_euler_tour_impl eulertour.cpp:33-127):
+ * one element per directed edge in CSR order (edge id = CSR position), then
+ * one dummy per root in vertex order (roots without children keep their
+ * self-looping slot at the end).  Fills tsucc/tw/up/efirst/esecond (ne =
+ * 2(n - #roots) + #roots entries, allocated here); returns -1 on failure. */
+static int reference_tour(const uint64_t *succ, uint64_t n, uint64_t **tsucc_o, int64_t **tw_o,
+                          unsigned char **up_o, uint64_t **efirst_o, uint64_t **esecond_o, uint64_t *ne_o) {
   uint64_t nroots = 0, m = 0;
-  for (uint64_t i = 0; i < n; i++) {
-    rank[i] = (succ[i] == i) ? 0 : 1;
+  for (uint64_t i = 0; i < n; i++)
     if (succ[i] == i) nroots++;
-  }
   m = 2 * (n - nroots); /* directed edges */
   /* reverse_rooted_tree: per non-root i, back edge (i, p) then (p, i) */
   uint64_t *esrc = (uint64_t *)malloc((m ? m : 1) * sizeof(uint64_t));
@@ -549,7 +554,8 @@ int orc_euler_tour_rooting(uint64_t *succ, int64_t *rank, uint64_t n) {
   uint64_t *esecond = (uint64_t *)malloc((ne ? ne : 1) * sizeof(uint64_t));
   for (uint64_t k = 0; k < ne; k++) tsucc[k] = k; /* iota :116 */
   uint64_t root_index = m;
-  for (uint64_t v = 0; v < n; v++) { /* compute_euler_tour_impl :76-111 */
+  int ok = 1;
+  for (uint64_t v = 0; v < n && ok; v++) { /* compute_euler_tour_impl :76-111 */
     uint64_t deg = xadj[v + 1] - xadj[v];
     int is_v_root = (succ[v] == v);
     for (uint64_t i = 0; i < deg; i++) {
@@ -561,7 +567,7 @@ int orc_euler_tour_rooting(uint64_t *succ, int64_t *rank, uint64_t n) {
       tw[id] = up[id] ? 1 : -1; /* assign_weight :45-53, no proxies */
       et_edge key = {u_prev, v, 0};
       et_edge *hit = (et_edge *)bsearch(&key, tab, m, sizeof(et_edge), et_cmp);
-      if (!hit) goto fail;
+      if (!hit) { ok = 0; break; }
       if (is_v_root && i == 0) { /* :92-104 break the tour at the root */
         tsucc[hit->pos] = root_index;
         tw[root_index] = 0;
@@ -573,19 +579,44 @@ int orc_euler_tour_rooting(uint64_t *succ, int64_t *rank, uint64_t n) {
       }
     }
   }
+  free(esrc); free(edst); free(xadj); free(adj); free(tab);
+  if (!ok) {
+    free(tsucc); free(tw); free(up); free(efirst); free(esecond);
+    return -1;
+  }
+  *tsucc_o = tsucc; *tw_o = tw; *up_o = up; *efirst_o = efirst; *esecond_o = esecond; *ne_o = ne;
+  return 0;
+}
+
+/* compute_euler_tour (eulertour.cpp:379-392): the tour successor array (and
+ * the +1 / -1 / 0 weights compute_euler_tour_impl assigns, :45-53) into
+ * arrays of capacity 2n; returns the tour's length, or -1. */
+int64_t orc_compute_euler_tour(const uint64_t *parent, uint64_t n, uint64_t *tsucc_out, int64_t *tw_out) {
+  uint64_t *tsucc, *efirst, *esecond, ne;
+  int64_t *tw;
+  unsigned char *up;
+  if (reference_tour(parent, n, &tsucc, &tw, &up, &efirst, &esecond, &ne) != 0) return -1;
+  memcpy(tsucc_out, tsucc, ne * sizeof(uint64_t));
+  if (tw_out) memcpy(tw_out, tw, ne * sizeof(int64_t));
+  free(tsucc); free(tw); free(up); free(efirst); free(esecond);
+  return (int64_t)ne;
+}
+
+int orc_euler_tour_rooting(uint64_t *succ, int64_t *rank, uint64_t n) {
+  /* set_initial_ranking_state (list_ranking.cpp:60-80) */
+  for (uint64_t i = 0; i < n; i++) rank[i] = (succ[i] == i) ? 0 : 1;
+  uint64_t *tsucc, *efirst, *esecond, ne;
+  int64_t *tw;
+  unsigned char *up;
+  if (reference_tour(succ, n, &tsucc, &tw, &up, &efirst, &esecond, &ne) != 0) return -1;
   orc_local_pointer_chasing(tsucc, tw, ne);
   /* map_euler_tour_back :235-278: upward edge (v, parent) -> (root, depth) */
-  for (uint64_t k = 0; k < m; k++) {
+  for (uint64_t k = 0; k < ne; k++) {
     if (!up[k]) continue;
     uint64_t v = efirst[k];
     succ[v] = esecond[tsucc[k]]; /* request_original_succesors: dummy -> root */
     rank[v] = tw[k];
   }
-  free(esrc); free(edst); free(xadj); free(adj); free(tab);
   free(tsucc); free(tw); free(up); free(efirst); free(esecond);
   return 0;
-fail:
-  free(esrc); free(edst); free(xadj); free(adj); free(tab);
-  free(tsucc); free(tw); free(up); free(efirst); free(esecond);
-  return -1;
 }

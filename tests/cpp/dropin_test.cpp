@@ -6,12 +6,15 @@
 // verification.cpp:9-88).  Exit status 0 = all checks passed.
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <numeric>
+#include <optional>
 #include <vector>
 
 #include "kascade_amd.hpp"
 
 extern "C" {  // CPU oracle (oracle/oracle.c), the checker
+int64_t orc_compute_euler_tour(const uint64_t* parent, uint64_t n, uint64_t* tsucc, int64_t* tw);
 void orc_gen_list(uint64_t n, uint64_t seed, int weighted, uint64_t* succ, int64_t* w);
 void orc_gen_tree(uint64_t n, uint64_t seed, uint64_t* parent);
 void orc_local_pointer_chasing(uint64_t* succ, int64_t* rank, uint64_t n);
@@ -170,7 +173,163 @@ static void utilities_match_oracle() {
   EXPECT((*st.extensive_stats->per_tree_stats)[0].size == n);
 }
 
+// Code written against lib/kascade's configuration and entry points, with
+// `namespace kascade = kascade_amd;`: a harness-style adapter fills every
+// configuration member the reference declares (configuration.hpp:39-103),
+// stores configs in the std::any members as bench/detail/algorithm_impl.hpp
+// does, and calls the six-argument entry points with a Distribution, a
+// communicator and a grid communicator of its own types.
+namespace kascade = kascade_amd;
+struct FakeComm {
+  int size_ = 1;
+  int rank() const { return 0; }
+  int size() const { return size_; }
+};
+struct FakeGrid {};
+// A stand-in for the reference's own configuration type (same members).
+struct RefSrsConfig : kascade_amd::SparseRulingSetConfig {};
+
+static void reference_shaped_calls() {
+  kascade::PointerDoublingConfig pd;
+  pd.use_local_preprocessing = true;
+  pd.grid_communicator_mode = kascade::GridCommunicatorMode::balanced;
+  pd.use_grid_communication = false;
+  pd.use_local_first_request_scheme = true;
+  pd.cache_succ_owners = true;
+  pd.aggregation_level = kascade::AggregationLevel::local;
+  pd.fallback_allgather_size_ratio = 3.0;
+  kascade::AsyncPointerChasingConfig apc;
+  apc.use_caching = true;
+  kascade::RMAPointerChasingConfig rma;
+  rma.sync_mode = kascade::RMASyncMode::fenced;
+  rma.batch_size = 16;
+  kascade::SparseRulingSetConfig srs;
+  EXPECT(srs.sparse_ruling_set_rounds == 1);  // the reference's default
+  srs.ruler_selection = kascade::RulerSelectionStrategy::ultimate_sanders;
+  srs.base_algorithm = kascade::Algorithm::PointerDoubling;
+  srs.base_algorithm_config = pd;
+  srs.base_algorithm_config = apc;
+  srs.base_algorithm_config = rma;
+  srs.base_algorithm_config = nullptr;
+  srs.current_sparse_ruling_set_round = 0;
+  srs.sparse_ruling_set_rounds = std::max(std::size_t{2}, srs.sparse_ruling_set_rounds);
+  srs.ruler_propagation_mode = kascade::RulerPropagationMode::pull;
+  srs.use_aggregation_in_ruler_propagation = false;
+  srs.use_local_contraction = true;
+  srs.reverse_list_locality_aware = true;
+  srs.use_locality_aware_in_ruler_propagation = true;
+  srs.dehne_factor = 1.0;
+  srs.heuristic_factor = 0.01;
+  srs.sanders_factor = 1.0;
+  srs.ultimate_factor = 0.1;
+  srs.ultimate_sanders_factor = 2.0;
+  srs.round_limit = 100;
+  srs.cache_owners = true;
+  srs.root_gather_threshold = 100;
+  srs.sync = true;
+  srs.sync_locality_aware = true;
+  srs.spawn = true;
+  srs.no_precompute_rulers = false;
+  srs.post_invert = true;
+  srs.post_invert_detect_leaves = false;
+  srs.briefkasten.local_threshold = 2048;
+  srs.briefkasten.poll_skip_threshold = 1000;
+  srs.grid_communicator_mode = kascade::GridCommunicatorMode::topology_aware;
+  srs.use_grid_communication = false;
+  srs.use_local_first_request_scheme = false;
+  kascade::EulerTourConfig et;
+  et.algorithm = kascade::Algorithm::SparseRulingSet;
+  et.use_high_degree_handling = false;
+  et.algo_config = srs;
+  et.grid_communicator_mode = kascade::GridCommunicatorMode::topology_aware;
+  et.use_grid_communication = false;
+
+  FakeComm comm;
+  std::optional<FakeGrid> grid;
+  const uint64_t n = 60013;
+  std::vector<kascade::idx_t> succ(n), root(n);
+  std::vector<kascade::rank_t> w(n), rank(n);
+  orc_gen_list(n, 123, 0, succ.data(), w.data());
+  std::vector<kascade::idx_t> eroot = succ;
+  std::vector<kascade::rank_t> erank(n);
+  for (uint64_t i = 0; i < n; i++) erank[i] = succ[i] == i ? 0 : 1;
+  orc_local_pointer_chasing(eroot.data(), erank.data(), n);
+  // every rounds / base-algorithm combination gives the same (unique) result
+  for (std::size_t rounds : {0, 1, 2, 5}) {
+    for (auto base : {kascade::Algorithm::PointerDoubling, kascade::Algorithm::GatherChase,
+                      kascade::Algorithm::AsyncPointerDoubling, kascade::Algorithm::RMAPointerDoubling}) {
+      auto dist = kascade::set_initial_ranking_state(succ, root, rank, comm);
+      EXPECT(dist.get_local_size() == n);
+      kascade::SparseRulingSetConfig c = srs;
+      c.sparse_ruling_set_rounds = rounds;
+      c.base_algorithm = base;
+      kascade::sparse_ruling_set(c, root, rank, dist, comm, grid);
+      EXPECT(root == eroot && rank == erank);
+    }
+  }
+  auto dist = kascade::set_initial_ranking_state(succ, root, rank, comm);
+  kascade::pointer_doubling(pd, root, rank, dist, comm, grid);
+  EXPECT(root == eroot && rank == erank);
+  // the legacy base algorithm is promoted by the adapter
+  kascade::SparseRulingSetConfig legacy;
+  legacy.base_algorithm = kascade::Algorithm::SparseRulingSet;
+  kascade::SparseRulingSet adapter(legacy);
+  adapter.ingest(succ);
+  adapter.run();
+  EXPECT(adapter.get_root_array() == eroot && adapter.get_rank_array() == erank);
+  // a base algorithm the reference rejects throws
+  kascade::SparseRulingSetConfig badc;
+  badc.base_algorithm = kascade::Algorithm::MPLR;
+  bool thrown = false;
+  try {
+    kascade::sparse_ruling_set(badc, root, rank, dist, comm, grid);
+  } catch (const std::runtime_error&) {
+    thrown = true;
+  }
+  EXPECT(thrown);
+  // more than one process: the whole array is not here
+  FakeComm two{2};
+  thrown = false;
+  try {
+    kascade::sparse_ruling_set(srs, root, rank, dist, two, grid);
+  } catch (const kascade_amd::Error& e) {
+    thrown = e.status == KC_EINVAL;
+  }
+  EXPECT(thrown);
+  // conversions from a reference-typed configuration keep every member
+  RefSrsConfig ref;
+  ref.sparse_ruling_set_rounds = 3;
+  ref.spawn = true;
+  ref.ruler_selection = kascade::RulerSelectionStrategy::heuristic;
+  auto conv = kascade_amd::from_reference(static_cast<kascade_amd::SparseRulingSetConfig const&>(ref));
+  EXPECT(conv.sparse_ruling_set_rounds == 3 && conv.spawn &&
+         conv.ruler_selection == kascade::RulerSelectionStrategy::heuristic);
+  auto pconv = kascade_amd::from_reference(pd);
+  EXPECT(pconv.aggregation_level == kascade::AggregationLevel::local && pconv.fallback_allgather_size_ratio == 3.0);
+  auto econv = kascade_amd::from_reference<kascade_amd::SparseRulingSetConfig>(et);
+  EXPECT(econv.algorithm == kascade::Algorithm::SparseRulingSet && econv.algo_config.has_value());
+
+  // tree rooting through the Euler tour with the SRS config in algo_config,
+  // and compute_euler_tour in the reference's order and length
+  const uint64_t nt = 20011;
+  std::vector<kascade::idx_t> parent(nt), troot(nt);
+  std::vector<kascade::rank_t> tdepth(nt);
+  orc_gen_tree(nt, 77, parent.data());
+  std::vector<kascade::idx_t> er2 = parent;
+  std::vector<kascade::rank_t> ed2(nt);
+  orc_euler_tour_rooting(er2.data(), ed2.data(), nt);
+  auto tdist = kascade::set_initial_ranking_state(parent, troot, tdepth, comm);
+  kascade::rank_via_euler_tour(et, troot, tdepth, tdist, comm, grid);
+  EXPECT(troot == er2 && tdepth == ed2);
+  auto tour = kascade::compute_euler_tour(parent);
+  std::vector<uint64_t> etour(2 * nt);
+  const int64_t len = orc_compute_euler_tour(parent.data(), nt, etour.data(), nullptr);
+  etour.resize(static_cast<size_t>(len));
+  EXPECT(tour == etour);
+}
+
 int main() {
+  reference_shaped_calls();
   pack_works();
   utilities_match_oracle();
   pack_sanity_check();

@@ -45,6 +45,7 @@ def lib():
             "orc_pack": (c.c_int, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_unpack": (None, [u64p, i64p, u64p, c.c_uint64, u64p, i64p]),
             "orc_euler_tour_rooting": (c.c_int, [u64p, i64p, c.c_uint64]),
+            "orc_compute_euler_tour": (c.c_int64, [u64p, c.c_uint64, u64p, i64p]),
             "orc_euler_tour_rooting_mt": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int]),
             "orc_sparse_ruling_set": (c.c_int, [u64p, i64p, c.c_uint64, c.c_int, c.c_int]),
             "orc_is_list": (c.c_int, [u64p, c.c_uint64]),
@@ -156,6 +157,17 @@ def euler_tour_rooting(parent):
     rc = lib().orc_euler_tour_rooting(_p64(s), _p64(r), len(s))
     assert rc == 0
     return s, r
+
+
+def compute_euler_tour(parent):
+    """compute_euler_tour on one PE: (tour successors, tour weights)."""
+    parent = np.ascontiguousarray(parent, dtype=np.uint64)
+    n = len(parent)
+    ts = np.empty(max(2 * n, 1), np.uint64)
+    tw = np.empty(max(2 * n, 1), np.int64)
+    k = lib().orc_compute_euler_tour(_p64(parent), n, _p64(ts), _p64(tw))
+    assert k >= 0
+    return ts[:k], tw[:k]
 
 
 def euler_tour_rooting_mt(parent, p=4):

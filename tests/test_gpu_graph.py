@@ -59,9 +59,11 @@ def test_graph_forest_to_tree_rooting_and_tour(ctx):
     er, edp = forest_root_depth(o.bfs_forest(ex, ea))
     assert (host(root) == er).all() and (host(depth) == edp).all()
     ts, tw = ctx.compute_euler_tour(parent)
+    es, ew = o.compute_euler_tour(host(parent))
+    assert (host(ts) == es).all() and (host(tw) == ew).all()
     troot, trank = ctx.rank_list(ts, tw)
-    # element 2v is v's upward edge: its rank is depth(v), its tail 2 * root(v)
-    assert (host(trank)[0::2] == edp).all() and (host(troot)[0::2] == 2 * er).all()
+    cr, ck = o.sequential_walk(es, ew)
+    assert (host(troot) == cr).all() and (host(trank) == ck).all()
 
 
 def test_bfs_forest_rejects_asymmetric_adjacency(ctx):

@@ -4,6 +4,7 @@ import json
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 import oracle_lib as o
@@ -104,16 +105,24 @@ def test_runner_graph_inputs(tmp_path):
     src, dst = o.gen_graph("rmat", 65536, 400000, 1)
     xadj, adj = o.csr(src, dst, 65536)
     parent = o.bfs_forest(xadj, adj)
-    import numpy as np
     wts = (parent != np.arange(65536, dtype=np.uint64)).astype(np.int64)
     er, ed = o.sequential_walk(parent, wts)
     assert rep["stats"]["input_stats"]["basic_stats"] == o.compute_stats(er, ed, "basic")["basic_stats"]
     r, rep = run(tmp_path, "--algorithm", "SparseRulingSet", "--input", "gnm", "--n", "100000", "--m", "150000",
                  "--input-processing", "eulertour")
     assert r.returncode == 0 and "Rank array matches reference." in r.stderr
-    assert rep["stats"]["input_distribution"]["total_size"] == 200000
+    # the reference's tour: 2 (n - roots) + roots elements
+    src, dst = o.gen_graph("gnm", 100000, 150000, 1)
+    xadj, adj = o.csr(src, dst, 100000)
+    parent = o.bfs_forest(xadj, adj)
+    roots = int(np.count_nonzero(parent == np.arange(100000, dtype=np.uint64)))
+    assert rep["stats"]["input_distribution"]["total_size"] == 2 * 100000 - roots
     # the submission suites' rgg2d input with eulertour processing
     r, rep = run(tmp_path, "--algorithm", "SparseRulingSet", "--input", "rgg2d", "--n", "65536", "--m", "262144",
                  "--input-processing", "eulertour", "--statistics", "basic")
     assert r.returncode == 0 and "Rank array matches reference." in r.stderr
-    assert rep["stats"]["input_distribution"]["total_size"] == 2 * 65536
+    src, dst = o.gen_rgg2d(65536, 262144, 1)
+    xadj, adj = o.csr(src, dst, 65536)
+    parent = o.bfs_forest(xadj, adj)
+    roots = int(np.count_nonzero(parent == np.arange(65536, dtype=np.uint64)))
+    assert rep["stats"]["input_distribution"]["total_size"] == 2 * 65536 - roots

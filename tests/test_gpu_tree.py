@@ -76,32 +76,33 @@ def test_in_place_and_host(ctx):
     assert (hs == er).all() and (hr == ed).all()
 
 
-def test_tour_structure(ctx):
-    """compute_euler_tour: one path per tree ending at the root's dummy, 2(n-1)
-    edges, weights summing to 0 over each tour, unused slots isolated."""
-    say("tour structure")
-    parent = relabel_forest(9)
+@pytest.mark.parametrize("seed", [9, 10])
+def test_tour_structure(ctx, seed):
+    """compute_euler_tour returns the reference's tour element for element
+    (the oracle's restatement of compute_euler_tour on one PE), and ranking
+    it gives every vertex's depth at its upward edge."""
+    say("tour structure", seed)
+    parent = relabel_forest(seed)
     n = len(parent)
     ts, tw = ctx.compute_euler_tour(dev(parent))
     ts, tw = host(ts), host(tw)
-    roots = np.nonzero(parent == np.arange(n))[0]
-    # ranking the tour with the oracle gives the same depths as the tree:
-    # element 2v is v's upward edge, whose tail is 2 x v's root
-    r, k = o.sequential_walk(ts, tw)
-    er, ed = o.euler_tour_rooting(parent)
-    assert (r[0::2] == 2 * er).all() and (k[0::2] == ed).all()
-    # tour heads: the first down edge of each root with children has rank 0
-    indeg = np.bincount(ts[ts != np.arange(2 * n)].astype(np.int64), minlength=2 * n)
-    assert indeg.max() <= 1                                  # a list
-    for rt in roots:
-        assert ts[2 * rt] == 2 * rt and tw[2 * rt] == 0      # dummy self loop
-        assert ts[2 * rt + 1] == 2 * rt + 1                  # unused slot
-    nonroot = parent != np.arange(n)
-    assert (tw[0::2][nonroot] == 1).all() and (tw[1::2][nonroot] == -1).all()
-    heads = np.nonzero((indeg == 0) & (ts != np.arange(2 * n)))[0]
-    assert (k[heads] == 0).all()                             # sum of weights per tour is 0
-    assert np.count_nonzero(tw == 1) == n - len(roots)
-    assert np.count_nonzero(tw == -1) == n - len(roots)
+    es, ew = o.compute_euler_tour(parent)
+    roots = np.count_nonzero(parent == np.arange(n))
+    assert len(ts) == 2 * (n - roots) + roots
+    assert (ts == es).all() and (tw == ew).all()
+    # ranked as a list on the GPU, the tour gives the sequential walk's ranks
+    r, k = ctx.rank_list(dev(ts), dev(tw))
+    cr, ck = o.sequential_walk(es, ew)
+    assert (host(r) == cr).all() and (host(k) == ck).all()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 70000])
+def test_tour_random_trees(ctx, n):
+    say("tour random tree", n)
+    parent = o.gen_tree(n, 31 + n)
+    ts, tw = ctx.compute_euler_tour(dev(parent))
+    es, ew = o.compute_euler_tour(parent)
+    assert (host(ts) == es).all() and (host(tw) == ew).all()
 
 
 def test_tree_errors(ctx):

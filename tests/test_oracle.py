@@ -490,3 +490,91 @@ def test_rgg2d_edge_count_near_target():
     src, _ = o.gen_rgg2d(n, m, 3)
     # pairs near the border lose part of their disc: expect a little under m
     assert 0.85 * m < len(src) <= 1.02 * m
+
+
+# compute_euler_tour (eulertour.cpp:379-392) on one PE, worked by hand from
+# reverse_rooted_tree (edges (i, p), (p, i) per non-root i, in order of i),
+# make_graph (counting sort by source, each bucket filled back to front) and
+# compute_euler_tour_impl (succ[(u_prev, v)] = (v, u), element ids in CSR
+# order, the wrap at a root to its dummy):
+#   parent [0, 0, 1]: CSR 0:[1] 1:[2, 0] 2:[1]; elements (0,1) (1,2) (1,0)
+#   (2,1), dummy of 0 at 4; tour (0,1)->(1,2)->(2,1)->(1,0)->dummy.
+#   parent [1, 1, 1, 0]: CSR 0:[3, 1] 1:[2, 0] 2:[1] 3:[0] (children above
+#   the vertex, its parent, children below it); elements (0,3) (0,1) (1,2)
+#   (1,0) (2,1) (3,0), dummy of 1 at 6; tour (1,2)->(2,1)->(1,0)->(0,3)->
+#   (3,0)->(0,1)->dummy.
+#   parent [0, 0, 2]: vertex 2 is a childless root: its slot comes after
+#   the dummy of 0.
+TOUR_CASES = [
+    ([0, 0, 1], [1, 3, 4, 2, 4], [-1, -1, 1, 1, 0]),
+    ([1, 1, 1, 0], [5, 6, 4, 0, 3, 1, 6], [-1, 1, -1, -1, 1, 1, 0]),
+    ([0, 0, 2], [1, 2, 2, 3], [-1, 1, 0, 0]),
+    ([0, 1], [0, 1], [0, 0]),
+    ([0], [0], [0]),
+]
+
+
+@pytest.mark.parametrize("parent,succ,w", TOUR_CASES)
+def test_compute_euler_tour_hand_cases(parent, succ, w):
+    ts, tw = o.compute_euler_tour(np.array(parent, np.uint64))
+    assert ts.tolist() == succ and tw.tolist() == w
+
+
+def tour_order_brute(parent):
+    """The CSR neighbour order by the rule make_graph produces: for vertex v,
+    children above v in descending order, then its parent, then children
+    below v in descending order."""
+    n = len(parent)
+    kids = [[] for _ in range(n)]
+    for c in range(n):
+        if parent[c] != c:
+            kids[int(parent[c])].append(c)
+    nb = []
+    for v in range(n):
+        above = sorted([c for c in kids[v] if c > v], reverse=True)
+        below = sorted([c for c in kids[v] if c < v], reverse=True)
+        nb.append(above + ([int(parent[v])] if parent[v] != v else []) + below)
+    return nb
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_compute_euler_tour_invariants(seed):
+    """On random forests: the element order is the CSR rule above (edge ids
+    by source, then neighbour position); each tree's tour visits every
+    element of the tree once and ends at its root's dummy; ranking the tour
+    with its weights gives every vertex's depth at its upward edge."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 400))
+    parent = np.arange(n, dtype=np.uint64)
+    for v in range(n):
+        if rng.random() < 0.9 and v > 0:
+            parent[v] = rng.integers(0, v)
+    sigma = rng.permutation(n)
+    relab = np.empty(n, np.uint64)
+    relab[sigma] = sigma[parent]
+    ts, tw = o.compute_euler_tour(relab)
+    nb = tour_order_brute(relab)
+    roots = [v for v in range(n) if relab[v] == v]
+    E = sum(len(x) for x in nb)
+    assert len(ts) == E + len(roots) and E == 2 * (n - len(roots))
+    # edge ids in CSR order; succ by the cyclic rule with the root wrap
+    eid, off = {}, [0] * (n + 1)
+    for v in range(n):
+        off[v + 1] = off[v] + len(nb[v])
+        for i, u in enumerate(nb[v]):
+            eid[(v, u)] = off[v] + i
+    with_kids = [v for v in roots if nb[v]]
+    dummy = {v: E + k for k, v in enumerate(with_kids)}
+    for (x, y), e in eid.items():
+        i = (nb[y].index(x) + 1) % len(nb[y])
+        exp = dummy[y] if (relab[y] == y and i == 0) else eid[(y, nb[y][i])]
+        assert ts[e] == exp
+        assert tw[e] == (1 if relab[x] == y else -1)
+    assert (ts[E:] == np.arange(E, E + len(roots))).all() and (tw[E:] == 0).all()
+    # ranking the tour: rank of (v, parent v) is depth(v)
+    r, k = o.sequential_walk(ts, tw)
+    br, bd = bfs_depth(relab)
+    for v in range(n):
+        if relab[v] != v:
+            e = eid[(v, int(relab[v]))]
+            assert k[e] == bd[v] and r[e] == dummy[int(br[v])]
