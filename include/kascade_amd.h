@@ -89,7 +89,23 @@ typedef struct kc_rank_cfg {
   uint32_t chase_blocks_per_cu; /* resident 256-lane chase workgroups per CU, 0 = auto */
   uint32_t chase_chunk;     /* rulers a wavefront takes per work-counter atomic, 0 = auto */
   uint32_t launch;          /* kc_launch */
+  uint32_t spawn;           /* kc_spawn */
+  uint32_t reserved[3];     /* zero */
 } kc_rank_cfg;
+
+/* Ruler spawn in the sparse ruling set's chase (kc_rank_cfg.spawn), the
+ * counterpart of the reference's `spawn` (sparse_ruling_set.cpp:186-231,
+ * :250-256): once most chases have ended, the remaining ones are suspended,
+ * new rulers are started at unreached elements (candidates at a hashed place
+ * in every block of ids, finer than the rulers) and everything resumes, so
+ * the longest remaining gaps are split instead of walked to the end by a few
+ * lanes.  List regions only (Euler tours start a chase at every tour head
+ * instead).  Results are identical either way. */
+typedef enum kc_spawn {
+  KC_SPAWN_AUTO = 0, /* on for list levels of at least 2^20 elements and 256 or more per ruler */
+  KC_SPAWN_OFF = 1,
+  KC_SPAWN_ON = 2    /* on every list level with more than 8 elements per ruler */
+} kc_spawn;
 
 /* ---- context ------------------------------------------------------------ */
 kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void **ctx_out);
@@ -143,6 +159,7 @@ typedef struct kc_trace_level {
   uint64_t chain_length_min;
   uint64_t chain_length_max;
   double chain_length_avg;
+  uint64_t spawned;   /* rulers spawned during the chase (spawned_rulers) */
 } kc_trace_level;
 kc_status kc_trace_enable(void *ctx, int on);
 uint32_t kc_trace_read(void *ctx, kc_trace_level *out, uint32_t max_entries);

@@ -32,7 +32,8 @@ class RankCfg(ctypes.Structure):
     _fields_ = [("algorithm", ctypes.c_int32), ("ruler_spacing", ctypes.c_uint32),
                 ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
                 ("validate", ctypes.c_uint32), ("chase_blocks_per_cu", ctypes.c_uint32),
-                ("chase_chunk", ctypes.c_uint32), ("launch", ctypes.c_uint32)]
+                ("chase_chunk", ctypes.c_uint32), ("launch", ctypes.c_uint32), ("spawn", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
 
 
 class TreeCfg(ctypes.Structure):
@@ -60,7 +61,7 @@ class TraceLevel(ctypes.Structure):
                 ("rounds", ctypes.c_uint32), ("n", ctypes.c_uint64), ("spacing", ctypes.c_uint64),
                 ("rulers", ctypes.c_uint64), ("tail_rulers", ctypes.c_uint64), ("unreached", ctypes.c_uint64),
                 ("chain_length_min", ctypes.c_uint64), ("chain_length_max", ctypes.c_uint64),
-                ("chain_length_avg", ctypes.c_double)]
+                ("chain_length_avg", ctypes.c_double), ("spawned", ctypes.c_uint64)]
 
 
 TRACE_KINDS = ("srs", "pd", "small")
@@ -214,9 +215,19 @@ def _st(stream):
         return None
     cur = torch.cuda.current_stream()
     if cur.cuda_stream == 0:
-        cur.synchronize()
+        _wait_for(cur)
         return None
     return cur.cuda_stream
+
+
+def _wait_for(stream):
+    """Waits on the host for the work queued so far on a torch stream, and no
+    other: synchronising HIP's null stream itself may also wait for the work
+    of every other stream on the device (the server's, for one)."""
+    import torch
+    ev = torch.cuda.Event()
+    ev.record(stream)
+    ev.synchronize()
 
 
 def _numel(a):
@@ -227,9 +238,13 @@ LAUNCHES = {"auto": 0, "device": 1, "host": 2}
 JOB_KINDS = {"list": 0, "tree": 1}  # kc_job_kind
 
 
+SPAWNS = {"auto": 0, "off": 1, "on": 2}  # kc_spawn
+
+
 def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
-         chase_blocks_per_cu=0, chase_chunk=0, launch="auto"):
+         chase_blocks_per_cu=0, chase_chunk=0, launch="auto", spawn="auto"):
     c = RankCfg()
+    c.spawn = SPAWNS[spawn] if isinstance(spawn, str) else int(spawn)
     c.launch = LAUNCHES[launch] if isinstance(launch, str) else int(launch)
     c.chase_blocks_per_cu = chase_blocks_per_cu
     c.chase_chunk = chase_chunk
@@ -641,7 +656,7 @@ class Server:
         # be complete before the job is handed over
         if any(getattr(a, "is_cuda", False) for a in (succ, weight, root_out, rank_out)):
             import torch
-            torch.cuda.current_stream().synchronize()
+            _wait_for(torch.cuda.current_stream())
         j = Job()
         j.kind = JOB_KINDS[kind] if isinstance(kind, str) else int(kind)
         j.n = n

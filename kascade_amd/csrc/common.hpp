@@ -229,6 +229,7 @@ struct RankTuning {
   bool spacing_top_fixed;
   bool spacing_deep_fixed;
   uint32_t launch;  // kc_launch
+  uint32_t spawn;   // kc_spawn
 };
 // Per-size-class tuning table row: log2(n) class and the knobs picked for it
 // (written by kascade_amd/autotune.py into tuning_table.txt, which contexts

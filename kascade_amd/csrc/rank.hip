@@ -37,7 +37,13 @@ enum Slot : int {
   kSlotChainMax = 5,
   kSlotChainMinC = 6,
   kSlotChainSum = 7,
-  kSlotCount = 8,
+  // ruler spawn: resumed chases handed out, idle lanes, spawns, continuations, stop flag
+  kSlotSpawnWork = 8,
+  kSlotSpawnIdle = 9,
+  kSlotSpawned = 10,
+  kSlotSpawnCont = 11,
+  kSlotSpawnStop = 12,
+  kSlotCount = 13,
 };
 
 // Writes element e's final (root, rank): as a segment-local u32 root (root_id
@@ -115,14 +121,51 @@ pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, ui
 // sublist link (ruler -> next ruler, gap) is the reversed-orientation message
 // of work_on_item :242-251, written directly in sublist ids.
 // ---------------------------------------------------------------------------
+// Ruler spawn (sparse_ruling_set.cpp:186-231, :250-256) in three launches.
+// The reference starts a new ruler at an unreached element whenever a chase
+// ends, so its last long gaps are split while chases still run.  Here:
+//  1. the chase (kChaseSuspend) runs as usual until fewer than a set number
+//     of lanes still chase (the lanes report in quanta once the ruler pool is
+//     empty); then every wave writes the state of its unfinished chases (and
+//     rulers it took but did not start) into a continuation list and exits;
+//  2. spawn_mark_kernel probes candidates, one per block of 2^logC ids at a
+//     hashed offset (computed, like the rulers): an unreached candidate c
+//     becomes a new ruler j (a fresh sublist id, as a reached tail gets),
+//     its record {j | root flag | aux flag, 0}, and its chase is appended to
+//     the continuation list;
+//  3. the chase (kChaseResume) runs the continuation list; a chase that
+//     reaches a spawned ruler links to it, as to a computed one.
+// The launch boundaries order every claim before the next pass, so no lane
+// ever waits on another and no access needs to be atomic.
+struct Cont {
+  uint64_t ruler;  // sublist id of the chase's ruler (bit 63: its first step is still to do)
+  uint64_t cur;    // next element to look at
+  int64_t dist;    // distance from the ruler to cur
+  uint64_t pad;
+};
+constexpr uint64_t kContFirst = uint64_t{1} << 63;
+constexpr uint64_t kSpawnBits = kRootFlag | kAuxFlag;
+
+struct SpawnArgs {
+  unsigned long long* idle_ctr = nullptr;  // lanes idle after the ruler pool (in quanta)
+  uint64_t idle_goal = 0;                  // suspend once this many are idle
+  unsigned long long* stop = nullptr;      // set when the goal is reached
+  Cont* cont = nullptr;                    // continuation list
+  unsigned long long* cont_ctr = nullptr;  // its length
+  uint64_t cont_cap = 0;
+};
+
+enum ChaseMode : int { kChasePlain = 0, kChaseSuspend = 1, kChaseResume = 2 };
+constexpr int kIdleQuantum = 16;  // lanes per idle report
+
 // kTrace: also count chain lengths (the per-level trace); off in plain calls.
-template <bool kTrace>
+template <bool kTrace, int kMode>
 __global__ void __launch_bounds__(kThreads)
 chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
              const uint64_t* __restrict__ forced, uint64_t nF, uint64_t tour_lo,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
-             uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev) {
+             uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev, SpawnArgs sp) {
   if (*err) return;  // an earlier kernel found invalid input: stop, as the host-sized path does
   const uint64_t lt = lanemask_lt();
   if (n_dev) {  // device-sized level: n is a capacity, the size is in memory
@@ -131,6 +174,12 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
   }
   // rulers [0, nR) sit at computed places; [nR, nRF) start at the tour heads
   const uint64_t nRF = nR + nF;
+  // work items: rulers, or (resume) continuations
+  uint64_t nwork = nRF;
+  if constexpr (kMode == kChaseResume) {
+    nwork = *sp.cont_ctr;
+    if (nwork > sp.cont_cap) nwork = sp.cont_cap;
+  }
   // downward tour edges carry no claim (their results are never read)
   auto claims = [&](uint64_t e) { return e < tour_lo || ((e - tour_lo) & 1) == 0; };
   // chain lengths of this lane (trace statistics, one reduction per wave at exit)
@@ -142,14 +191,31 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       len_sum += steps;
     }
   };
-  // Wave-uniform chunk of ruler ids [cbase, cend): one atomic per `chunk`
-  // rulers, so idle lanes refill every step without touching the counter.
+  // Wave-uniform chunk of work ids [cbase, cend): one atomic per `chunk`
+  // items, so idle lanes refill every step without touching the counter.
   uint64_t cbase = 0, cend = 0;
   bool wave_done = false;
   bool active = false;
   bool first = false;
   uint64_t ruler = 0, cur = 0;
   int64_t dist = 0;
+  uint32_t idle_reported = 0;  // (suspend) lanes of this wave reported idle
+  uint32_t iter = 0;
+  auto flush_trace = [&]() {
+    if constexpr (kTrace) {
+      for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t a = __shfl_xor(len_max, o), b = __shfl_xor(len_minc, o);
+        len_max = a > len_max ? a : len_max;
+        len_minc = b > len_minc ? b : len_minc;
+        len_sum += __shfl_xor(len_sum, o);
+      }
+      if (__lane_id() == 0 && len_sum) {
+        atomicMax(&chain[0], static_cast<unsigned long long>(len_max));
+        atomicMax(&chain[1], static_cast<unsigned long long>(len_minc));
+        atomicAdd(&chain[2], static_cast<unsigned long long>(len_sum));
+      }
+    }
+  };
   for (;;) {
     uint64_t need = __ballot(!active);
     if (need != 0 && !wave_done) {
@@ -157,41 +223,78 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
         unsigned long long b = 0;
         if (__lane_id() == 0) b = atomicAdd(work_ctr, static_cast<unsigned long long>(chunk));
         b = __shfl(b, 0);
-        if (b >= nRF) {
+        if (b >= nwork) {
           wave_done = true;
         } else {
           cbase = b;
-          cend = b + chunk < nRF ? b + chunk : nRF;
+          cend = b + chunk < nwork ? b + chunk : nwork;
         }
       }
       if (!wave_done) {
         const uint64_t avail = cend - cbase;
         const uint64_t r = __popcll(need & lt);
         if (!active && r < avail) {
-          ruler = cbase + r;
-          cur = ruler < nR ? ruler_of_block(ruler, n, logK, seed) : forced[ruler - nR];
-          first = true;
+          if constexpr (kMode == kChaseResume) {
+            const Cont c = sp.cont[cbase + r];
+            ruler = c.ruler & ~kContFirst;
+            first = (c.ruler & kContFirst) != 0;
+            cur = c.cur;
+            dist = c.dist;
+            if constexpr (kTrace) steps = first ? 0 : 1;
+          } else {
+            ruler = cbase + r;
+            cur = ruler < nR ? ruler_of_block(ruler, n, logK, seed) : forced[ruler - nR];
+            first = true;
+          }
           active = true;
         }
         const uint64_t cnt = __popcll(need);
         cbase += cnt < avail ? cnt : avail;
       }
     }
+    if constexpr (kMode == kChaseSuspend) {
+      if (wave_done) {
+        // report idle lanes in quanta (a handful of atomics per wave)
+        const uint32_t idle = static_cast<uint32_t>(__popcll(__ballot(!active)));
+        const uint32_t q = idle / kIdleQuantum * kIdleQuantum;
+        if (q > idle_reported) {
+          if (__lane_id() == 0) atomicAdd(sp.idle_ctr, static_cast<unsigned long long>(q - idle_reported));
+          idle_reported = q;
+        }
+      }
+      if ((iter & 31) == 0) {
+        unsigned long long stop = 0;
+        if (__lane_id() == 0) {
+          stop = __hip_atomic_load(sp.stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (!stop && wave_done &&
+              __hip_atomic_load(sp.idle_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= sp.idle_goal) {
+            stop = 1;
+            __hip_atomic_store(sp.stop, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        if (__shfl(stop, 0) != 0) {
+          // hand over the unfinished chases and the rulers taken but not started
+          const uint64_t k = wave_append(sp.cont_ctr, active);
+          if (active && k < sp.cont_cap)
+            sp.cont[k] = Cont{ruler | (first ? kContFirst : 0), cur, dist, 0};
+          for (uint64_t b = cbase; b < cend; b += 64) {
+            const uint64_t id = b + __lane_id();
+            const bool mine = id < cend;
+            const uint64_t k2 = wave_append(sp.cont_ctr, mine);
+            if (mine && k2 < sp.cont_cap) {
+              const uint64_t at = id < nR ? ruler_of_block(id, n, logK, seed) : forced[id - nR];
+              sp.cont[k2] = Cont{id | kContFirst, at, 0, 0};
+            }
+          }
+          flush_trace();
+          break;
+        }
+      }
+      iter++;
+    }
     if (__ballot(active) == 0) {
       if (wave_done) {
-        if constexpr (kTrace) {
-        for (int o = 32; o > 0; o >>= 1) {
-          const uint64_t a = __shfl_xor(len_max, o), b = __shfl_xor(len_minc, o);
-          len_max = a > len_max ? a : len_max;
-          len_minc = b > len_minc ? b : len_minc;
-          len_sum += __shfl_xor(len_sum, o);
-        }
-        if (__lane_id() == 0 && len_sum) {
-          atomicMax(&chain[0], static_cast<unsigned long long>(len_max));
-          atomicMax(&chain[1], static_cast<unsigned long long>(len_minc));
-          atomicAdd(&chain[2], static_cast<unsigned long long>(len_sum));
-        }
-        }
+        flush_trace();
         break;
       }
       continue;
@@ -230,7 +333,15 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       continue;
     }
     const Rec r = rec[cur];
-    if (r.s & (kRootFlag | kAuxFlag)) {
+    if (r.s & kSpawnBits) {
+      if (kMode == kChaseResume && (r.s & kSpawnBits) == kSpawnBits) {
+        const uint64_t id = r.s & kIdMask;
+        if (id == ruler) raise_err(err, kErrCycle);  // came back to its own spawned ruler
+        sub[ruler] = Rec{id, dist};  // reached a spawned ruler
+        active = false;
+        chain_end();
+        continue;
+      }
       raise_err(err, kErrNotList);  // reached twice: in-degree > 1
       active = false;
     } else if (r.s == cur) {
@@ -252,6 +363,46 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       if constexpr (kTrace) steps++;
     }
   }
+}
+
+// Step 2 of the spawn: one thread per candidate block.  An unreached
+// candidate (not a computed ruler, record unclaimed) becomes spawned ruler j
+// while spawn slots last; its chase starts at its successor.
+__global__ void __launch_bounds__(kThreads)
+spawn_mark_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, uint64_t nF,
+                  uint32_t logC, uint32_t cseed, uint64_t cap, uint64_t budget, Rec* __restrict__ sub,
+                  uint64_t* __restrict__ sub_orig, unsigned long long* extra_ctr, unsigned long long* spawn_ctr,
+                  SpawnArgs sp, const uint64_t* __restrict__ n_dev, const uint32_t* __restrict__ err) {
+  if (*err) return;
+  if (n_dev) {
+    n = *n_dev < n ? *n_dev : n;
+    nR = (n + (uint64_t{1} << logK) - 1) >> logK;
+  }
+  const uint64_t nC = (n + (uint64_t{1} << logC) - 1) >> logC;
+  const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  bool want = false;
+  uint64_t c = 0;
+  Rec r{0, 0};
+  if (b < nC) {
+    c = ruler_of_block(b, n, logC, cseed);
+    if (!is_ruler(c, n, logK, seed)) {
+      r = rec[c];
+      want = !(r.s & kSpawnBits);
+    }
+  }
+  const uint64_t slot = wave_append(spawn_ctr, want);
+  const bool go = want && slot < budget;
+  const uint64_t j = nR + nF + wave_append(extra_ctr, go);
+  if (!go) return;
+  if (j >= cap) return;  // cannot happen: cap holds every tail plus the budget
+  rec[c] = Rec{j | kSpawnBits, 0};
+  sub_orig[j] = c;
+  if (r.s == c) {
+    sub[j] = Rec{j, 0};  // a tail: the spawned ruler ends its list
+    return;
+  }
+  const uint64_t k = wave_append(sp.cont_ctr, true);
+  if (k < sp.cont_cap) sp.cont[k] = Cont{j, r.s, r.w, 0};
 }
 
 // T[i].s (sublist root id) -> this level's id; T8[i] = T[i].w; mixed_roots
@@ -765,6 +916,83 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
   return spacing;
 }
 
+// Ruler spawn for a level: candidates one per block of K / 8 ids (at least
+// 2), the chase suspended once fewer than lanes / 16 lanes still chase, at
+// most one spawn per two rulers (`budget`: the spawned rulers' share of the
+// sublist capacity).  Automatic for list levels (no tours) of at least 2^20
+// elements with K >= 256: the tail it cuts grows with K, and below that its
+// two extra launches cost more than they save (2^27 elements, 6 runs each:
+// K = 128 chase 7.15 ms without spawn, 7.37-7.50 with it; K = 256 7.96
+// without, 7.49-7.54 with it, whose 4 MB rank table then makes propagation
+// 0.45 ms shorter than at K = 128).  KC_SPAWN_ON forces it on any list level
+// whose K exceeds 8.
+constexpr uint32_t kSpawnShift = 3;    // candidates per ruler: 2^kSpawnShift
+constexpr uint64_t kSpawnActive = 16;  // suspend when fewer than lanes / this still chase
+struct SpawnPlan {
+  bool on = false;
+  uint32_t logC = 0;
+  uint64_t budget = 0;
+  uint64_t idle_goal = 0;
+  uint64_t cont_cap = 0;
+};
+SpawnPlan spawn_plan(const RankTuning& tune, uint64_t n, uint32_t logK, bool has_tours, unsigned chase_blocks) {
+  SpawnPlan p;
+  if (has_tours || tune.spawn == KC_SPAWN_OFF || logK <= kSpawnShift) return p;
+  if (tune.spawn != KC_SPAWN_ON && (n < (uint64_t{1} << 20) || logK < 8)) return p;
+  p.on = true;
+  p.logC = logK - kSpawnShift;
+  p.budget = std::max<uint64_t>(((n + (uint64_t{1} << logK) - 1) >> logK) / 2, 64);
+  const uint64_t lanes = uint64_t{chase_blocks} * kThreads;
+  p.idle_goal = lanes - lanes / kSpawnActive;
+  // every lane's chase, the rulers a wave took but did not start, the spawns
+  p.cont_cap = lanes + uint64_t{chase_blocks} * (kThreads / 64) * tune.chase_chunk + p.budget;
+  return p;
+}
+
+// Counter slots of one level's chase.
+struct ChaseSlots {
+  unsigned long long* work;
+  unsigned long long* extra;
+  unsigned long long* chain;
+  unsigned long long* resume_work;
+  unsigned long long* idle;
+  unsigned long long* spawned;
+  unsigned long long* cont_ctr;
+  unsigned long long* stop;
+};
+
+// The level's chase: one launch, or with spawn the suspend, mark and resume
+// launches.  The caller zeroed the counters.
+void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trace, unsigned chase_blocks, Rec* rec,
+                  uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, const uint64_t* forced, uint64_t nF,
+                  uint64_t tour_lo, uint64_t cap, Rec* sub, uint64_t* sub_orig, const ChaseSlots& cs,
+                  uint32_t chunk, const uint64_t* n_dev, int level, hipStream_t st) {
+  Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
+  SpawnArgs sa;
+  if (!sp.on) {
+    auto kern = trace ? chase_kernel<true, kChasePlain> : chase_kernel<false, kChasePlain>;
+    hipLaunchKernelGGL(kern, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF,
+                       tour_lo, cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+    return;
+  }
+  sa.idle_ctr = cs.idle;
+  sa.idle_goal = sp.idle_goal;
+  sa.stop = cs.stop;
+  sa.cont = scratch.get<Cont>(sp.cont_cap);
+  sa.cont_ctr = cs.cont_ctr;
+  sa.cont_cap = sp.cont_cap;
+  auto k1 = trace ? chase_kernel<true, kChaseSuspend> : chase_kernel<false, kChaseSuspend>;
+  hipLaunchKernelGGL(k1, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
+                     cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+  const uint64_t nC = (n + (uint64_t{1} << sp.logC) - 1) >> sp.logC;
+  hipLaunchKernelGGL(spawn_mark_kernel, dim3(blocks_for(nC, kThreads)), dim3(kThreads), 0, st, rec, n, logK, seed,
+                     nR, nF, sp.logC, seed ^ 0x5bd1e995u, cap, sp.budget, sub, sub_orig, cs.extra, cs.spawned, sa,
+                     n_dev, ctx->d_err);
+  auto k3 = trace ? chase_kernel<true, kChaseResume> : chase_kernel<false, kChaseResume>;
+  hipLaunchKernelGGL(k3, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
+                     cap, sub, sub_orig, cs.resume_work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+}
+
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune,
                int level, hipStream_t st, const TourRegion* tours, const JobOut& jo) {
@@ -775,7 +1003,12 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   const uint32_t spacing = level_spacing(tune, level, n, max_tails);
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
-  const uint64_t cap = nR + nF + std::min<uint64_t>(max_tails, n);
+  const uint64_t lanes_wanted = nR + nF;
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  const unsigned chase_blocks = static_cast<unsigned>(
+      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
+  const SpawnPlan sp = spawn_plan(tune, n, logK, tours != nullptr, chase_blocks);
+  const uint64_t cap = nR + nF + std::min<uint64_t>(max_tails, n) + sp.budget;
   Rec* sub = scratch.get<Rec>(cap);
   Rec* T = scratch.get<Rec>(cap);
   uint64_t* sub_orig = scratch.get<uint64_t>(cap);
@@ -783,20 +1016,13 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
 
   zero_counters(ctx, st);
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
-  const uint64_t lanes_wanted = nR + nF;
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
-  const unsigned chase_blocks = static_cast<unsigned>(
-      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
-  {
-    Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(ctx->trace_on ? chase_kernel<true> : chase_kernel<false>, dim3(chase_blocks),
-                       dim3(kThreads), 0, st, rec, n, logK, seed,
-                       nR, tours ? tours->heads : nullptr, nF, tour_lo, cap, sub, sub_orig,
-                       ctr + kSlotWork, ctr + kSlotExtra, ctx->d_err,
-                       tune.chase_chunk, ctr + kSlotChainMax, nullptr);
-  }
+  const ChaseSlots cs{ctr + kSlotWork,      ctr + kSlotExtra,   ctr + kSlotChainMax, ctr + kSlotSpawnWork,
+                      ctr + kSlotSpawnIdle, ctr + kSlotSpawned, ctr + kSlotSpawnCont, ctr + kSlotSpawnStop};
+  launch_chase(ctx, scratch, sp, ctx->trace_on, chase_blocks, rec, n, logK, seed, nR, tours ? tours->heads : nullptr,
+               nF, tour_lo, cap, sub, sub_orig, cs, tune.chase_chunk, nullptr, level, st);
   uint64_t extra = 0;
   read_counters(ctx, st, kSlotExtra, 1, &extra);  // also raises device errors
+  const uint64_t spawned = std::min<uint64_t>(ctx->h_pinned[kSlotSpawned], sp.budget);  // copied by read_counters
   const uint64_t m = nR + nF + extra;
   if (jo.seg_off)
     hipLaunchKernelGGL(seg_local_tails_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, sub, sub_orig,
@@ -811,10 +1037,13 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
     t.n = n;
     t.spacing = spacing;
     t.rulers = nR + nF;
-    t.tail_rulers = extra;
+    t.tail_rulers = extra - spawned;
+    t.spawned = spawned;
     t.chain_length_max = ctx->h_pinned[kSlotChainMax];
     t.chain_length_min = nR + nF ? ~ctx->h_pinned[kSlotChainMinC] : 0;
-    t.chain_length_avg = nR + nF ? static_cast<double>(ctx->h_pinned[kSlotChainSum]) / static_cast<double>(nR + nF) : 0;
+    t.chain_length_avg = nR + nF + spawned
+                             ? static_cast<double>(ctx->h_pinned[kSlotChainSum]) / static_cast<double>(nR + nF + spawned)
+                             : 0;
     trace_at = ctx->trace.size();
     ctx->trace.push_back(t);
   }
@@ -923,7 +1152,10 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 // sequence with no host round trip, which a HIP graph can capture.  Each
 // level zeroes its own counters with a kernel (no memset nodes).
 // ---------------------------------------------------------------------------
-enum DsSlot : int { kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsChain = 4, kDsSlots = 8 };
+enum DsSlot : int {
+  kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsChain = 4,  // chain: 3 slots
+  kDsSpawnWork = 7, kDsSpawnIdle = 8, kDsSpawned = 9, kDsSpawnCont = 10, kDsSpawnStop = 11, kDsSlots = 12
+};
 
 void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
              int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
@@ -961,7 +1193,11 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t lists = std::min<uint64_t>(max_tails, n);
-  const uint64_t cap = nR + nF + lists;
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  const unsigned chase_blocks = static_cast<unsigned>(
+      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (nR + nF + kThreads - 1) / kThreads)));
+  const SpawnPlan sp = spawn_plan(tune, n, logK, tours != nullptr, chase_blocks);
+  const uint64_t cap = nR + nF + lists + sp.budget;
   Rec* sub = scratch.get<Rec>(cap);
   Rec* T = scratch.get<Rec>(cap);
   uint64_t* sub_orig = scratch.get<uint64_t>(cap);
@@ -971,16 +1207,10 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   unsigned long long* ctr = scratch.get<unsigned long long>(kDsSlots);
   const uint32_t seed = static_cast<uint32_t>(splitmix64(0xC0FFEEull + level * 0x1234567ull + n));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, ctr, kDsSlots);
-
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
-  const unsigned chase_blocks = static_cast<unsigned>(
-      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (nR + nF + kThreads - 1) / kThreads)));
-  {
-    Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(chase_kernel<false>, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR,
-                       tours ? tours->heads : nullptr, nF, tour_lo, cap, sub, sub_orig, ctr + kDsWork,
-                       ctr + kDsExtra, ctx->d_err, tune.chase_chunk, ctr + kDsChain, n_dev);
-  }
+  const ChaseSlots cs{ctr + kDsWork,      ctr + kDsExtra,   ctr + kDsChain,     ctr + kDsSpawnWork,
+                      ctr + kDsSpawnIdle, ctr + kDsSpawned, ctr + kDsSpawnCont, ctr + kDsSpawnStop};
+  launch_chase(ctx, scratch, sp, false, chase_blocks, rec, n, logK, seed, nR, tours ? tours->heads : nullptr, nF,
+               tour_lo, cap, sub, sub_orig, cs, tune.chase_chunk, n_dev, level, st);
   uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
   hipLaunchKernelGGL(level_size_kernel, dim3(1), dim3(64), 0, st, n_dev, n, logK, nF, ctr + kDsExtra, m_dev);
   if (jo.seg_off)
@@ -1103,6 +1333,8 @@ RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bo
     if (cfg->max_levels) t.max_levels = cfg->max_levels;
     if (cfg->launch > KC_LAUNCH_HOST) throw KcError{KC_EINVAL, "invalid launch mode"};
     t.launch = cfg->launch;
+    if (cfg->spawn > KC_SPAWN_ON) throw KcError{KC_EINVAL, "invalid spawn mode"};
+    t.spawn = cfg->spawn;
   }
   return t;
 }
@@ -1126,7 +1358,7 @@ std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_
   std::vector<uint64_t> k = {reinterpret_cast<uint64_t>(rec), n, n_out, reinterpret_cast<uint64_t>(root),
                              reinterpret_cast<uint64_t>(rank), max_tails, static_cast<uint64_t>(algorithm),
                              out_shift, t.spacing_top, t.spacing_deep, t.max_levels, t.small_n,
-                             t.chase_blocks_per_cu, t.chase_chunk, t.spacing_top_fixed, t.spacing_deep_fixed,
+                             t.chase_blocks_per_cu, t.chase_chunk, t.spacing_top_fixed, t.spacing_deep_fixed, t.spawn,
                              reinterpret_cast<uint64_t>(ctx->d_err), static_cast<uint64_t>(ctx->num_cus)};
   if (tours) {
     k.push_back(tours->lo);
