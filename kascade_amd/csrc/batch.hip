@@ -47,18 +47,44 @@ __device__ __forceinline__ uint64_t chunk_last(uint64_t count) {
   return std::min<uint64_t>(chunk_first() + uint64_t{kPer} * kThreads, count) - 1;
 }
 
+// The job of a thread's current item, its fields kept in registers: a
+// thread's items ascend, so it only moves forward, reloading the job when an
+// item passes the next job's base (jobs are at least one item long).
+struct JobCursor {
+  const BatchJob* jobs;
+  uint32_t njobs;
+  uint32_t j;
+  uint64_t next_base;
+  BatchJob job;
+  __device__ void start(const BatchJob* js, uint32_t nj, uint32_t j0) {
+    jobs = js;
+    njobs = nj;
+    j = j0;
+    job = jobs[j];
+    next_base = j + 1 < njobs ? jobs[j + 1].base : ~uint64_t{0};
+  }
+  __device__ void seek(uint64_t e) {
+    while (e >= next_base) {
+      j++;
+      job = jobs[j];
+      next_base = j + 1 < njobs ? jobs[j + 1].base : ~uint64_t{0};
+    }
+  }
+};
+
 __global__ void __launch_bounds__(kThreads)
 batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nl,
                        Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nl), s_range);
-  uint32_t jl = s_range[0];
+  JobCursor jc;
+  jc.start(jobs, njobs, s_range[0]);
   for (unsigned q = 0; q < kPer; q++) {
     const uint64_t e = chunk_first() + q * kThreads + threadIdx.x;
     bool tail = false;
     if (e < nl) {
-      jl = job_of(jobs, jl, s_range[1], e);
-      const BatchJob j = jobs[jl];
+      jc.seek(e);
+      const BatchJob& j = jc.job;
       const uint64_t i = e - j.base;
       uint64_t s = j.succ[i];
       if (s >= j.n) {
@@ -86,13 +112,14 @@ batch_tree_keys_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
                        unsigned long long* roots) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
-  uint32_t jl = s_range[0];
+  JobCursor jc;
+  jc.start(jobs, njobs, s_range[0]);
   for (unsigned q = 0; q < kPer; q++) {
     const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
     bool root = false;
     if (t < nt) {
-      jl = job_of(jobs, jl, s_range[1], t);
-      const BatchJob j = jobs[jl];
+      jc.seek(t);
+      const BatchJob& j = jc.job;
       const uint64_t i = t - j.base;
       uint64_t p = j.succ[i];
       if (p >= j.n) {
@@ -148,13 +175,14 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
                        unsigned long long* roots) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
-  uint32_t jl = s_range[0];
+  JobCursor jc;
+  jc.start(jobs, njobs, s_range[0]);
   for (unsigned q = 0; q < kPer; q++) {
     const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
     bool root = false;
     if (t < nt) {
-      jl = job_of(jobs, jl, s_range[1], t);
-      const BatchJob j = jobs[jl];
+      jc.seek(t);
+      const BatchJob& j = jc.job;
       const uint64_t i = t - j.base;
       const uint64_t p = j.succ[i];
       if (p >= j.n) raise_err(err, kErrRange);
@@ -172,14 +200,15 @@ batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint6
                         Rec* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
-  uint32_t jl = s_range[0];
+  JobCursor jc;
+  jc.start(jobs, njobs, s_range[0]);
   for (unsigned q = 0; q < kPer; q++) {
     const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
     bool tour_head = false;
     uint32_t h = 0xFFFFFFFFu;
     if (t < nt) {
-      jl = job_of(jobs, jl, s_range[1], t);
-      const BatchJob j = jobs[jl];
+      jc.seek(t);
+      const BatchJob& j = jc.job;
       const uint64_t i = t - j.base;
       const uint64_t p = j.succ[i];
       h = head[t];
@@ -208,11 +237,6 @@ job_block_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t cou
   const uint64_t b = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (b >= (count + 255) / 256) return;
   block_job[b] = job_of(jobs, 0, njobs - 1, b * 256);
-}
-
-__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t value) {
-  uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
-  if (i < n) p[i] = value;
 }
 
 // Copies outputs back: list element e -> its job (shift 0, step 1); tree
@@ -324,8 +348,7 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
     }
     if (nt && linked) {
       Phase ph(ctx, "batch_tour", st);
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks_for(nt, kThreads)), dim3(kThreads), 0, st, head, nt,
-                         0xFFFFFFFFu);
+      KC_HIP(hipMemsetAsync(head, 0xFF, nt * sizeof(uint32_t), st));  // no child yet
       hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
       hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
