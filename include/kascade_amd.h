@@ -90,8 +90,21 @@ typedef struct kc_rank_cfg {
   uint32_t chase_chunk;     /* rulers a wavefront takes per work-counter atomic, 0 = auto */
   uint32_t launch;          /* kc_launch */
   uint32_t spawn;           /* kc_spawn */
-  uint32_t reserved[3];     /* zero */
+  uint32_t local_contraction; /* kc_contract */
+  uint32_t reserved[2];     /* zero */
 } kc_rank_cfg;
+
+/* Local contraction of list rankings (kc_rank_cfg.local_contraction), the
+ * counterpart of the reference's use_local_contraction
+ * (pointer_doubling_generic.hpp:471-543, sparse_ruling_set_detail/
+ * local_contraction.hpp:28-104): runs of consecutive successors
+ * (succ[v] = v + 1) are collapsed into one element each before ranking and
+ * expanded afterwards.  Results are identical either way. */
+typedef enum kc_contract {
+  KC_CONTRACT_AUTO = 0, /* when a sample finds at least half of the elements inside runs */
+  KC_CONTRACT_OFF = 1,
+  KC_CONTRACT_ON = 2
+} kc_contract;
 
 /* Ruler spawn in the sparse ruling set's chase (kc_rank_cfg.spawn), the
  * counterpart of the reference's `spawn` (sparse_ruling_set.cpp:186-231,

@@ -74,7 +74,10 @@ struct RMAPointerChasingConfig {  // :28-31 (RMA doubling = sync doubling here)
 
 enum class GridCommunicatorMode : std::uint8_t { topology_aware, balanced, invalid };  // :33-37
 
-// :39-47.  Every member is an MPI-routing knob: accepted, no effect.
+// :39-47.  use_local_preprocessing = false switches local contraction off
+// (on, the library contracts runs of consecutive successors when a sample of
+// the input finds them); every other member is an MPI-routing knob:
+// accepted, no effect.
 struct PointerDoublingConfig {
   bool use_local_preprocessing = true;
   GridCommunicatorMode grid_communicator_mode = GridCommunicatorMode::topology_aware;
@@ -109,10 +112,13 @@ struct BriefkastenConfig {  // :60-63 (async message queue; no effect)
 //     (or ruler_spacing below), so they have no effect, and neither do
 //     current_sparse_ruling_set_round, round_limit, ruler_propagation_mode
 //     (always pull, the reference's default), sync / sync_locality_aware /
-//     spawn / no_precompute_rulers / post_invert(_detect_leaves) (the chase
-//     engine is this library's own), use_local_contraction, cache_owners,
-//     root_gather_threshold, the aggregation and locality switches, the
-//     Briefkasten and grid settings.
+//     no_precompute_rulers / post_invert(_detect_leaves) (the chase engine
+//     is this library's own), cache_owners, root_gather_threshold, the
+//     aggregation and locality switches, the Briefkasten and grid settings.
+//   spawn and use_local_contraction set true force the GPU's counterparts on
+//     (ruler spawn in the chase; contraction of runs of consecutive
+//     successors); false leaves both to the library.  Results are identical
+//     either way.
 struct SparseRulingSetConfig {
   RulerSelectionStrategy ruler_selection = RulerSelectionStrategy::sanders;
   Algorithm base_algorithm = Algorithm::PointerDoubling;
@@ -243,12 +249,20 @@ inline kc_rank_cfg srs_cfg(SparseRulingSetConfig const& config) {
   if (config.ruler_selection == RulerSelectionStrategy::invalid)
     throw Error(KC_EINVAL, "Invalid ruler selection strategy");  // ruler_selection.hpp:74-75
   const bool gather = config.base_algorithm == Algorithm::GatherChase;
-  if (config.sparse_ruling_set_rounds == 0)  // the base algorithm alone
-    return rank_cfg(gather ? KC_ALGO_SPARSE_RULING_SET : KC_ALGO_POINTER_DOUBLING, config.ruler_spacing,
-                    config.ruler_spacing_deep, 0, config.validate_list);
-  const std::size_t levels = std::min<std::size_t>(config.sparse_ruling_set_rounds, 64);
-  return rank_cfg(KC_ALGO_SPARSE_RULING_SET, config.ruler_spacing, config.ruler_spacing_deep,
-                  gather ? 0u : static_cast<std::uint32_t>(levels), config.validate_list);
+  kc_rank_cfg c{};
+  if (config.sparse_ruling_set_rounds == 0) {  // the base algorithm alone
+    c = rank_cfg(gather ? KC_ALGO_SPARSE_RULING_SET : KC_ALGO_POINTER_DOUBLING, config.ruler_spacing,
+                 config.ruler_spacing_deep, 0, config.validate_list);
+  } else {
+    const std::size_t levels = std::min<std::size_t>(config.sparse_ruling_set_rounds, 64);
+    c = rank_cfg(KC_ALGO_SPARSE_RULING_SET, config.ruler_spacing, config.ruler_spacing_deep,
+                 gather ? 0u : static_cast<std::uint32_t>(levels), config.validate_list);
+  }
+  // Results are identical either way: true forces the GPU's counterpart on,
+  // false leaves the choice to the library (a sample of the input decides).
+  c.local_contraction = config.use_local_contraction ? KC_CONTRACT_ON : KC_CONTRACT_AUTO;
+  c.spawn = config.spawn ? KC_SPAWN_ON : KC_SPAWN_AUTO;
+  return c;
 }
 inline int algorithm_code(Algorithm a) {
   switch (a) {
@@ -286,10 +300,13 @@ auto set_initial_ranking_state(std::span<const idx_t> succ_array, std::span<idx_
 
 // kascade::pointer_doubling (pointer_doubling.hpp:18-23): in place,
 // succ -> root, rank (distance to successor, 0 on roots) -> distance to root.
-inline void pointer_doubling(PointerDoublingConfig /*config*/, std::span<idx_t> succ_array,
+inline void pointer_doubling(PointerDoublingConfig config, std::span<idx_t> succ_array,
                              std::span<rank_t> rank_array, Context& ctx = Context::default_context()) {
   if (rank_array.size() != succ_array.size()) throw Error(KC_EINVAL, "array sizes differ");
   kc_rank_cfg c = detail::rank_cfg(KC_ALGO_POINTER_DOUBLING);
+  // pointer_doubling_generic.hpp:639: the local preprocessing contracts runs
+  // before the doubling; off, nothing is contracted
+  if (!config.use_local_preprocessing) c.local_contraction = KC_CONTRACT_OFF;
   ctx.check(kc_pointer_doubling(ctx.handle(), &c, succ_array.data(), rank_array.data(), succ_array.size(),
                                 nullptr));
 }

@@ -33,7 +33,7 @@ class RankCfg(ctypes.Structure):
                 ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
                 ("validate", ctypes.c_uint32), ("chase_blocks_per_cu", ctypes.c_uint32),
                 ("chase_chunk", ctypes.c_uint32), ("launch", ctypes.c_uint32), ("spawn", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("local_contraction", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class TreeCfg(ctypes.Structure):
@@ -239,11 +239,14 @@ JOB_KINDS = {"list": 0, "tree": 1}  # kc_job_kind
 
 
 SPAWNS = {"auto": 0, "off": 1, "on": 2}  # kc_spawn
+CONTRACTIONS = {"auto": 0, "off": 1, "on": 2}  # kc_contract
 
 
 def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
-         chase_blocks_per_cu=0, chase_chunk=0, launch="auto", spawn="auto"):
+         chase_blocks_per_cu=0, chase_chunk=0, launch="auto", spawn="auto", local_contraction="auto"):
     c = RankCfg()
+    c.local_contraction = (CONTRACTIONS[local_contraction] if isinstance(local_contraction, str)
+                           else int(local_contraction))
     c.spawn = SPAWNS[spawn] if isinstance(spawn, str) else int(spawn)
     c.launch = LAUNCHES[launch] if isinstance(launch, str) else int(launch)
     c.chase_blocks_per_cu = chase_blocks_per_cu

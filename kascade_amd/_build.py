@@ -23,7 +23,7 @@ CFLAGS = [
 ]
 
 SOURCES = ["api.cpp", "rank.hip", "gen.hip", "util.hip", "euler.hip", "segmented.hip", "batch.hip",
-           "server.cpp", "probe.hip", "listutil.hip", "graph.hip", "noalloc.hip"]
+           "server.cpp", "probe.hip", "listutil.hip", "graph.hip", "noalloc.hip", "contract.hip"]
 
 
 def _compile(src):

@@ -522,6 +522,7 @@ kc_status kc_rank_list_noalloc(void* vctx, const uint64_t* succ, const int64_t* 
     kc_rank_cfg c{};
     if (cfg) c = *cfg;
     if (c.launch == KC_LAUNCH_AUTO) c.launch = KC_LAUNCH_DEVICE;  // a graph capture may allocate
+    c.local_contraction = KC_CONTRACT_OFF;  // the scratch bound is for the uncontracted ranking
     // every allocation of the ranking comes from the static array; phase
     // timing (which creates events) is off for the call
     struct Restore {

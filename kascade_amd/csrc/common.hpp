@@ -326,6 +326,37 @@ struct TourRegion {
 // True when rank_records ranks n records with the sparse ruling set at the top
 // level (the only top level that writes through a JobOut).
 bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm);
+// Local contraction of a list ranking's level 0 (contract.hip).
+// contraction_mode gives KC_CONTRACT_ON or OFF, or AUTO when a sample must
+// decide: contraction_sample then adds to *out (zeroed before) how many of
+// 65536 evenly spread packed records lie inside runs of consecutive
+// successors, and
+// contraction_sample_wins turns the count into the decision.  The ranking
+// itself contracts, ranks the contracted list and writes every element's
+// result (root element ids to root, or u32 ids local to their segment to
+// seg_root): rank_contracted_list from the caller's arrays, with the checks
+// pack_records makes, rank_contracted from packed records (unit_weights: the
+// caller gave no weights).
+struct ContractOut {
+  uint64_t* root = nullptr;
+  int64_t* rank = nullptr;
+  uint32_t* seg_root = nullptr;
+  const uint64_t* seg_off = nullptr;
+  uint64_t nseg = 0;
+};
+uint32_t contraction_mode(const kc_rank_cfg* cfg, uint64_t n);
+void contraction_sample(Context* ctx, const Rec* rec, uint64_t n, unsigned long long* out, hipStream_t st);
+bool contraction_sample_wins(uint64_t sampled_interior);
+// The same decision from the caller's successor array, before any pack
+// (synchronises `st`): for lists of kContractEarlySample elements or more,
+// where the synchronisation costs about 1 % and a contracting ranking saves
+// the pack.
+bool contraction_sample_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st);
+constexpr uint64_t kContractEarlySample = uint64_t{1} << 24;
+void rank_contracted_list(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, const kc_rank_cfg* cfg,
+                          int algorithm, const ContractOut& out, hipStream_t st);
+void rank_contracted(Context* ctx, const Rec* rec, uint64_t n, bool unit_weights, const kc_rank_cfg* cfg,
+                     int algorithm, const ContractOut& out, hipStream_t st);
 void pd_rounds_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint32_t rounds, uint64_t* succ_out, int64_t* rank_out, hipStream_t st);
 // Packs succ/weight into records with validation (range, tail weight).

@@ -1463,14 +1463,33 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
     throw KcError{KC_EINVAL, "invalid algorithm"};
   if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
   if (cfg && cfg->validate) validate_list(ctx, succ, n, st);
+  uint32_t contract = contraction_mode(cfg, n);
+  ContractOut out;
+  out.root = root;
+  out.rank = rank;
+  if (contract == KC_CONTRACT_AUTO && n >= kContractEarlySample)
+    contract = contraction_sample_list(ctx, succ, n, st) ? KC_CONTRACT_ON : KC_CONTRACT_OFF;
+  if (contract == KC_CONTRACT_ON) {
+    rank_contracted_list(ctx, succ, w, n, cfg, algo, out, st);
+    check_err(ctx, st);
+    return;
+  }
   Scratch scratch(ctx);
   Rec* rec = scratch.get<Rec>(n);
-  for (;;) {
-    pack_records(ctx, succ, w, n, rec, st);
-    uint64_t tails = 0;
-    read_counters(ctx, st, kSlotTails, 1, &tails);
-    if (rank_records(ctx, rec, n, n, root, rank, tails, tune, algo, st)) break;
+  pack_records(ctx, succ, w, n, rec, st);
+  if (contract == KC_CONTRACT_AUTO)
+    contraction_sample(ctx, rec, n, reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotWork, st);
+  uint64_t c[2] = {0, 0};
+  read_counters(ctx, st, kSlotTails, 2, c);
+  const uint64_t tails = c[0];
+  if (contract == KC_CONTRACT_AUTO && contraction_sample_wins(c[1])) {
+    rank_contracted(ctx, rec, n, w == nullptr, cfg, algo, out, st);
+    check_err(ctx, st);
+    return;
+  }
+  while (!rank_records(ctx, rec, n, n, root, rank, tails, tune, algo, st)) {
     tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+    pack_records(ctx, succ, w, n, rec, st);
   }
   check_err(ctx, st);
 }

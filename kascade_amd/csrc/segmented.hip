@@ -127,17 +127,32 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   jo.seg_root = root;
   jo.seg_off = seg_off;
   jo.nseg = nseg;
-  for (;;) {
-    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
+  const uint32_t contract = contraction_mode(cfg, total);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  auto pack = [&] {
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
     hipLaunchKernelGGL(seg_pack_kernel, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st, succ,
-                       seg_off, nseg, total, w, rec, ctx->d_err,
-                       reinterpret_cast<unsigned long long*>(ctx->d_counters));
-    uint64_t tails = 0;
-    read_counters(ctx, st, 0, 1, &tails);
-    if (rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st, 0, nullptr,
-                     local_roots ? &jo : nullptr))
-      break;
+                       seg_off, nseg, total, w, rec, ctx->d_err, ctr);
+  };
+  pack();
+  if (contract == KC_CONTRACT_AUTO) contraction_sample(ctx, rec, total, ctr + 1, st);
+  uint64_t c[2] = {0, 0};
+  read_counters(ctx, st, 0, 2, c);
+  const uint64_t tails = c[0];
+  if (contract == KC_CONTRACT_ON || (contract == KC_CONTRACT_AUTO && contraction_sample_wins(c[1]))) {
+    ContractOut out;
+    out.rank = rank;
+    out.seg_root = root;
+    out.seg_off = seg_off;
+    out.nseg = nseg;
+    rank_contracted(ctx, rec, total, w == nullptr, cfg, algo, out, st);
+    check_err(ctx, st);
+    return;
+  }
+  while (!rank_records(ctx, rec, total, total, root_global, rank, tails, tune, algo, st, 0, nullptr,
+                       local_roots ? &jo : nullptr)) {
     tune.launch = KC_LAUNCH_HOST;  // a capacity overflowed: redo host-sized
+    pack();
   }
   check_err(ctx, st);
   if (local_roots) return;
