@@ -26,10 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
-# HBM bytes per element moved by the level-0 chase, from rocprofv3 FETCH_SIZE
-# (x2.0, calibrated on the pack kernel's known 2 GiB read) and WRITE_SIZE (x1.0)
-# at n = 2^27: 8,376,237 KiB -> 127.8 B read and 4,253,444 KiB -> 32.5 B written.
-CHASE_TRAFFIC_BYTES_PER_ELEMENT = 127.8 + 32.5
+# The level-0 chase's measured HBM traffic per element (rocprofv3 FETCH_SIZE and
+# WRITE_SIZE, scaled by a copy kernel of known size) and the configuration it
+# was measured in, as the committed profile summary states them.
+CHASE_PROFILE = os.path.join("profiles", "chase_propagate_list_2e27_weighted.md")
 BYTES_PER_ELEMENT = 32  # succ 8 in + weight 8 in + root 8 out + rank 8 out
 CHASE_BYTES_PER_ELEMENT = 32  # chase: read (succ, w) 16 B + write (ruler, dist) 16 B per element
 
@@ -55,7 +55,9 @@ def parse():
     ap.add_argument("--spacing", type=int, default=0)
     ap.add_argument("--spacing-deep", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1 << 24)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 25,
+                    help="elements in the CPU baseline's sample (trees: a quarter as many vertices); "
+                         "about 1-2 s on 16 threads, 15-30 s of CPU work")
     ap.add_argument("--no-check", action="store_true", help="skip the bit-exact check of the last step")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-host-buffers", action="store_true",
@@ -78,13 +80,37 @@ def cpu_model():
     return "unknown"
 
 
+def profiled_traffic(phase):
+    """(bytes per element, {key: value} of the profiled configuration) for
+    `phase` from the profile summary, or (None, {}) when it has none."""
+    per_elem, conf = None, {}
+    try:
+        with open(os.path.join(ROOT, CHASE_PROFILE)) as f:
+            for line in f:
+                tok = line.split()
+                if len(tok) == 3 and tok[0] == "traffic_bytes_per_element" and tok[1] == phase:
+                    per_elem = float(tok[2])
+                if len(tok) >= 2 and tok[0] == "profiled_config" and tok[1] == phase:
+                    conf = dict(t.split("=", 1) for t in tok[2:])
+    except OSError:
+        pass
+    return per_elem, conf
+
+
+# One GPU's share of the host: the MI355X hosts this is measured on allot 16
+# CPUs to each GPU (OMP_NUM_THREADS=16), whatever os.cpu_count() shows.
+CORE_CAP = 16
+CORE_CAP_NOTE = ("threads capped at 16, the host CPU share of one GPU on this machine; "
+                 "the reference's own runs use one MPI rank per core")
+
+
 def host_cores():
-    """Cores this process may run on (the box's CPU share), capped at 16."""
+    """Cores this process may run on (the box's CPU share), capped at CORE_CAP."""
     try:
         c = len(os.sched_getaffinity(0))
     except AttributeError:
         c = os.cpu_count() or 1
-    return max(1, min(16, c))
+    return max(1, min(CORE_CAP, c))
 
 
 def cpu_baseline(n_sample, seed):
@@ -111,7 +137,7 @@ def cpu_baseline(n_sample, seed):
     dt_pd, ok_pd = run(lambda: o.pointer_doubling(succ, w, ranks))
     dt_seq, ok_seq = run(lambda: o.sequential_walk(succ, w))
     rate = lambda t: n_sample / t
-    return {"value": rate(dt), "unit": "elements/s", "cores": ranks, "kind": "port",
+    return {"value": rate(dt), "unit": "elements/s", "cores": ranks, "cores_note": CORE_CAP_NOTE, "kind": "port",
             "cpu_model": cpu_model(),
             "sample": f"random weighted list n={n_sample} (2^{n_sample.bit_length() - 1}) of the same generator: "
                       f"sparse ruling set (ultimate-sanders f=2, spawn, post-invert, local contraction, "
@@ -320,7 +346,7 @@ def tree_main(args, ctx):
         er, ed = o.tree_closed_form(n, args.seed)
         out["bit_exact_vs_closed_form"] = bool((root.cpu().numpy() == er).all() and (depth.cpu().numpy() == ed).all())
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = tree_cpu_baseline(min(args.cpu_sample, n, 1 << 22), args.seed)
+        out["cpu_baseline"] = tree_cpu_baseline(min(args.cpu_sample // 4, n), args.seed)
     print(json.dumps(out), flush=True)
 
 
@@ -345,7 +371,7 @@ def tree_cpu_baseline(n_sample, seed):
     t0 = time.perf_counter()
     r1, d1 = o.euler_tour_rooting(p1)
     dt1 = time.perf_counter() - t0
-    return {"value": n_sample / dt, "unit": "vertices/s", "cores": ranks, "kind": "port", "cpu_model": cpu_model(),
+    return {"value": n_sample / dt, "unit": "vertices/s", "cores": ranks, "cores_note": CORE_CAP_NOTE, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"random rooted tree n={n_sample} of the same generator, Euler-tour rooting on {ranks} threads "
                       f"(parallel tour build and linking, sparse ruling set on {ranks} emulated ranks, map back), "
                       f"{dt:.2f} s",
@@ -424,7 +450,7 @@ def segmented_cpu_baseline(off, seeds, local, max_elements):
     r, rk = o.sparse_ruling_set(succ, w, ranks)
     dt = time.perf_counter() - t0
     er, ek = o.sequential_walk(succ, w)
-    return {"value": total / dt, "unit": "elements/s", "cores": ranks, "kind": "port", "cpu_model": cpu_model(),
+    return {"value": total / dt, "unit": "elements/s", "cores": ranks, "cores_note": CORE_CAP_NOTE, "kind": "port", "cpu_model": cpu_model(),
             "sample": f"the batch's first {k} segments ({total} elements) as one forest: sparse ruling set "
                       f"(ultimate-sanders f=2, spawn, post-invert, local contraction, 2 rounds + PD) on {ranks} "
                       f"emulated ranks, {dt:.2f} s",
@@ -474,7 +500,8 @@ def main():
         alg_bytes = BYTES_PER_ELEMENT * n
     achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
     whole_gbps = BYTES_PER_ELEMENT * n / (ms_per_step * 1e-3) / 1e9
-    traffic = CHASE_TRAFFIC_BYTES_PER_ELEMENT * n if dom == "chase_ruler_L0" else None
+    traffic_per_elem, traffic_conf = profiled_traffic(dom) if n == 1 << 27 else (None, {})
+    traffic = traffic_per_elem * n if traffic_per_elem is not None else None
 
     check = None
     if not args.no_check:
@@ -514,6 +541,19 @@ def main():
     ctx.rank_list(succ, w, root, rank, **cfg)
     out["ruler_trace"] = ctx.trace_read()
     ctx.trace(False)
+    if traffic is not None:
+        # the profiled configuration must be the one this run used
+        lvl0 = out["ruler_trace"][0]
+        spawn_on = lvl0.get("spawned", 0) > 0
+        matches = (str(lvl0.get("spacing")) == traffic_conf.get("spacing")
+                   and ("on" if spawn_on else "off") == traffic_conf.get("spawn"))
+        out["roofline"]["traffic_source"] = CHASE_PROFILE
+        out["roofline"]["traffic_config_matches"] = matches
+        if not matches:
+            out["roofline"]["traffic"] = None
+            out["roofline"]["traffic_note"] = (f"profile measured {traffic_conf}, this run used spacing "
+                                               f"{lvl0.get('spacing')} spawn {'on' if spawn_on else 'off'}")
+            traffic = None
     if not args.no_host_buffers:
         out["host_buffers"] = host_buffer_rates(ctx, succ, w, cfg)
     del succ, w, root, rank
