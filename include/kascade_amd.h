@@ -57,7 +57,11 @@ typedef enum kc_status {
 /* Mirrors kascade::Algorithm (lib/kascade/configuration.hpp:9-18) for the
  * members that exist on one GPU. */
 typedef enum kc_algorithm {
-  KC_ALGO_AUTO = 0,
+  KC_ALGO_AUTO = 0,             /* pointer doubling up to 2^19 list elements or 2^17 tree
+                                   vertices (measured faster there), the sparse ruling set above
+                                   or when the config sets a ruling-set knob (ruler_spacing,
+                                   ruler_spacing_deep, max_levels, spawn ON); batched serving
+                                   always ranks with the sparse ruling set */
   KC_ALGO_POINTER_DOUBLING = 1, /* sync Wyllie doubling, pointer_doubling_generic.hpp:608-700 */
   KC_ALGO_SPARSE_RULING_SET = 2 /* ruler chase + recursion, sparse_ruling_set.cpp:45-393 */
 } kc_algorithm;

@@ -323,6 +323,15 @@ struct TourRegion {
                                 int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
                                 hipStream_t st, uint32_t out_shift = 0, const TourRegion* tours = nullptr,
                                 const JobOut* jobs_out = nullptr);
+// KC_ALGO_AUTO: pointer doubling up to these sizes, the sparse ruling set
+// above.  Measured on MI355X (whole calls, random weighted lists: PD 0.28 ms
+// against SRS 0.31 at 2^19, 0.51 against 0.34 at 2^20; random trees: PD 0.23
+// against 0.26 at 2^17 vertices, 0.29 against 0.27 at 2^18).
+constexpr uint64_t kAutoPdMaxList = uint64_t{1} << 19;   // list elements
+constexpr uint64_t kAutoPdMaxTree = uint64_t{1} << 17;   // tree vertices
+// Validates cfg->algorithm and resolves KC_ALGO_AUTO for a ranking of n list
+// elements (tree = false) or n tree vertices (tree = true).
+int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree);
 // True when rank_records ranks n records with the sparse ruling set at the top
 // level (the only top level that writes through a JobOut).
 bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm);

@@ -346,10 +346,7 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
   RankTuning tune = tuning_for(ctx, n, rc, /*tree=*/true);
-  int algo = rc ? rc->algorithm : KC_ALGO_AUTO;
-  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
-    throw KcError{KC_EINVAL, "invalid algorithm"};
-  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  const int algo = resolve_algorithm(rc, n, true);
   for (;;) {
     uint64_t nheads = 0;
     const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, heads, &nheads, st);

@@ -728,7 +728,10 @@ pd_flag_roots_kernel(Rec* __restrict__ a, uint64_t n, uint32_t* err, const uint6
 }
 
 // `active` is a flag (any element still unfinished), written by one lane per
-// wave that has one; no counting, so no same-address atomics.
+// wave that has one: a plain store of 1 into the XCD's L2, read only by later
+// launches (kernel boundaries write it back).  Agent-scope atomics here (a
+// load, then a store, per wave) went past L2 to one address and cost about
+// 85 us per round at 2^20.
 __global__ void __launch_bounds__(kThreads)
 pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
                 unsigned long long* active) {
@@ -745,10 +748,7 @@ pd_round_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
     }
   }
   const uint64_t m = __ballot(still);
-  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) {
-    if (__hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-      __hip_atomic_store(active, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) *active = 1ull;
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -784,10 +784,7 @@ pd_round_ds_kernel(const Rec* __restrict__ a, Rec* __restrict__ b, uint64_t n,
     }
   }
   const uint64_t m = __ballot(still);
-  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) {
-    if (__hip_atomic_load(&active[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-      __hip_atomic_store(&active[r], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (m != 0 && __lane_id() == static_cast<unsigned>(__ffsll(static_cast<long long>(m)) - 1)) active[r] = 1u;
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -1382,6 +1379,17 @@ std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_
 
 }  // namespace
 
+int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree) {
+  const int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
+  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
+    throw KcError{KC_EINVAL, "invalid algorithm"};
+  if (algo != KC_ALGO_AUTO) return algo;
+  // a configuration that sets ruling-set knobs asks for the ruling set
+  if (cfg && (cfg->ruler_spacing || cfg->ruler_spacing_deep || cfg->max_levels || cfg->spawn == KC_SPAWN_ON))
+    return KC_ALGO_SPARSE_RULING_SET;
+  return n <= (tree ? kAutoPdMaxTree : kAutoPdMaxList) ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET;
+}
+
 bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm) {
   return !(n <= tune.small_n && n <= kSmallMax) && algorithm == KC_ALGO_SPARSE_RULING_SET &&
          tune.max_levels > 0;
@@ -1458,10 +1466,7 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
   if (n == 0) return;
   if (n >= kRootFlag >> 1) throw KcError{KC_EINVAL, "n must be below 2^62"};
   RankTuning tune = tuning_for(ctx, n, cfg);
-  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
-  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
-    throw KcError{KC_EINVAL, "invalid algorithm"};
-  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  const int algo = resolve_algorithm(cfg, n, false);
   if (cfg && cfg->validate) validate_list(ctx, succ, n, st);
   uint32_t contract = contraction_mode(cfg, n);
   ContractOut out;

@@ -112,10 +112,7 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
   // size class of the whole forest; the ruler spacing per level is capped by
   // the mean list length inside the ranking
   RankTuning tune = tuning_for(ctx, total, cfg);
-  int algo = cfg ? cfg->algorithm : KC_ALGO_AUTO;
-  if (algo != KC_ALGO_AUTO && algo != KC_ALGO_POINTER_DOUBLING && algo != KC_ALGO_SPARSE_RULING_SET)
-    throw KcError{KC_EINVAL, "invalid algorithm"};
-  if (algo == KC_ALGO_AUTO) algo = KC_ALGO_SPARSE_RULING_SET;
+  const int algo = resolve_algorithm(cfg, total, false);
   // With the ruling set at the top level the ranking writes segment-local
   // roots directly; otherwise global roots go to a scratch array and a last
   // pass converts them.

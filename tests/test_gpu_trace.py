@@ -31,7 +31,7 @@ def test_trace_single_list(ctx, n):
     say("trace list", n)
     succ, w = ctx.gen_list(n, 3, weighted=True)
     ctx.trace(True)
-    root, rank = ctx.rank_list(succ, w)
+    root, rank = ctx.rank_list(succ, w, algorithm="srs")
     tr = ctx.trace_read()
     ctx.trace(False)
     check_levels(tr, n)
@@ -44,7 +44,7 @@ def test_trace_forest_and_unclaimed(ctx):
     say("trace forest")
     succ, w = random_forest_lists(200000, 3000, 5)
     ctx.trace(True)
-    ctx.rank_list(dev(succ), dev(w))
+    ctx.rank_list(dev(succ), dev(w), algorithm="srs")
     tr = ctx.trace_read()
     ctx.trace(False)
     check_levels(tr, 200000)
