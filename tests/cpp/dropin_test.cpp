@@ -16,6 +16,7 @@
 extern "C" {  // CPU oracle (oracle/oracle.c), the checker
 int64_t orc_compute_euler_tour(const uint64_t* parent, uint64_t n, uint64_t* tsucc, int64_t* tw);
 void orc_gen_list(uint64_t n, uint64_t seed, int weighted, uint64_t* succ, int64_t* w);
+void orc_gen_list_local(uint64_t n, uint64_t seed, int weighted, uint64_t* succ, int64_t* w);
 void orc_gen_tree(uint64_t n, uint64_t seed, uint64_t* parent);
 void orc_local_pointer_chasing(uint64_t* succ, int64_t* rank, uint64_t n);
 int orc_euler_tour_rooting(uint64_t* succ, int64_t* rank, uint64_t n);
@@ -270,6 +271,29 @@ static void reference_shaped_calls() {
   auto dist = kascade::set_initial_ranking_state(succ, root, rank, comm);
   kascade::pointer_doubling(pd, root, rank, dist, comm, grid);
   EXPECT(root == eroot && rank == erank);
+  // a locality-biased list through the contraction switches: forced on for
+  // the ruling set, off for doubling; the results do not change
+  {
+    std::vector<kascade::idx_t> ls(n), lroot(n);
+    std::vector<kascade::rank_t> lw(n), lrank(n);
+    orc_gen_list_local(n, 77, 0, ls.data(), lw.data());
+    std::vector<kascade::idx_t> er2 = ls;
+    std::vector<kascade::rank_t> ek2(n);
+    for (uint64_t i = 0; i < n; i++) ek2[i] = ls[i] == i ? 0 : 1;
+    orc_local_pointer_chasing(er2.data(), ek2.data(), n);
+    for (bool contract : {true, false}) {
+      auto d2 = kascade::set_initial_ranking_state(ls, lroot, lrank, comm);
+      kascade::SparseRulingSetConfig c = srs;
+      c.use_local_contraction = contract;
+      kascade::sparse_ruling_set(c, lroot, lrank, d2, comm, grid);
+      EXPECT(lroot == er2 && lrank == ek2);
+      d2 = kascade::set_initial_ranking_state(ls, lroot, lrank, comm);
+      kascade::PointerDoublingConfig p2 = pd;
+      p2.use_local_preprocessing = contract;
+      kascade::pointer_doubling(p2, lroot, lrank, d2, comm, grid);
+      EXPECT(lroot == er2 && lrank == ek2);
+    }
+  }
   // the legacy base algorithm is promoted by the adapter
   kascade::SparseRulingSetConfig legacy;
   legacy.base_algorithm = kascade::Algorithm::SparseRulingSet;

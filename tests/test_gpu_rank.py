@@ -244,9 +244,13 @@ def test_cpp_dropin_header():
     """The C++ drop-in layer (include/kascade_amd.hpp): pack KAT, identity
     sanity check, AbstractAlgorithm adapters against GatherRank, errors."""
     import subprocess
-    exe = os.path.join(os.path.dirname(__file__), "cpp", "dropin_test")
-    if not os.path.exists(exe):
-        subprocess.run(["sh", os.path.join(os.path.dirname(__file__), "cpp", "build.sh")], check=True)
+    here = os.path.dirname(__file__)
+    exe = os.path.join(here, "cpp", "dropin_test")
+    inputs = [os.path.join(here, "cpp", "dropin_test.cpp"),
+              os.path.join(here, "..", "include", "kascade_amd.hpp"),
+              os.path.join(here, "..", "include", "kascade_amd.h")]
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(f) for f in inputs):
+        subprocess.run(["sh", os.path.join(here, "cpp", "build.sh")], check=True)
     say("cpp dropin")
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
