@@ -278,11 +278,14 @@ kc_status kc_unpack(void *ctx, const uint64_t *psucc, const int64_t *prank,
  * forest with global ids seg_off[s] + local.  root_local receives local ids.
  * weight == NULL: unit weights as set_initial_ranking_state.  The whole batch
  * is one ranking with one ruler spacing: the autotuned spacing of the
- * batch's total size, capped at the mean segment length / 16 and at 128 when
- * the batch holds more than 64 lists (a chase ends at every segment's tail,
- * so short segments need no denser rulers; on 20,000 lists of 2^8..2^20
- * elements spacings 128 / 256 / 512 measured 114 / 130 / 151 ms on MI355X).
- * cfg->local_contraction applies to the batch as a whole. */
+ * batch's total size, capped at the mean segment length / 16 and, when the
+ * batch holds more than 64 lists, so that the chases in flight (taken in id
+ * order, K ids apart) stay inside a window of records that fits the GPU's
+ * 256 MB Infinity Cache (a chase ends at every segment's tail, so short
+ * segments need no denser rulers of their own; on 20,000 lists of
+ * 2^8..2^20 elements spacings 32 / 64 / 128 / 256 measured 124 / 106 / 116 /
+ * 130 ms on MI355X).  cfg->local_contraction applies to the batch as a
+ * whole. */
 kc_status kc_rank_lists_segmented(void *ctx, const uint32_t *succ_local, const uint64_t *seg_off,
                                   uint64_t nseg, const int64_t *weight, uint32_t *root_local,
                                   int64_t *rank_out, const kc_rank_cfg *cfg, void *stream);

@@ -898,21 +898,32 @@ uint32_t pd_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uin
   return rounds;
 }
 
+// The MI355X's Infinity Cache (MALL), shared by its eight XCDs.
+constexpr uint64_t kMallBytes = uint64_t{256} << 20;
+
 // Ruler spacing of a level: the tuned K, unless the caller fixed it, capped
 // for a forest of max_tails lists at the first power of two above
 // (mean list length) / 16, so the unreached head prefixes (about K per list)
 // stay within about an eighth of the level.  The table's spacings above 128
-// (sizes beyond 2^27) were tuned on single lists; a top level of more than
-// 64 lists keeps 128 (20,000 lists, 2.5e9 elements: 114 ms at 128, 130 ms at
-// 256, 151 ms at 512).
-uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails) {
+// (sizes beyond 2^27) were tuned on single lists.  A top level of more than
+// 64 lists takes its rulers in id order, so the chases in flight (`lanes` of
+// them, K ids apart) work in a window of about lanes x K records, and the
+// lists inside that window are all they touch: the spacing is capped so the
+// window fits the MALL (20,000 lists, 2.5e9 elements, 3 chase workgroups per
+// CU: 116 ms at K = 128 (a 402 MB window), 106 ms at 64 (201 MB), 124 ms at
+// 32; with 2 workgroups per CU 106.6 ms at 128 (268 MB)).
+uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails, uint64_t lanes) {
   uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
   if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
     const uint64_t mean = n / std::max<uint64_t>(1, max_tails);
     uint32_t cap = 2;
     while (cap < spacing && uint64_t{cap} * 16 <= mean) cap *= 2;
     spacing = std::min(spacing, cap);
-    if (level == 0 && max_tails > 64) spacing = std::min(spacing, 128u);
+    if (level == 0 && max_tails > 64) {
+      uint32_t window = 128;
+      while (window > 2 && lanes * window * sizeof(Rec) > kMallBytes) window /= 2;
+      spacing = std::min(spacing, window);
+    }
   }
   return spacing;
 }
@@ -1001,11 +1012,11 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   const uint64_t nF = tours ? tours->nheads : 0;
   const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
-  const uint32_t spacing = level_spacing(tune, level, n, max_tails);
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads);
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t lanes_wanted = nR + nF;
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
   const unsigned chase_blocks = static_cast<unsigned>(
       std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (lanes_wanted + kThreads - 1) / kThreads)));
   const SpawnPlan sp = spawn_plan(tune, n, logK, tours != nullptr, chase_blocks);
@@ -1190,11 +1201,11 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   const uint64_t nF = tours ? tours->nheads : 0;
   const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
-  const uint32_t spacing = level_spacing(tune, level, n, max_tails);
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads);
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t lists = std::min<uint64_t>(max_tails, n);
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
   const unsigned chase_blocks = static_cast<unsigned>(
       std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (nR + nF + kThreads - 1) / kThreads)));
   const SpawnPlan sp = spawn_plan(tune, n, logK, tours != nullptr, chase_blocks);

@@ -53,12 +53,18 @@ def test_server_mix_matches_alone(ctx, mode):
     jobs = make_jobs(120, 5)
     sv = kascade_amd.Server(ctx, one_job_per_launch=(mode == "one_job_per_launch"), num_slots=3,
                             tour_build="sorted" if mode == "batched_sorted_tours" else "auto")
-    tickets = []
+    # every job's arrays first, then the submissions in one burst: jobs that
+    # arrive while earlier batches run are coalesced
+    arrays = []
     for kind, succ, w in jobs:
         s = dev(succ)
         ww = dev(w) if w is not None else None
         ro = torch.empty(len(succ), dtype=torch.uint64, device="cuda")
         ra = torch.empty(len(succ), dtype=torch.int64, device="cuda")
+        arrays.append((s, ww, ro, ra))
+    torch.cuda.synchronize()
+    tickets = []
+    for (kind, _, _), (s, ww, ro, ra) in zip(jobs, arrays):
         t, _, _ = sv.submit(kind, s, ww, ro, ra)
         tickets.append((t, s, ww, ro, ra))
     for t, *_ in tickets:
