@@ -9,6 +9,9 @@
  * Conventions (lib/kascade/types.hpp:8-9):
  *   ids are uint64_t (idx_t), ranks are int64_t (rank_t); ids must be < 2^63
  *   because bit 63 is the root flag (lib/kascade/bits.hpp:9-21).
+ *   One call ranks at most 2^32 - 256 records: list elements, or twice the
+ *   vertices of a tree (its Euler tour); kernel grids count work-items in 32
+ *   bits.  Larger calls return KC_EINVAL.
  *
  * Pointers may be device memory (hipMalloc / torch tensors) or host memory;
  * host arrays are staged through device buffers by the library.  All entry

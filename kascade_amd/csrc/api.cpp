@@ -295,6 +295,7 @@ class Staged {
          hipStream_t st)
       : user_(user), count_(count), out_(out), st_(st) {
     (void)ctx;
+    check_records(count);
     if (user == nullptr || is_device_ptr(user)) {
       dev_ = const_cast<M*>(user);
       return;

@@ -202,6 +202,14 @@ class Phase {
   hipStream_t st_;
 };
 
+// Kernel grids count work-items in 32 bits (blocks x threads < 2^32), and the
+// one-thread-per-record passes launch ceil(records / 256) blocks of 256, so a
+// ranking holds at most kMaxRecords records (a tree's tour: two per vertex).
+constexpr uint64_t kMaxRecords = (uint64_t{1} << 32) - 256;
+inline void check_records(uint64_t records) {
+  if (records > kMaxRecords) throw KcError{KC_EINVAL, "more than 2^32 - 256 records (elements, or twice the vertices)"};
+}
+
 inline unsigned blocks_for(uint64_t n, unsigned threads) {
   uint64_t b = (n + threads - 1) / threads;
   return static_cast<unsigned>(b ? b : 1);

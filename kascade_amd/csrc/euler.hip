@@ -304,7 +304,7 @@ ref_dummies_kernel(uint64_t E, uint64_t roots, uint64_t* __restrict__ tsucc, int
 uint64_t compute_euler_tour_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* tour_succ,
                                    int64_t* tour_w, hipStream_t st) {
   if (n == 0) return 0;
-  if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "compute_euler_tour supports n < 2^32 - 1"};
+  check_records(2 * n);
   Scratch scratch(ctx);
   uint32_t* keys = scratch.get<uint32_t>(n);
   uint32_t* vals = scratch.get<uint32_t>(n);
@@ -340,6 +340,7 @@ uint64_t compute_euler_tour_device(Context* ctx, const uint64_t* parent, uint64_
 void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t* root,
                       int64_t* depth, const kc_tree_cfg* cfg, hipStream_t st) {
   if (n == 0) return;
+  check_records(2 * n);
   Scratch scratch(ctx);
   Rec* tour = scratch.get<Rec>(2 * n);
   uint64_t* heads = scratch.get<uint64_t>(n);

@@ -1479,7 +1479,7 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
                       uint64_t* root, int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st) {
   if (n == 0) return;
-  if (n >= kRootFlag >> 1) throw KcError{KC_EINVAL, "n must be below 2^62"};
+  check_records(n);
   RankTuning tune = tuning_for(ctx, n, cfg);
   const int algo = resolve_algorithm(cfg, n, false);
   if (cfg && cfg->validate) validate_list(ctx, succ, n, st);

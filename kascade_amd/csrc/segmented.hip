@@ -106,6 +106,7 @@ void rank_segmented_device(Context* ctx, const uint32_t* succ, const uint64_t* s
                            uint64_t nseg, uint64_t total, const int64_t* w, uint32_t* root,
                            int64_t* rank, const kc_rank_cfg* cfg, hipStream_t st) {
   if (nseg == 0 || total == 0) return;
+  check_records(total);
   hipLaunchKernelGGL(seg_check_kernel, dim3(blocks_for(nseg, kThreads)), dim3(kThreads), 0, st,
                      seg_off, nseg, ctx->d_err);
   check_err(ctx, st);

@@ -213,6 +213,13 @@ def test_errors(ctx, algo):
     with pytest.raises(kascade_amd.KascadeError) as e:
         ctx.rank_list(dev(tree), None, algorithm=algo, validate=True)
     assert e.value.status == kascade_amd.KC_EINVAL
+    # more records than 32-bit kernel grids can cover: rejected before any
+    # array is touched
+    import ctypes
+    small = dev(np.zeros(4, np.uint64))
+    p = ctypes.c_void_p(small.data_ptr())
+    assert ctx._lib.kc_rank_list(ctx.handle, p, None, ctypes.c_uint64(1 << 32), p, p, None,
+                                 None) == kascade_amd.KC_EINVAL
     # the context stays usable
     succ, w = o.gen_list(1000, 1, True)
     root, rank = ctx.rank_list(dev(succ), dev(w), algorithm=algo)
