@@ -253,16 +253,17 @@ void check_err(Context* ctx, hipStream_t st) {
   raise_from_bits(ctx, st, static_cast<uint32_t>(ctx->h_pinned[kErrSlot]));
 }
 
-bool check_err_overflow(Context* ctx, hipStream_t st) {
+uint32_t check_err_overflow(Context* ctx, hipStream_t st) {
   KC_HIP(hipMemcpyAsync(ctx->h_pinned + kErrSlot, ctx->d_counters + kErrSlot, sizeof(uint64_t),
                         hipMemcpyDeviceToHost, st));
   KC_HIP(hipStreamSynchronize(st));
   const uint32_t e = static_cast<uint32_t>(ctx->h_pinned[kErrSlot]);
-  raise_from_bits(ctx, st, e & ~kErrOverflow);
-  if (!(e & kErrOverflow)) return false;
+  const uint32_t redo = kErrOverflow | kErrNarrow;
+  raise_from_bits(ctx, st, e & ~redo);
+  if (!(e & redo)) return 0;
   KC_HIP(hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), st));
   KC_HIP(hipStreamSynchronize(st));
-  return true;
+  return e & redo;
 }
 
 void read_counters(Context* ctx, hipStream_t st, int first, int count, uint64_t* out) {

@@ -24,6 +24,21 @@ struct alignas(16) Rec {
   int64_t w;
 };
 
+// The narrow form of a level-0 list record, 8 bytes: ids below 2^30 with the
+// root and aux flags in bits 31 and 30, and weights and claim distances that
+// fit int32.  A dependent random walk that claims every record it reads runs
+// 10 % faster over 8-byte records than over 16-byte ones (2^27 records on
+// MI355X: 5.61 against 6.27 ms), because twice as many of them stay in the
+// caches.  Used for the top level of single-list rankings; anything that
+// does not fit is redone with wide records.
+struct alignas(8) Rec8 {
+  uint32_t s;
+  int32_t w;
+};
+constexpr uint32_t kRoot8 = 1u << 31;
+constexpr uint32_t kAux8 = 1u << 30;
+constexpr uint64_t kNarrowMaxIds = uint64_t{1} << 30;
+
 // Device-side error word: kernels OR these bits in; the host maps them to kc_status.
 enum ErrBits : uint32_t {
   kErrRange = 1u,      // successor id >= n
@@ -31,6 +46,7 @@ enum ErrBits : uint32_t {
   kErrNotList = 4u,    // an element reached twice (in-degree > 1)
   kErrCycle = 8u,      // a cycle without a tail
   kErrOverflow = 16u,  // a device-sized level outgrew a capacity (redo host-sized; never reported)
+  kErrNarrow = 32u,    // a claim distance outgrew a narrow record (redo wide; never reported)
 };
 
 // Scratch pool: chunks reused across calls, handed out as a stack.  When a
@@ -156,6 +172,7 @@ struct Context {
   uint64_t graph_replays = 0;
   uint64_t graph_captures = 0;
   uint64_t host_redos = 0;  // device-sized rankings redone host-sized after an overflow
+  uint64_t narrow_redos = 0;  // narrow-record rankings redone with wide records
 };
 
 // Scoped stack allocation from the arena.
@@ -218,8 +235,10 @@ inline unsigned blocks_for(uint64_t n, unsigned threads) {
 // Reads the device error word (synchronising `stream`) and throws on error.
 void check_err(Context* ctx, hipStream_t stream);
 // After a device-sized ranking (synchronising `stream`): raises input errors;
-// true when a capacity overflowed and the call must be redone host-sized.
-bool check_err_overflow(Context* ctx, hipStream_t stream);
+// returns the redo bits, 0 when the call is done: kErrOverflow (a capacity
+// overflowed: redo host-sized), kErrNarrow (a claim distance outgrew a narrow
+// record: redo with wide records).
+uint32_t check_err_overflow(Context* ctx, hipStream_t stream);
 // Copies `count` counters starting at slot `first` to the host (synchronising).
 void read_counters(Context* ctx, hipStream_t stream, int first, int count, uint64_t* out);
 
@@ -340,6 +359,11 @@ constexpr uint64_t kAutoPdMaxTree = uint64_t{1} << 17;   // tree vertices
 // Validates cfg->algorithm and resolves KC_ALGO_AUTO for a ranking of n list
 // elements (tree = false) or n tree vertices (tree = true).
 int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree);
+// The narrow form of rank_records for a single list region ranked by the
+// sparse ruling set with device-sized levels: returns the redo bits of
+// check_err_overflow (0: done).
+uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st);
 // True when rank_records ranks n records with the sparse ruling set at the top
 // level (the only top level that writes through a JobOut).
 bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm);

@@ -71,6 +71,40 @@ __device__ __forceinline__ uint64_t segment_of(const uint64_t* __restrict__ seg_
   return lo;
 }
 
+// Level-0 records of either width, seen as wide ones: decode() gives the
+// wide meaning (ids with kRootFlag / kAuxFlag), encode() stores one back.
+template <class R>
+struct RecOps;
+template <>
+struct RecOps<Rec> {
+  static constexpr bool kNarrow = false;
+  __device__ static __forceinline__ Rec decode(Rec r) { return r; }
+  __device__ static __forceinline__ Rec encode(uint64_t s, int64_t w) { return Rec{s, w}; }
+  __device__ static __forceinline__ void set_s(Rec* p, uint64_t s) { p->s = s; }
+  __device__ static __forceinline__ bool fits(int64_t) { return true; }
+};
+template <>
+struct RecOps<Rec8> {
+  static constexpr bool kNarrow = true;
+  __device__ static __forceinline__ Rec decode(Rec8 r) {
+    uint64_t s = r.s & (kAux8 - 1);
+    if (r.s & kRoot8) s |= kRootFlag;
+    if (r.s & kAux8) s |= kAuxFlag;
+    return Rec{s, r.w};
+  }
+  __device__ static __forceinline__ uint32_t narrow_s(uint64_t s) {
+    uint32_t t = static_cast<uint32_t>(s & (kAux8 - 1));
+    if (s & kRootFlag) t |= kRoot8;
+    if (s & kAuxFlag) t |= kAux8;
+    return t;
+  }
+  __device__ static __forceinline__ Rec8 encode(uint64_t s, int64_t w) {
+    return Rec8{narrow_s(s), static_cast<int32_t>(w)};
+  }
+  __device__ static __forceinline__ void set_s(Rec8* p, uint64_t s) { p->s = narrow_s(s); }
+  __device__ static __forceinline__ bool fits(int64_t w) { return w >= INT32_MIN && w <= INT32_MAX; }
+};
+
 // Streaming accesses (read or written once, coalesced): non-temporal, so they
 // do not evict the gather tables the same kernel keeps in L2.
 __device__ __forceinline__ Rec load_stream(const Rec* p) {
@@ -82,6 +116,10 @@ __device__ __forceinline__ Rec load_stream(const Rec* p) {
 template <typename T>
 __device__ __forceinline__ void store_stream(T* p, T v) {
   __builtin_nontemporal_store(v, p);
+}
+__device__ __forceinline__ Rec8 load_stream(const Rec8* p) {
+  const uint64_t u = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
+  return Rec8{static_cast<uint32_t>(u), static_cast<int32_t>(static_cast<uint32_t>(u >> 32))};
 }
 
 }  // namespace kc

@@ -86,10 +86,12 @@ __global__ void zero_u64_kernel(unsigned long long* p, int count) {
 }
 
 // set_initial_ranking_state semantics for NULL weights (list_ranking.cpp:60-80);
-// range and tail-weight validation; tail count.
+// range and tail-weight validation; tail count.  Narrow records: a weight
+// that does not fit sets *wide (the caller then packs wide records).
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, uint64_t n,
-            Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
+            R* __restrict__ rec, uint32_t* err, unsigned long long* tails, unsigned long long* wide) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool tail = false;
   if (v < n) {
@@ -103,10 +105,11 @@ pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, ui
     if (w) {
       wt = w[v];
       if (tail && wt != 0) raise_err(err, kErrRootWeight);
+      if (!RecOps<R>::fits(wt)) *wide = 1;
     } else {
       wt = tail ? 0 : 1;
     }
-    rec[v] = Rec{s, wt};
+    rec[v] = RecOps<R>::encode(s, wt);
   }
   wave_append(tails, tail);
 }
@@ -164,14 +167,17 @@ constexpr int kIdleQuantum = 16;  // lanes per idle report
 constexpr uint32_t kStopPollMask = 127;
 
 // kTrace: also count chain lengths (the per-level trace); off in plain calls.
-template <bool kTrace, int kMode>
+// R: the level's record width (Rec, or Rec8 at a narrow top level).
+template <bool kTrace, int kMode, class R>
 __global__ void __launch_bounds__(kThreads)
-chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
+chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR,
              const uint64_t* __restrict__ forced, uint64_t nF, uint64_t tour_lo,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
              uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev, SpawnArgs sp) {
   if (*err) return;  // an earlier kernel found invalid input: stop, as the host-sized path does
+  using Ops = RecOps<R>;
+  bool narrow_over = false;  // a claim distance did not fit the narrow record
   const uint64_t lt = lanemask_lt();
   if (n_dev) {  // device-sized level: n is a capacity, the size is in memory
     n = *n_dev < n ? *n_dev : n;
@@ -315,8 +321,8 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
         continue;
       }
       // the ruler's own record: claim it with distance 0
-      const Rec rr = rec[cur];
-      if (claims(cur)) rec[cur] = Rec{ruler | kRootFlag, 0};
+      const Rec rr = Ops::decode(rec[cur]);
+      if (claims(cur)) rec[cur] = Ops::encode(ruler | kRootFlag, 0);
       if constexpr (kTrace) steps = 1;
       if (rr.s == cur) {
         sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
@@ -336,7 +342,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       chain_end();
       continue;
     }
-    const Rec r = rec[cur];
+    const Rec r = Ops::decode(rec[cur]);
     if (r.s & kSpawnBits) {
       if (kMode == kChaseResume && (r.s & kSpawnBits) == kSpawnBits) {
         const uint64_t id = r.s & kIdMask;
@@ -353,7 +359,7 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       if (j < cap) {
         sub[j] = Rec{j, 0};
         sub_orig[j] = cur;
-        rec[cur] = Rec{j | kRootFlag, 0};
+        rec[cur] = Ops::encode(j | kRootFlag, 0);
         sub[ruler] = Rec{j, dist};
       } else {
         raise_err(err, kErrNotList);
@@ -361,19 +367,25 @@ chase_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, ui
       active = false;
       chain_end();
     } else {
-      if (claims(cur)) rec[cur] = Rec{ruler | kRootFlag, dist};
+      if (claims(cur)) {
+        if constexpr (Ops::kNarrow) narrow_over |= !Ops::fits(dist);
+        rec[cur] = Ops::encode(ruler | kRootFlag, dist);
+      }
       dist += r.w;
       cur = r.s;
       if constexpr (kTrace) steps++;
     }
   }
+  if constexpr (Ops::kNarrow)
+    if (narrow_over) raise_err(err, kErrNarrow);
 }
 
 // Step 2 of the spawn: one thread per candidate block.  An unreached
 // candidate (not a computed ruler, record unclaimed) becomes spawned ruler j
 // while spawn slots last; its chase starts at its successor.
+template <class R>
 __global__ void __launch_bounds__(kThreads)
-spawn_mark_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, uint64_t nF,
+spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, uint64_t nF,
                   uint32_t logC, uint32_t cseed, uint64_t cap, uint64_t budget, Rec* __restrict__ sub,
                   uint64_t* __restrict__ sub_orig, unsigned long long* extra_ctr, unsigned long long* spawn_ctr,
                   SpawnArgs sp, const uint64_t* __restrict__ n_dev, const uint32_t* __restrict__ err) {
@@ -390,7 +402,7 @@ spawn_mark_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t see
   if (b < nC) {
     c = ruler_of_block(b, n, logC, cseed);
     if (!is_ruler(c, n, logK, seed)) {
-      r = rec[c];
+      r = RecOps<R>::decode(rec[c]);
       want = !(r.s & kSpawnBits);
     }
   }
@@ -399,7 +411,7 @@ spawn_mark_kernel(Rec* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t see
   const uint64_t j = nR + nF + wave_append(extra_ctr, go);
   if (!go) return;
   if (j >= cap) return;  // cannot happen: cap holds every tail plus the budget
-  rec[c] = Rec{j | kSpawnBits, 0};
+  rec[c] = RecOps<R>::encode(j | kSpawnBits, 0);
   sub_orig[j] = c;
   if (r.s == c) {
     sub[j] = Rec{j, 0};  // a tail: the spawned ruler ends its list
@@ -455,8 +467,9 @@ seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_o
 // ruler.  Unclaimed elements (list prefixes no ruler reaches, the analogue of
 // fixup_unreached, post_invert.hpp:195-264) are appended to a compact list and
 // their record is re-marked with that compact id.
+template <class R>
 __global__ void __launch_bounds__(kThreads)
-propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
+propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
                  const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
@@ -472,7 +485,7 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
-    r = load_stream(&rec[v]);
+    r = RecOps<R>::decode(load_stream(&rec[v]));
     if (r.s & kRootFlag) {
       if (one_root) {
         emit(jo, om, n_out, root, rank, v, root0, T8[r.s & kIdMask] - r.w);
@@ -494,7 +507,7 @@ propagate_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* _
   if (unclaimed && k < ucap) {
     u_ids[k] = v;
     u_orig[k] = r;
-    rec[v].s = k | kAuxFlag;
+    RecOps<R>::set_s(&rec[v], k | kAuxFlag);
   } else if (unclaimed && device_sized) {
     // beyond the capacity: host-sized levels collect the rest in a second
     // pass; device-sized ones flag the call to be redone host-sized
@@ -526,8 +539,9 @@ collect_unclaimed_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t* __restrict
 // Unclaimed sub-problem: an unclaimed element's successor is unclaimed (link
 // inside the sub-problem) or already final (it becomes a local tail whose
 // offset/root come from that final element), or it is a true tail.
+template <class R>
 __global__ void __launch_bounds__(kThreads)
-unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
+unclaimed_build_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u_ids,
                        const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T,
                        Rec* __restrict__ urec, Rec* __restrict__ ext,
                        const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err,
@@ -545,7 +559,7 @@ unclaimed_build_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__
     ext[k] = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
     return;
   }
-  const Rec ms = rec[s];
+  const Rec ms = RecOps<R>::decode(rec[s]);
   if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
     urec[k] = Rec{ms.s & kIdMask, r0.w};
     ext[k] = Rec{0, 0};
@@ -666,8 +680,9 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
 // kSmallMax elements: unclaimed_build_kernel's links, the LDS solver and
 // unclaimed_fix_kernel's outputs in one launch instead of three.  ext (nu
 // records of global scratch) is written and read back by this workgroup only.
+template <class R>
 __global__ void __launch_bounds__(kSmallThreads)
-unclaimed_small_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__ u_ids,
+unclaimed_small_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u_ids,
                        const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T, Rec* ext,
                        uint64_t n_out, uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                        const unsigned long long* __restrict__ nu_dev, uint32_t* err, JobOut jo,
@@ -686,7 +701,7 @@ unclaimed_small_kernel(const Rec* __restrict__ rec, const uint64_t* __restrict__
     if (r0.s == v) {
       e = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
     } else {
-      const Rec ms = rec[r0.s];
+      const Rec ms = RecOps<R>::decode(rec[r0.s]);
       if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
         link = static_cast<uint32_t>(ms.s & kIdMask);
         w = r0.w;
@@ -975,14 +990,15 @@ struct ChaseSlots {
 
 // The level's chase: one launch, or with spawn the suspend, mark and resume
 // launches.  The caller zeroed the counters.
-void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trace, unsigned chase_blocks, Rec* rec,
+template <class R>
+void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trace, unsigned chase_blocks, R* rec,
                   uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, const uint64_t* forced, uint64_t nF,
                   uint64_t tour_lo, uint64_t cap, Rec* sub, uint64_t* sub_orig, const ChaseSlots& cs,
                   uint32_t chunk, const uint64_t* n_dev, int level, hipStream_t st) {
   Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
   SpawnArgs sa;
   if (!sp.on) {
-    auto kern = trace ? chase_kernel<true, kChasePlain> : chase_kernel<false, kChasePlain>;
+    auto kern = trace ? chase_kernel<true, kChasePlain, R> : chase_kernel<false, kChasePlain, R>;
     hipLaunchKernelGGL(kern, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF,
                        tour_lo, cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
     return;
@@ -993,14 +1009,14 @@ void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trac
   sa.cont = scratch.get<Cont>(sp.cont_cap);
   sa.cont_ctr = cs.cont_ctr;
   sa.cont_cap = sp.cont_cap;
-  auto k1 = trace ? chase_kernel<true, kChaseSuspend> : chase_kernel<false, kChaseSuspend>;
+  auto k1 = trace ? chase_kernel<true, kChaseSuspend, R> : chase_kernel<false, kChaseSuspend, R>;
   hipLaunchKernelGGL(k1, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
                      cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
   const uint64_t nC = (n + (uint64_t{1} << sp.logC) - 1) >> sp.logC;
-  hipLaunchKernelGGL(spawn_mark_kernel, dim3(blocks_for(nC, kThreads)), dim3(kThreads), 0, st, rec, n, logK, seed,
+  hipLaunchKernelGGL(spawn_mark_kernel<R>, dim3(blocks_for(nC, kThreads)), dim3(kThreads), 0, st, rec, n, logK, seed,
                      nR, nF, sp.logC, seed ^ 0x5bd1e995u, cap, sp.budget, sub, sub_orig, cs.extra, cs.spawned, sa,
                      n_dev, ctx->d_err);
-  auto k3 = trace ? chase_kernel<true, kChaseResume> : chase_kernel<false, kChaseResume>;
+  auto k3 = trace ? chase_kernel<true, kChaseResume, R> : chase_kernel<false, kChaseResume, R>;
   hipLaunchKernelGGL(k3, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
                      cap, sub, sub_orig, cs.resume_work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
 }
@@ -1079,7 +1095,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* u_orig = scratch.get<Rec>(ucap);
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
+    hipLaunchKernelGGL(propagate_kernel<Rec>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
                        n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
                        ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false, jo);
   }
@@ -1103,13 +1119,13 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   if (nu <= kSmallMax && nu <= tune.small_n && !ctx->trace_on) {
     // small enough for one workgroup (the per-level trace keeps the
     // sub-problem as its own ranking)
-    hipLaunchKernelGGL(unclaimed_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, nu, T,
+    hipLaunchKernelGGL(unclaimed_small_kernel<Rec>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, nu, T,
                        ext, n_out, root, rank, om, nullptr, ctx->d_err, jo, jo.seg_off, jo.nseg);
     return;
   }
   Rec* urec = scratch.get<Rec>(nu);
   Rec* ures = scratch.get<Rec>(nu);
-  hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
+  hipLaunchKernelGGL(unclaimed_build_kernel<Rec>, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
                      rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err, jo.seg_off, jo.nseg);
   // every unclaimed prefix belongs to a different list of this level
   ctx->trace_subproblem = 1;
@@ -1169,7 +1185,8 @@ enum DsSlot : int {
   kDsSpawnWork = 7, kDsSpawnIdle = 8, kDsSpawned = 9, kDsSpawnCont = 10, kDsSpawnStop = 11, kDsSlots = 12
 };
 
-void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+template <class R>
+void ds_rank(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
              int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
              hipStream_t st, const TourRegion* tours, const JobOut& jo = JobOut{});
 
@@ -1194,7 +1211,8 @@ void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
                      ctx->d_err);
 }
 
-void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+template <class R>
+void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
                   int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int level,
                   hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
@@ -1243,7 +1261,7 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   Rec* u_orig = scratch.get<Rec>(std::max<uint64_t>(ucap, 1));
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(propagate_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out, T,
+    hipLaunchKernelGGL(propagate_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out, T,
                        T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
                        ctx->d_err, n_dev, true, jo);
   }
@@ -1252,13 +1270,13 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
   Rec* ext = scratch.get<Rec>(ucap);
   const unsigned long long* nu_dev = ctr + kDsUnclaimed;
   if (ucap <= kSmallMax && ucap <= tune.small_n) {
-    hipLaunchKernelGGL(unclaimed_small_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, ucap, T,
+    hipLaunchKernelGGL(unclaimed_small_kernel<R>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, ucap, T,
                        ext, n_out, root, rank, om, nu_dev, ctx->d_err, jo, jo.seg_off, jo.nseg);
     return;
   }
   Rec* urec = scratch.get<Rec>(ucap);
   Rec* ures = scratch.get<Rec>(ucap);
-  hipLaunchKernelGGL(unclaimed_build_kernel, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
+  hipLaunchKernelGGL(unclaimed_build_kernel<R>, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
                      u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err, jo.seg_off, jo.nseg);
   ds_rank(ctx, urec, ucap, reinterpret_cast<const uint64_t*>(nu_dev), ucap, &ures[0].s, &ures[0].w, OutMap{2, 0},
           std::min(ucap, max_tails), tune, ucap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
@@ -1267,22 +1285,30 @@ void ds_srs_level(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uin
                      n_out, ures, ext, root, rank, om, nu_dev, ctx->d_err, jo);
 }
 
-void ds_rank(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
+template <class R>
+void ds_rank(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
              int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int algorithm, int level,
              hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   if (n == 0) return;
-  if (n <= tune.small_n && n <= kSmallMax) {
-    Phase ph(ctx, "small_rank", st);
-    hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, static_cast<uint32_t>(n),
-                       static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om, ctx->d_err, n_dev);
-    return;
+  if constexpr (RecOps<R>::kNarrow) {
+    // a narrow top level is always a ruling-set level over a single list region
+    (void)algorithm;
+    (void)tours;
+    ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st, nullptr, jo);
+  } else {
+    if (n <= tune.small_n && n <= kSmallMax) {
+      Phase ph(ctx, "small_rank", st);
+      hipLaunchKernelGGL(small_rank_kernel, dim3(1), dim3(kSmallThreads), 0, st, rec, static_cast<uint32_t>(n),
+                         static_cast<uint32_t>(std::min<uint64_t>(n_out, n)), root, rank, om, ctx->d_err, n_dev);
+      return;
+    }
+    if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
+      ds_pd(ctx, rec, n, n_dev, n_out, root, rank, om, st);
+      return;
+    }
+    ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st,
+                 tours && tours->lo < n ? tours : nullptr, jo);
   }
-  if (algorithm == KC_ALGO_POINTER_DOUBLING || level >= static_cast<int>(tune.max_levels)) {
-    ds_pd(ctx, rec, n, n_dev, n_out, root, rank, om, st);
-    return;
-  }
-  ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st,
-               tours && tours->lo < n ? tours : nullptr, jo);
 }
 
 }  // namespace
@@ -1355,16 +1381,43 @@ void pack_records(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t
                   hipStream_t st) {
   Phase ph(ctx, "pack", st);
   zero_counters(ctx, st);
-  hipLaunchKernelGGL(pack_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, succ, w,
-                     n, rec, ctx->d_err,
-                     reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotTails);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  hipLaunchKernelGGL(pack_kernel<Rec>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, succ, w, n, rec,
+                     ctx->d_err, ctr + kSlotTails, ctr + kSlotExtra);
 }
+
+namespace {
+
+// Packs narrow records; counter kSlotExtra becomes nonzero when a weight does
+// not fit int32 (the records are then not usable).
+void pack_records_narrow(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n, Rec8* rec,
+                         hipStream_t st) {
+  Phase ph(ctx, "pack", st);
+  zero_counters(ctx, st);
+  unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
+  hipLaunchKernelGGL(pack_kernel<Rec8>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, succ, w, n, rec,
+                     ctx->d_err, ctr + kSlotTails, ctr + kSlotExtra);
+}
+
+// Narrow records apply to a single list region ranked by the ruling set with
+// device-sized levels, from 2^20 elements (below, the pack and the chase are
+// a small part of a call) and below 2^30 (ids, flags in bits 31 and 30).
+constexpr uint64_t kNarrowMin = uint64_t{1} << 20;
+bool narrow_wanted(const Context* ctx, uint64_t n, int algo, const RankTuning& tune) {
+  return n >= kNarrowMin && n < kNarrowMaxIds && algo == KC_ALGO_SPARSE_RULING_SET &&
+         tune.launch != KC_LAUNCH_HOST && !ctx->trace_on && tune.max_levels > 0;
+}
+// The level-0 ids (rulers, tails, spawned rulers) must stay below 2^30 too:
+// rulers are at most n / 2, spawned ones at most a quarter of n.
+bool narrow_ids_fit(uint64_t n, uint64_t tails) { return n / 2 + n / 4 + tails < kNarrowMaxIds; }
+
+}  // namespace
 
 namespace {
 
 // Everything the device-sized launch sequence depends on: equal keys give the
 // same kernels with the same arguments.
-std::vector<uint64_t> launch_key(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+std::vector<uint64_t> launch_key(Context* ctx, const void* rec, uint64_t n, uint64_t n_out, uint64_t* root,
                                  int64_t* rank, uint64_t max_tails, const RankTuning& t, int algorithm,
                                  uint32_t out_shift, const TourRegion* tours, const JobOut& jo) {
   std::vector<uint64_t> k = {reinterpret_cast<uint64_t>(rec), n, n_out, reinterpret_cast<uint64_t>(root),
@@ -1410,17 +1463,25 @@ bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm) 
          tune.max_levels > 0;
 }
 
-bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
-                  int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
-                  hipStream_t st, uint32_t out_shift, const TourRegion* tours, const JobOut* jobs_out) {
+namespace {
+
+// rank_records for either record width; returns the redo bits (0: done).
+template <class R>
+uint32_t rank_records_t(Context* ctx, R* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                        int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
+                        hipStream_t st, uint32_t out_shift, const TourRegion* tours, const JobOut* jobs_out) {
   const OutMap om{1, out_shift};
   const JobOut jo = jobs_out ? *jobs_out : JobOut{};
   if (jobs_out && !top_level_is_ruling_set(n, tune, algorithm))
     throw KcError{KC_EINVAL, "per-job outputs need the sparse ruling set at the top level"};
   // the per-level trace reads every level's counts on the host
   if (tune.launch == KC_LAUNCH_HOST || ctx->trace_on) {
-    rank_records_impl(ctx, rec, n, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours, jo);
-    return true;
+    if constexpr (RecOps<R>::kNarrow) {
+      throw KcError{KC_EINVAL, "narrow records need device-sized levels"};
+    } else {
+      rank_records_impl(ctx, rec, n, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours, jo);
+      return 0;
+    }
   }
   auto launch_all = [&] {
     ds_rank(ctx, rec, n, nullptr, n_out, root, rank, om, max_tails, tune, algorithm, 0, st, tours, jo);
@@ -1432,6 +1493,7 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
     // allocated, so the capture allocates nothing).
     std::vector<uint64_t> key = launch_key(ctx, rec, n, n_out, root, rank, max_tails, tune, algorithm, out_shift,
                                            tours, jo);
+    key.push_back(sizeof(R));
     if (ctx->graph_exec && key == ctx->graph_key) {
       KC_HIP(hipGraphLaunch(ctx->graph_exec, st));
       ctx->graph_replays++;
@@ -1471,9 +1533,24 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
   } else {
     launch_all();
   }
-  if (!check_err_overflow(ctx, st)) return true;
-  ctx->host_redos++;
-  return false;
+  const uint32_t redo = check_err_overflow(ctx, st);
+  if (redo & kErrOverflow) ctx->host_redos++;
+  return redo;
+}
+
+}  // namespace
+
+bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
+                  int64_t* rank, uint64_t max_tails, const RankTuning& tune, int algorithm,
+                  hipStream_t st, uint32_t out_shift, const TourRegion* tours, const JobOut* jobs_out) {
+  return rank_records_t(ctx, rec, n, n_out, root, rank, max_tails, tune, algorithm, st, out_shift, tours,
+                        jobs_out) == 0;
+}
+
+uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st) {
+  return rank_records_t(ctx, rec, n, n, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, 0, nullptr,
+                        nullptr);
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
@@ -1495,6 +1572,22 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
     return;
   }
   Scratch scratch(ctx);
+  if (contract == KC_CONTRACT_OFF && narrow_wanted(ctx, n, algo, tune)) {
+    Scratch narrow(ctx);  // released before the wide records are taken
+    Rec8* rec8 = narrow.get<Rec8>(n);
+    pack_records_narrow(ctx, succ, w, n, rec8, st);
+    uint64_t c[3] = {0, 0, 0};
+    read_counters(ctx, st, kSlotTails, 3, c);  // tails, -, "a weight does not fit"
+    if (c[2] == 0 && narrow_ids_fit(n, c[0])) {
+      const uint32_t redo = rank_records_narrow(ctx, rec8, n, root, rank, c[0], tune, st);
+      if (redo == 0) {
+        check_err(ctx, st);
+        return;
+      }
+      ctx->narrow_redos++;
+      if (redo & kErrOverflow) tune.launch = KC_LAUNCH_HOST;  // and wide
+    }
+  }
   Rec* rec = scratch.get<Rec>(n);
   pack_records(ctx, succ, w, n, rec, st);
   if (contract == KC_CONTRACT_AUTO)

@@ -1,0 +1,97 @@
+"""Narrow (8-byte) level-0 records: single-list rankings from 2^20 elements
+pack their top level into {u32 successor, i32 weight} records and claim
+with i32 distances.  Weights that do not fit, distances that outgrow int32
+and id spaces beyond 2^30 fall back to 16-byte records; every result is
+bit-exact against the oracle either way."""
+import numpy as np
+import pytest
+
+import oracle_lib as o
+from test_gpu_rank import dev, host, random_forest_lists, say
+
+pytestmark = pytest.mark.gpu
+
+
+def check(ctx, succ, w, **cfg):
+    root, rank = ctx.rank_list(dev(succ), dev(w) if w is not None else None, **cfg)
+    if w is None:
+        w = (succ != np.arange(len(succ), dtype=np.uint64)).astype(np.int64)
+    er, ek = o.sequential_walk(succ, w)
+    assert (host(root) == er).all(), cfg
+    assert (host(rank) == ek).all(), cfg
+
+
+@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 12345, 1 << 22])
+@pytest.mark.parametrize("launch", ["auto", "device"])
+def test_narrow_random_lists(ctx, n, launch):
+    say("narrow random", n, launch)
+    succ, w = o.gen_list(n, n + 7, True)
+    for _ in range(3):  # the third call replays a captured graph
+        root, rank = ctx.rank_list(dev(succ), dev(w), launch=launch)
+        er, ek = o.list_closed_form(n, n + 7, w)
+        assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
+@pytest.mark.parametrize("weight", [1 << 30, -(1 << 30), (1 << 31) - 1, -(1 << 31), 1 << 31, -(1 << 31) - 1])
+def test_narrow_weight_limits(ctx, weight):
+    """Weights at and beyond the int32 limits: the first four fit the narrow
+    records but their claim distances overflow after a step or two (the call
+    is redone with wide records), the last two do not fit at all."""
+    n = 1 << 20
+    say("narrow weights", weight)
+    succ, _ = o.gen_list(n, 3, False)
+    w = np.full(n, weight, np.int64)
+    w[succ == np.arange(n, dtype=np.uint64)] = 0
+    check(ctx, succ, w)
+
+
+def test_narrow_unit_and_signed_weights(ctx):
+    n = 1 << 21
+    say("narrow unit/signed")
+    succ, _ = o.gen_list(n, 5, False)
+    check(ctx, succ, None)
+    rng = np.random.default_rng(5)
+    w = rng.integers(-1000, 1001, size=n).astype(np.int64)
+    w[succ == np.arange(n, dtype=np.uint64)] = 0
+    check(ctx, succ, w)
+
+
+@pytest.mark.parametrize("nlists", [3, 5000, 200000])
+def test_narrow_forests(ctx, nlists):
+    n = (1 << 20) + 3
+    say("narrow forest", nlists)
+    succ, w = random_forest_lists(n, nlists, nlists + 1)
+    check(ctx, succ, w)
+
+
+def test_narrow_spawn_and_spacings(ctx):
+    n = 1 << 21
+    say("narrow spawn")
+    succ, w = o.gen_list(n, 9, True)
+    for spacing, spawn in ((16, "off"), (512, "on"), (4096, "on")):
+        check(ctx, succ, w, ruler_spacing=spacing, spawn=spawn)
+
+
+def test_narrow_host_launch_and_errors(ctx):
+    """Host-sized levels use wide records; errors are reported as on the
+    wide path and the context stays usable."""
+    import kascade_amd
+    n = 1 << 20
+    say("narrow errors")
+    succ, w = o.gen_list(n, 11, True)
+    check(ctx, succ, w, launch="host")
+    cyc = np.roll(np.arange(n, dtype=np.uint64), -1)
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        ctx.rank_list(dev(cyc), None, algorithm="srs")
+    assert e.value.status == kascade_amd.KC_ECYCLE
+    bad = succ.copy()
+    bad[5] = n + 1
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        ctx.rank_list(dev(bad), dev(w))
+    assert e.value.status == kascade_amd.KC_EINVAL
+    wt = w.copy()
+    wt[succ == np.arange(n, dtype=np.uint64)] = 7
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        ctx.rank_list(dev(succ), dev(wt))
+    assert e.value.status == kascade_amd.KC_EINVAL
+    check(ctx, succ, w)
