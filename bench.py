@@ -31,7 +31,10 @@ HBM_PEAK_GBPS = 8000.0
 # was measured in, as the committed profile summary states them.
 CHASE_PROFILE = os.path.join("profiles", "chase_propagate_list_2e27_weighted.md")
 BYTES_PER_ELEMENT = 32  # succ 8 in + weight 8 in + root 8 out + rank 8 out
-CHASE_BYTES_PER_ELEMENT = 32  # chase: read (succ, w) 16 B + write (ruler, dist) 16 B per element
+# The chase is credited with the ranking's compulsory bytes per element (succ
+# and weight in, root and rank out: 32 B), whatever record width it moves them
+# in: 16-byte records, or the narrow 8-byte level-0 records.
+CHASE_BYTES_PER_ELEMENT = 32
 
 
 def parse():
@@ -484,9 +487,14 @@ def main():
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
     cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep)
+    rs0 = ctx.record_stats()
     dt, phases = timed(ctx, lambda: ctx.rank_list(succ, w, root, rank, **cfg), args.steps, args.warmup)
+    rs1 = ctx.record_stats()
     ms_per_step = dt * 1e3
     value = n / dt
+    # level-0 record width of the timed calls (every one narrow, or none)
+    narrow_calls = rs1["narrow_calls"] - rs0["narrow_calls"]
+    record_bytes = 8 if narrow_calls > 0 else 16
 
     # dominant kernel and its roofline (HIP events on the library's stream)
     kernel_phases = {k: v for k, v in phases.items() if k not in ("unclaimed",)}
@@ -529,7 +537,8 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic,
-                     "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes},
+                     "kernel_ms": dom_ms, "algorithmic_bytes": alg_bytes,
+                     "level0_record_bytes": record_bytes},
         "whole_call_effective_gbps": whole_gbps,
         "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
         "launch": ctx.launch_stats(),
@@ -546,13 +555,15 @@ def main():
         lvl0 = out["ruler_trace"][0]
         spawn_on = lvl0.get("spawned", 0) > 0
         matches = (str(lvl0.get("spacing")) == traffic_conf.get("spacing")
-                   and ("on" if spawn_on else "off") == traffic_conf.get("spawn"))
+                   and ("on" if spawn_on else "off") == traffic_conf.get("spawn")
+                   and str(record_bytes) == traffic_conf.get("records"))
         out["roofline"]["traffic_source"] = CHASE_PROFILE
         out["roofline"]["traffic_config_matches"] = matches
         if not matches:
             out["roofline"]["traffic"] = None
             out["roofline"]["traffic_note"] = (f"profile measured {traffic_conf}, this run used spacing "
-                                               f"{lvl0.get('spacing')} spawn {'on' if spawn_on else 'off'}")
+                                               f"{lvl0.get('spacing')} spawn {'on' if spawn_on else 'off'} "
+                                               f"records {record_bytes}")
             traffic = None
     if not args.no_host_buffers:
         out["host_buffers"] = host_buffer_rates(ctx, succ, w, cfg)

@@ -146,6 +146,11 @@ kc_status kc_release_cached_scratch(int device);
  * out3 = {HIP graphs captured, graph replays, device-sized rankings redone
  * host-sized after a capacity overflow}.  No reference counterpart. */
 kc_status kc_context_launch_stats(void *ctx, uint64_t *out3);
+/* Level-0 record width statistics since the context's creation: out2 =
+ * {list rankings whose top level used 8-byte records, such rankings redone
+ * with 16-byte records (a weight or distance outside int32)}.  No reference
+ * counterpart. */
+kc_status kc_context_record_stats(void *ctx, uint64_t *out2);
 
 /* Per-phase device timing (HIP events on the launching stream), the
  * counterpart of the reference's kamping::measurements::timer phases

@@ -103,6 +103,7 @@ _SIGS = {
     "kc_context_trim": (ctypes.c_int, [_P]),
     "kc_release_cached_scratch": (ctypes.c_int, [ctypes.c_int]),
     "kc_context_launch_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "kc_context_record_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "kc_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "kc_profile_reset": (ctypes.c_int, [_P]),
     "kc_profile_read": (ctypes.c_uint32, [_P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_char_p),
@@ -615,6 +616,13 @@ class Context:
         out = (ctypes.c_uint64 * 3)()
         self._check(self._lib.kc_context_launch_stats(self.handle, out))
         return {"graph_captures": out[0], "graph_replays": out[1], "host_redos": out[2]}
+
+    def record_stats(self):
+        """{'narrow_calls', 'narrow_redos'}: list rankings whose top level used
+        8-byte records, and those redone with 16-byte records."""
+        out = (ctypes.c_uint64 * 2)()
+        self._check(self._lib.kc_context_record_stats(self.handle, out))
+        return {"narrow_calls": out[0], "narrow_redos": out[1]}
 
 
 def release_cached_scratch(device=0):

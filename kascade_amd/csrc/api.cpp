@@ -482,6 +482,14 @@ kc_status kc_context_launch_stats(void* vctx, uint64_t* out3) {
   return KC_OK;
 }
 
+kc_status kc_context_record_stats(void* vctx, uint64_t* out2) {
+  Context* ctx = static_cast<Context*>(vctx);
+  if (!ctx || !out2) return KC_EINVAL;
+  out2[0] = ctx->narrow_calls;
+  out2[1] = ctx->narrow_redos;
+  return KC_OK;
+}
+
 kc_status kc_release_cached_scratch(int device) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return KC_EDEVICE;

@@ -172,6 +172,7 @@ struct Context {
   uint64_t graph_replays = 0;
   uint64_t graph_captures = 0;
   uint64_t host_redos = 0;  // device-sized rankings redone host-sized after an overflow
+  uint64_t narrow_calls = 0;  // list rankings whose top level used narrow records
   uint64_t narrow_redos = 0;  // narrow-record rankings redone with wide records
 };
 
@@ -387,6 +388,7 @@ struct ContractOut {
 };
 uint32_t contraction_mode(const kc_rank_cfg* cfg, uint64_t n);
 void contraction_sample(Context* ctx, const Rec* rec, uint64_t n, unsigned long long* out, hipStream_t st);
+void contraction_sample(Context* ctx, const Rec8* rec, uint64_t n, unsigned long long* out, hipStream_t st);
 bool contraction_sample_wins(uint64_t sampled_interior);
 // The same decision from the caller's successor array, before any pack
 // (synchronises `st`): for lists of kContractEarlySample elements or more,

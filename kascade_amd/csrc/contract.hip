@@ -51,6 +51,13 @@ struct ListIn {
   __device__ __forceinline__ int64_t wt(uint64_t v) const { return __builtin_nontemporal_load(w + v); }
 };
 
+// Narrow packed records (successor ids below 2^30, no flags yet): for the
+// AUTO sample only.
+struct Rec8In {
+  const Rec8* rec;
+  __device__ __forceinline__ uint64_t s_neighbour(uint64_t v) const { return rec[v].s; }
+};
+
 // Packed records (pack_records, the segmented pack).
 struct RecIn {
   const Rec* rec;
@@ -373,6 +380,11 @@ uint32_t contraction_mode(const kc_rank_cfg* cfg, uint64_t n) {
 void contraction_sample(Context* ctx, const Rec* rec, uint64_t n, unsigned long long* out, hipStream_t st) {
   (void)ctx;
   hipLaunchKernelGGL(lc_sample_kernel<RecIn>, dim3(kSample / kThreads), dim3(kThreads), 0, st, RecIn{rec}, n, out);
+}
+
+void contraction_sample(Context* ctx, const Rec8* rec, uint64_t n, unsigned long long* out, hipStream_t st) {
+  (void)ctx;
+  hipLaunchKernelGGL(lc_sample_kernel<Rec8In>, dim3(kSample / kThreads), dim3(kThreads), 0, st, Rec8In{rec}, n, out);
 }
 
 bool contraction_sample_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st) {

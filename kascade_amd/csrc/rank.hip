@@ -1572,16 +1572,24 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
     return;
   }
   Scratch scratch(ctx);
-  if (contract == KC_CONTRACT_OFF && narrow_wanted(ctx, n, algo, tune)) {
+  if (contract != KC_CONTRACT_ON && narrow_wanted(ctx, n, algo, tune)) {
     Scratch narrow(ctx);  // released before the wide records are taken
     Rec8* rec8 = narrow.get<Rec8>(n);
     pack_records_narrow(ctx, succ, w, n, rec8, st);
+    if (contract == KC_CONTRACT_AUTO)
+      contraction_sample(ctx, rec8, n, reinterpret_cast<unsigned long long*>(ctx->d_counters) + kSlotWork, st);
     uint64_t c[3] = {0, 0, 0};
-    read_counters(ctx, st, kSlotTails, 3, c);  // tails, -, "a weight does not fit"
+    read_counters(ctx, st, kSlotTails, 3, c);  // tails, sample, "a weight does not fit"
+    if (contract == KC_CONTRACT_AUTO && contraction_sample_wins(c[1])) {
+      rank_contracted_list(ctx, succ, w, n, cfg, algo, out, st);
+      check_err(ctx, st);
+      return;
+    }
     if (c[2] == 0 && narrow_ids_fit(n, c[0])) {
       const uint32_t redo = rank_records_narrow(ctx, rec8, n, root, rank, c[0], tune, st);
       if (redo == 0) {
         check_err(ctx, st);
+        ctx->narrow_calls++;
         return;
       }
       ctx->narrow_redos++;

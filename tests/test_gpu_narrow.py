@@ -26,10 +26,14 @@ def check(ctx, succ, w, **cfg):
 def test_narrow_random_lists(ctx, n, launch):
     say("narrow random", n, launch)
     succ, w = o.gen_list(n, n + 7, True)
+    before = ctx.record_stats()
     for _ in range(3):  # the third call replays a captured graph
         root, rank = ctx.rank_list(dev(succ), dev(w), launch=launch)
         er, ek = o.list_closed_form(n, n + 7, w)
         assert (host(root) == er).all() and (host(rank) == ek).all()
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 3
+    assert after["narrow_redos"] == before["narrow_redos"]
 
 
 @pytest.mark.parametrize("weight", [1 << 30, -(1 << 30), (1 << 31) - 1, -(1 << 31), 1 << 31, -(1 << 31) - 1])
@@ -42,7 +46,12 @@ def test_narrow_weight_limits(ctx, weight):
     succ, _ = o.gen_list(n, 3, False)
     w = np.full(n, weight, np.int64)
     w[succ == np.arange(n, dtype=np.uint64)] = 0
+    before = ctx.record_stats()
     check(ctx, succ, w)
+    after = ctx.record_stats()
+    fits = -(1 << 31) <= weight < (1 << 31)
+    assert after["narrow_redos"] - before["narrow_redos"] == (1 if fits else 0)
+    assert after["narrow_calls"] == before["narrow_calls"]
 
 
 def test_narrow_unit_and_signed_weights(ctx):
