@@ -29,8 +29,8 @@ struct alignas(16) Rec {
 // fit int32.  A dependent random walk that claims every record it reads runs
 // 10 % faster over 8-byte records than over 16-byte ones (2^27 records on
 // MI355X: 5.61 against 6.27 ms), because twice as many of them stay in the
-// caches.  Used for the top level of single-list rankings; anything that
-// does not fit is redone with wide records.
+// caches.  Used for the top level of single-list rankings from 2^22
+// elements; anything that does not fit is redone with wide records.
 struct alignas(8) Rec8 {
   uint32_t s;
   int32_t w;

@@ -1400,9 +1400,11 @@ void pack_records_narrow(Context* ctx, const uint64_t* succ, const int64_t* w, u
 }
 
 // Narrow records apply to a single list region ranked by the ruling set with
-// device-sized levels, from 2^20 elements (below, the pack and the chase are
-// a small part of a call) and below 2^30 (ids, flags in bits 31 and 30).
-constexpr uint64_t kNarrowMin = uint64_t{1} << 20;
+// device-sized levels, below 2^30 elements (ids, flags in bits 31 and 30) and
+// from 2^22 (below, the records fit the caches either way: A/B on MI355X,
+// 2^20 0.353 against 0.338 ms wide, 2^22 0.524 both, 2^24 1.29 against 1.44,
+// 2^26 4.31 against 4.97).
+constexpr uint64_t kNarrowMin = uint64_t{1} << 22;
 bool narrow_wanted(const Context* ctx, uint64_t n, int algo, const RankTuning& tune) {
   return n >= kNarrowMin && n < kNarrowMaxIds && algo == KC_ALGO_SPARSE_RULING_SET &&
          tune.launch != KC_LAUNCH_HOST && !ctx->trace_on && tune.max_levels > 0;

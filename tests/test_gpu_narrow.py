@@ -1,4 +1,4 @@
-"""Narrow (8-byte) level-0 records: single-list rankings from 2^20 elements
+"""Narrow (8-byte) level-0 records: single-list rankings from 2^22 elements
 pack their top level into {u32 successor, i32 weight} records and claim
 with i32 distances.  Weights that do not fit, distances that outgrow int32
 and id spaces beyond 2^30 fall back to 16-byte records; every result is
@@ -21,7 +21,7 @@ def check(ctx, succ, w, **cfg):
     assert (host(rank) == ek).all(), cfg
 
 
-@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 12345, 1 << 22])
+@pytest.mark.parametrize("n", [1 << 22, (1 << 22) + 12345, 1 << 23])
 @pytest.mark.parametrize("launch", ["auto", "device"])
 def test_narrow_random_lists(ctx, n, launch):
     say("narrow random", n, launch)
@@ -41,7 +41,7 @@ def test_narrow_weight_limits(ctx, weight):
     """Weights at and beyond the int32 limits: the first four fit the narrow
     records but their claim distances overflow after a step or two (the call
     is redone with wide records), the last two do not fit at all."""
-    n = 1 << 20
+    n = 1 << 22
     say("narrow weights", weight)
     succ, _ = o.gen_list(n, 3, False)
     w = np.full(n, weight, np.int64)
@@ -55,7 +55,7 @@ def test_narrow_weight_limits(ctx, weight):
 
 
 def test_narrow_unit_and_signed_weights(ctx):
-    n = 1 << 21
+    n = 1 << 22
     say("narrow unit/signed")
     succ, _ = o.gen_list(n, 5, False)
     check(ctx, succ, None)
@@ -67,14 +67,14 @@ def test_narrow_unit_and_signed_weights(ctx):
 
 @pytest.mark.parametrize("nlists", [3, 5000, 200000])
 def test_narrow_forests(ctx, nlists):
-    n = (1 << 20) + 3
+    n = (1 << 22) + 3
     say("narrow forest", nlists)
     succ, w = random_forest_lists(n, nlists, nlists + 1)
     check(ctx, succ, w)
 
 
 def test_narrow_spawn_and_spacings(ctx):
-    n = 1 << 21
+    n = 1 << 22
     say("narrow spawn")
     succ, w = o.gen_list(n, 9, True)
     for spacing, spawn in ((16, "off"), (512, "on"), (4096, "on")):
@@ -85,7 +85,7 @@ def test_narrow_host_launch_and_errors(ctx):
     """Host-sized levels use wide records; errors are reported as on the
     wide path and the context stays usable."""
     import kascade_amd
-    n = 1 << 20
+    n = 1 << 22
     say("narrow errors")
     succ, w = o.gen_list(n, 11, True)
     check(ctx, succ, w, launch="host")
