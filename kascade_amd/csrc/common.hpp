@@ -360,11 +360,22 @@ constexpr uint64_t kAutoPdMaxTree = uint64_t{1} << 17;   // tree vertices
 // Validates cfg->algorithm and resolves KC_ALGO_AUTO for a ranking of n list
 // elements (tree = false) or n tree vertices (tree = true).
 int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree);
-// The narrow form of rank_records for a single list region ranked by the
-// sparse ruling set with device-sized levels: returns the redo bits of
-// check_err_overflow (0: done).
-uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t* root, int64_t* rank,
-                             uint64_t max_tails, const RankTuning& tune, hipStream_t st);
+// The narrow form of rank_records (the sparse ruling set, device-sized
+// levels, no per-job outputs): returns the redo bits of check_err_overflow
+// (0: done).
+uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st, uint32_t out_shift = 0,
+                             const TourRegion* tours = nullptr);
+// Narrow records apply to a top level ranked by the ruling set with
+// device-sized levels, below 2^30 records (ids, flags in bits 31 and 30) and
+// from 2^22 (below, the records fit the caches either way: A/B on MI355X,
+// lists of 2^20 0.353 against 0.338 ms wide, 2^22 0.524 both, 2^24 1.29
+// against 1.44, 2^26 4.31 against 4.97).
+constexpr uint64_t kNarrowMin = uint64_t{1} << 22;
+bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune);
+// The level-0 ids (rulers, tails, spawned rulers) must stay below 2^30 too:
+// rulers are at most records / 2, spawned ones at most a quarter.
+bool narrow_ids_fit(uint64_t records, uint64_t tails);
 // True when rank_records ranks n records with the sparse ruling set at the top
 // level (the only top level that writes through a JobOut).
 bool top_level_is_ruling_set(uint64_t n, const RankTuning& tune, int algorithm);

@@ -29,9 +29,12 @@ namespace {
 
 constexpr unsigned kThreads = 256;
 
+// R: the tour's record width (Rec, or the narrow Rec8 for tours below 2^30
+// records).
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __restrict__ keys,
-                 uint32_t* __restrict__ vals, Rec* __restrict__ tour, uint32_t* err,
+                 uint32_t* __restrict__ vals, R* __restrict__ tour, uint32_t* err,
                  unsigned long long* roots) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool is_root = false;
@@ -45,18 +48,19 @@ tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __re
     keys[v] = is_root ? static_cast<uint32_t>(n) : static_cast<uint32_t>(p);
     vals[v] = static_cast<uint32_t>(v);
     if (is_root) {
-      tour[2 * v] = Rec{2 * v, 0};          // terminal dummy
-      tour[2 * v + 1] = Rec{2 * v + 1, 0};  // unused slot
+      tour[2 * v] = RecOps<R>::encode(2 * v, 0);          // terminal dummy
+      tour[2 * v + 1] = RecOps<R>::encode(2 * v + 1, 0);  // unused slot
     } else {
-      tour[2 * v + 1] = Rec{2 * v, -1};     // leaf default: down(v) -> up(v)
+      tour[2 * v + 1] = RecOps<R>::encode(2 * v, -1);     // leaf default: down(v) -> up(v)
     }
   }
   wave_append(roots, is_root);
 }
 
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                 const uint64_t* __restrict__ parent, uint64_t n, Rec* __restrict__ tour,
+                 const uint64_t* __restrict__ parent, uint64_t n, R* __restrict__ tour,
                  uint64_t* __restrict__ heads, unsigned long long* nheads) {
   uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool head = false;
@@ -68,12 +72,12 @@ tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__
       const bool first = (j == 0) || keys[j - 1] != p;
       const bool last = (j + 1 == n) || keys[j + 1] != p;
       // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
-      tour[2 * uint64_t{c}] = Rec{last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1};
+      tour[2 * uint64_t{c}] = RecOps<R>::encode(last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1);
       // down(p) -> down(first child); a root p has no down edge, and its
       // first child's down edge is where its tour starts
       if (first) {
         if (parent[p] != p)
-          tour[2 * uint64_t{p} + 1] = Rec{2 * uint64_t{c} + 1, -1};
+          tour[2 * uint64_t{p} + 1] = RecOps<R>::encode(2 * uint64_t{c} + 1, -1);
         else
           head = true;
       }
@@ -105,9 +109,10 @@ tour_link_atomic_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_
 //   up(v)   -> down(sibling inserted before v), or up(p) / p's dummy
 //   down(v) -> down(head child of v), or up(v) for a leaf
 // and for a root the terminal dummy plus the unused slot.
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 tour_pairs_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_t* __restrict__ head,
-                  const uint32_t* __restrict__ sib, Rec* __restrict__ tour,
+                  const uint32_t* __restrict__ sib, R* __restrict__ tour,
                   uint64_t* __restrict__ heads, unsigned long long* nheads) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool tour_head = false;
@@ -125,8 +130,8 @@ tour_pairs_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_
       up = Rec{s != 0xFFFFFFFFu ? 2 * uint64_t{s} + 1 : 2 * p, 1};
       down = Rec{h != 0xFFFFFFFFu ? 2 * uint64_t{h} + 1 : 2 * v, -1};
     }
-    tour[2 * v] = up;
-    tour[2 * v + 1] = down;
+    tour[2 * v] = RecOps<R>::encode(up.s, up.w);
+    tour[2 * v + 1] = RecOps<R>::encode(down.s, down.w);
   }
   const uint64_t k = wave_append(nheads, tour_head);
   if (tour_head) heads[k] = 2 * uint64_t{h} + 1;
@@ -152,7 +157,8 @@ unsigned bits_for(uint64_t x) {
 // never do).
 // heads (capacity n) receives the element where each rooted tour starts (the
 // down edge of the root's first child); *nheads their number.
-static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec* tour, bool sorted,
+template <class R>
+static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, R* tour, bool sorted,
                            uint64_t* heads, uint64_t* nheads, hipStream_t st) {
   if (n >= (uint64_t{1} << 32) - 1) throw KcError{KC_EINVAL, "tree rooting supports n < 2^32 - 1"};
   Phase ph(ctx, "tour_build", st);
@@ -167,7 +173,7 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
                        0xFFFFFFFFu);
     hipLaunchKernelGGL(tour_link_atomic_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st,
                        parent, n, head, sib, ctx->d_err, ctr);
-    hipLaunchKernelGGL(tour_pairs_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
+    hipLaunchKernelGGL(tour_pairs_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent, n,
                        head, sib, tour, heads, ctr + 1);
     read_counters(ctx, st, 0, 2, counts);
     *nheads = counts[1];
@@ -176,10 +182,10 @@ static uint64_t build_tour(Context* ctx, const uint64_t* parent, uint64_t n, Rec
   uint32_t* keys = scratch.get<uint32_t>(n);
   uint32_t* vals = scratch.get<uint32_t>(n);
   KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 2 * sizeof(uint64_t), st));
-  hipLaunchKernelGGL(tree_keys_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent,
+  hipLaunchKernelGGL(tree_keys_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, parent,
                      n, keys, vals, tour, ctx->d_err, ctr);
   radix_sort_pairs_u32(ctx, keys, vals, n, bits_for(n), st);
-  hipLaunchKernelGGL(tour_link_kernel, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, keys,
+  hipLaunchKernelGGL(tour_link_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, keys,
                      vals, parent, n, tour, heads, ctr + 1);
   read_counters(ctx, st, 0, 2, counts);
   *nheads = counts[1];
@@ -342,12 +348,30 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   if (n == 0) return;
   check_records(2 * n);
   Scratch scratch(ctx);
-  Rec* tour = scratch.get<Rec>(2 * n);
   uint64_t* heads = scratch.get<uint64_t>(n);
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
   RankTuning tune = tuning_for(ctx, n, rc, /*tree=*/true);
   const int algo = resolve_algorithm(rc, n, true);
+  if (narrow_wanted(ctx, 2 * n, algo, tune)) {
+    // an 8-byte tour (weights +-1 and distances below 2n always fit)
+    Scratch narrow(ctx);
+    Rec8* tour8 = narrow.get<Rec8>(2 * n);
+    uint64_t nheads = 0;
+    const uint64_t roots = build_tour(ctx, parent, n, tour8, sorted, heads, &nheads, st);
+    if (narrow_ids_fit(2 * n, 2 * roots)) {
+      const TourRegion tr{0, heads, nheads};
+      const uint32_t redo = rank_records_narrow(ctx, tour8, 2 * n, n, root, depth, 2 * roots, tune, st, 1, &tr);
+      if (redo == 0) {
+        check_err(ctx, st);
+        ctx->narrow_calls++;
+        return;
+      }
+      ctx->narrow_redos++;
+      if (redo & kErrOverflow) tune.launch = KC_LAUNCH_HOST;
+    }
+  }
+  Rec* tour = scratch.get<Rec>(2 * n);
   for (;;) {
     uint64_t nheads = 0;
     const uint64_t roots = build_tour(ctx, parent, n, tour, sorted, heads, &nheads, st);

@@ -1291,10 +1291,10 @@ void ds_rank(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
              hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   if (n == 0) return;
   if constexpr (RecOps<R>::kNarrow) {
-    // a narrow top level is always a ruling-set level over a single list region
+    // a narrow top level is always a ruling-set level (rank_records_narrow)
     (void)algorithm;
-    (void)tours;
-    ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st, nullptr, jo);
+    ds_srs_level(ctx, rec, n, n_dev, n_out, root, rank, om, max_tails, tune, level, st,
+                 tours && tours->lo < n ? tours : nullptr, jo);
   } else {
     if (n <= tune.small_n && n <= kSmallMax) {
       Phase ph(ctx, "small_rank", st);
@@ -1399,21 +1399,16 @@ void pack_records_narrow(Context* ctx, const uint64_t* succ, const int64_t* w, u
                      ctx->d_err, ctr + kSlotTails, ctr + kSlotExtra);
 }
 
-// Narrow records apply to a single list region ranked by the ruling set with
-// device-sized levels, below 2^30 elements (ids, flags in bits 31 and 30) and
-// from 2^22 (below, the records fit the caches either way: A/B on MI355X,
-// 2^20 0.353 against 0.338 ms wide, 2^22 0.524 both, 2^24 1.29 against 1.44,
-// 2^26 4.31 against 4.97).
-constexpr uint64_t kNarrowMin = uint64_t{1} << 22;
-bool narrow_wanted(const Context* ctx, uint64_t n, int algo, const RankTuning& tune) {
-  return n >= kNarrowMin && n < kNarrowMaxIds && algo == KC_ALGO_SPARSE_RULING_SET &&
+}  // namespace
+
+bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune) {
+  return records >= kNarrowMin && records < kNarrowMaxIds && algo == KC_ALGO_SPARSE_RULING_SET &&
          tune.launch != KC_LAUNCH_HOST && !ctx->trace_on && tune.max_levels > 0;
 }
-// The level-0 ids (rulers, tails, spawned rulers) must stay below 2^30 too:
-// rulers are at most n / 2, spawned ones at most a quarter of n.
-bool narrow_ids_fit(uint64_t n, uint64_t tails) { return n / 2 + n / 4 + tails < kNarrowMaxIds; }
 
-}  // namespace
+bool narrow_ids_fit(uint64_t records, uint64_t tails) {
+  return records / 2 + records / 4 + tails < kNarrowMaxIds;
+}
 
 namespace {
 
@@ -1549,10 +1544,11 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
                         jobs_out) == 0;
 }
 
-uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t* root, int64_t* rank,
-                             uint64_t max_tails, const RankTuning& tune, hipStream_t st) {
-  return rank_records_t(ctx, rec, n, n, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, 0, nullptr,
-                        nullptr);
+uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st, uint32_t out_shift,
+                             const TourRegion* tours) {
+  return rank_records_t(ctx, rec, n, n_out, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, out_shift,
+                        tours, nullptr);
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,
@@ -1588,7 +1584,7 @@ void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint
       return;
     }
     if (c[2] == 0 && narrow_ids_fit(n, c[0])) {
-      const uint32_t redo = rank_records_narrow(ctx, rec8, n, root, rank, c[0], tune, st);
+      const uint32_t redo = rank_records_narrow(ctx, rec8, n, n, root, rank, c[0], tune, st);
       if (redo == 0) {
         check_err(ctx, st);
         ctx->narrow_calls++;

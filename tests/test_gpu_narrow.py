@@ -104,3 +104,69 @@ def test_narrow_host_launch_and_errors(ctx):
         ctx.rank_list(dev(succ), dev(wt))
     assert e.value.status == kascade_amd.KC_EINVAL
     check(ctx, succ, w)
+
+
+def fast_forest(n, trees, seed):
+    """A forest of `trees` random recursive trees (each vertex's parent is
+    uniform among the earlier vertices of its tree), relabelled by a seeded
+    permutation; vectorised for large n."""
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.choice(np.arange(1, n), size=trees - 1, replace=False))
+    start = np.zeros(n, np.int64)
+    start[cuts] = cuts
+    start = np.maximum.accumulate(start)
+    v = np.arange(n, dtype=np.int64)
+    parent = start + np.floor(rng.random(n) * (v - start)).astype(np.int64)
+    parent[v == start] = v[v == start]
+    sigma = rng.permutation(n)
+    out = np.empty(n, np.uint64)
+    out[sigma] = sigma[parent]
+    return out
+
+
+@pytest.mark.parametrize("build", ["linked", "sorted"])
+@pytest.mark.parametrize("n", [1 << 21, (1 << 22) + 7])
+def test_narrow_trees(ctx, build, n):
+    """Tree rooting from 2^21 vertices ranks an 8-byte tour."""
+    say("narrow tree", build, n)
+    parent = ctx.gen_tree(n, 31)
+    before = ctx.record_stats()
+    root, depth = ctx.root_tree(parent, tour_build=build)
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 1
+    er, ed = o.tree_closed_form(n, 31)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+@pytest.mark.parametrize("trees", [7, 3000])
+def test_narrow_tree_forests(ctx, trees):
+    n = (1 << 21) + 5
+    say("narrow tree forest", trees)
+    parent = fast_forest(n, trees, trees)
+    before = ctx.record_stats()
+    for build in ("linked", "sorted"):
+        root, depth = ctx.root_tree(dev(parent), tour_build=build)
+        er, ed = o.euler_tour_rooting(parent)
+        assert (host(root) == er).all() and (host(depth) == ed).all(), build
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 2
+    assert after["narrow_redos"] == before["narrow_redos"]
+
+
+def test_narrow_tree_cycle(ctx):
+    """A 2-cycle (a root made the child of its own child) beside 19 proper
+    trees fails with KC_ECYCLE, and the context ranks a tree afterwards."""
+    import kascade_amd
+    n = 1 << 21
+    say("narrow tree cycle")
+    parent = fast_forest(n, 20, 3)
+    v = np.arange(n, dtype=np.uint64)
+    r = int(np.nonzero(parent == v)[0][0])
+    parent[r] = int(np.nonzero((parent == r) & (v != r))[0][0])
+    with pytest.raises(kascade_amd.KascadeError) as e:
+        ctx.root_tree(dev(parent))
+    assert e.value.status == kascade_amd.KC_ECYCLE
+    parent = fast_forest(n, 20, 4)
+    root, depth = ctx.root_tree(dev(parent))
+    er, ed = o.euler_tour_rooting(parent)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
