@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--algorithm", default="srs")
     ap.add_argument("--spacing", type=int, default=0)
     ap.add_argument("--spacing-deep", type=int, default=0)
+    ap.add_argument("--spawn", choices=["auto", "off", "on"], default="auto", help="ruler spawn (kc_spawn)")
+    ap.add_argument("--chase-blocks", type=int, default=0,
+                    help="resident chase workgroups per CU (0: the tuning table's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1 << 25,
                     help="elements in the CPU baseline's sample (trees: a quarter as many vertices); "
@@ -486,7 +489,8 @@ def main():
     succ, w = ctx.gen_list(n, args.seed, weighted=True)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
-    cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep)
+    cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
+               chase_blocks_per_cu=args.chase_blocks, spawn=args.spawn)
     rs0 = ctx.record_stats()
     dt, phases = timed(ctx, lambda: ctx.rank_list(succ, w, root, rank, **cfg), args.steps, args.warmup)
     rs1 = ctx.record_stats()
