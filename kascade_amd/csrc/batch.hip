@@ -72,9 +72,13 @@ struct JobCursor {
   }
 };
 
+// R: Rec, or Rec8 for a narrow batch (a weight outside int32 then sets *wide
+// and the batch is packed again with wide records).
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nl,
-                       Rec* __restrict__ rec, uint32_t* err, unsigned long long* tails) {
+                       R* __restrict__ rec, uint32_t* err, unsigned long long* tails,
+                       unsigned long long* wide) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nl), s_range);
   JobCursor jc;
@@ -96,10 +100,11 @@ batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
       if (j.w) {
         w = j.w[i];
         if (tail && w != 0) raise_err(err, kErrRootWeight);
+        if (!RecOps<R>::fits(w)) *wide = 1;
       } else {
         w = tail ? 0 : 1;
       }
-      rec[e] = Rec{j.base + s, w};
+      rec[e] = RecOps<R>::encode(j.base + s, w);
     }
     wave_append(tails, tail);
   }
@@ -194,10 +199,11 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
 }
 
 // Both tour elements of tree vertex t as one 32-byte pair (see tour_pairs_kernel).
+template <class R>
 __global__ void __launch_bounds__(kThreads)
 batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64_t nt, uint64_t nl,
                         const uint32_t* __restrict__ head, const uint32_t* __restrict__ sib,
-                        Rec* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
+                        R* __restrict__ rec, uint64_t* __restrict__ heads, unsigned long long* nheads) {
   __shared__ uint32_t s_range[2];
   block_job_range(jobs, njobs, chunk_first(), chunk_last(nt), s_range);
   JobCursor jc;
@@ -222,8 +228,8 @@ batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint6
         up = Rec{sb != 0xFFFFFFFFu ? nl + 2 * uint64_t{sb} + 1 : nl + 2 * (j.base + p), 1};
         down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
       }
-      rec[nl + 2 * t] = up;
-      rec[nl + 2 * t + 1] = down;
+      rec[nl + 2 * t] = RecOps<R>::encode(up.s, up.w);
+      rec[nl + 2 * t + 1] = RecOps<R>::encode(down.s, down.w);
     }
     const uint64_t k = wave_append(nheads, tour_head);
     if (tour_head) heads[k] = nl + 2 * uint64_t{h} + 1;
@@ -297,7 +303,6 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
   // With the ruling set at the top level the ranking writes every job's
   // results in place; otherwise into batch-wide arrays that are scattered.
   const bool in_place = top_level_is_ruling_set(total, tune, algo);
-  Rec* rec = scratch.get<Rec>(total);
   uint64_t* root = in_place ? nullptr : scratch.get<uint64_t>(total);
   int64_t* rank = in_place ? nullptr : scratch.get<int64_t>(total);
   BatchJob* d_lists = lists.empty() ? nullptr : scratch.get<BatchJob>(lists.size());
@@ -339,20 +344,57 @@ void rank_batch_device(Context* ctx, const std::vector<BatchJob>& jobs_in, const
       jo.tree_block = tb;
     }
   }
-  for (;;) {
-    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
+  // Narrow (8-byte) records first, as for single lists: every id of a batch
+  // fits 30 bits; a list weight outside int32, or a claim distance that
+  // outgrows it, sends the batch to the wide records below.
+  if (in_place && linked && narrow_wanted(ctx, total, algo, tune)) {
+    Scratch narrow(ctx);  // released before the wide records are taken
+    Rec8* rec8 = narrow.get<Rec8>(total);
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(uint64_t), st));
     if (nl) {
       Phase ph(ctx, "batch_pack", st);
-      hipLaunchKernelGGL(batch_list_pack_kernel, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st,
-                         d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr);
+      hipLaunchKernelGGL(batch_list_pack_kernel<Rec8>, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st,
+                         d_lists, static_cast<uint32_t>(lists.size()), nl, rec8, ctx->d_err, ctr, ctr + 3);
+    }
+    if (nt) {
+      Phase ph(ctx, "batch_tour", st);
+      KC_HIP(hipMemsetAsync(head, 0xFF, nt * sizeof(uint32_t), st));  // no child yet
+      hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
+                         d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
+      hipLaunchKernelGGL(batch_tree_pairs_kernel<Rec8>, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0,
+                         st, d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec8, heads, ctr + 2);
+    }
+    uint64_t counts[4] = {0, 0, 0, 0};
+    read_counters(ctx, st, 0, 4, counts);  // tails, roots, tour heads, a weight outside int32
+    const uint64_t tails = counts[0] + 2 * counts[1];
+    if (counts[3] == 0 && narrow_ids_fit(total, tails)) {
+      const TourRegion tr{nl, heads, counts[2]};
+      const uint32_t redo = rank_records_narrow(ctx, rec8, total, total, nullptr, nullptr, tails, tune, st, 0,
+                                                nt ? &tr : nullptr, &jo);
+      if (redo == 0) {
+        check_err(ctx, st);
+        ctx->narrow_calls++;
+        return;
+      }
+      ctx->narrow_redos++;
+      if (redo & kErrOverflow) tune.launch = KC_LAUNCH_HOST;  // and wide
+    }
+  }
+  Rec* rec = scratch.get<Rec>(total);
+  for (;;) {
+    KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(uint64_t), st));
+    if (nl) {
+      Phase ph(ctx, "batch_pack", st);
+      hipLaunchKernelGGL(batch_list_pack_kernel<Rec>, dim3(blocks_for(nl, kPer * kThreads)), dim3(kThreads), 0, st,
+                         d_lists, static_cast<uint32_t>(lists.size()), nl, rec, ctx->d_err, ctr, ctr + 3);
     }
     if (nt && linked) {
       Phase ph(ctx, "batch_tour", st);
       KC_HIP(hipMemsetAsync(head, 0xFF, nt * sizeof(uint32_t), st));  // no child yet
       hipLaunchKernelGGL(batch_tree_link_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
                          d_trees, static_cast<uint32_t>(trees.size()), nt, head, sib, ctx->d_err, ctr + 1);
-      hipLaunchKernelGGL(batch_tree_pairs_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,
-                         d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
+      hipLaunchKernelGGL(batch_tree_pairs_kernel<Rec>, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0,
+                         st, d_trees, static_cast<uint32_t>(trees.size()), nt, nl, head, sib, rec, heads, ctr + 2);
     } else if (nt) {
       Phase ph(ctx, "batch_tour", st);
       hipLaunchKernelGGL(batch_tree_keys_kernel, dim3(blocks_for(nt, kPer * kThreads)), dim3(kThreads), 0, st,

@@ -365,7 +365,7 @@ int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree);
 // (0: done).
 uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
                              uint64_t max_tails, const RankTuning& tune, hipStream_t st, uint32_t out_shift = 0,
-                             const TourRegion* tours = nullptr);
+                             const TourRegion* tours = nullptr, const JobOut* jobs_out = nullptr);
 // Narrow records apply to a top level ranked by the ruling set with
 // device-sized levels, below 2^30 records (ids, flags in bits 31 and 30) and
 // from 2^22 (below, the records fit the caches either way: A/B on MI355X,

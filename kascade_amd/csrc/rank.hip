@@ -1546,9 +1546,9 @@ bool rank_records(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* 
 
 uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
                              uint64_t max_tails, const RankTuning& tune, hipStream_t st, uint32_t out_shift,
-                             const TourRegion* tours) {
+                             const TourRegion* tours, const JobOut* jobs_out) {
   return rank_records_t(ctx, rec, n, n_out, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, out_shift,
-                        tours, nullptr);
+                        tours, jobs_out);
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,

@@ -219,3 +219,50 @@ def test_server_concurrent_submitters(ctx):
         cr, ck = o.sequential_walk(succ, w) if kind == "list" else o.euler_tour_rooting(succ)
         assert (host(ro) == cr).all() and (host(ra) == ck).all()
     sv.close()
+
+
+@pytest.mark.parametrize("weights", ["small", "distance_overflow", "beyond_int32"])
+def test_server_narrow_batches(ctx, weights):
+    """Batches of 2^22 records and more pack 8-byte records (ids, weights and
+    claim distances in 32 bits).  A weight outside int32 in any list job, or
+    claim distances that outgrow int32, must send the batch to 16-byte
+    records with every result unchanged."""
+    import torch
+    import kascade_amd
+    say("server narrow batches", weights)
+    rng = np.random.default_rng(17)
+    jobs = []
+    for k in range(14):
+        if k % 3 == 2:
+            n = int(2 ** rng.uniform(18, 19.5))
+            jobs.append(("tree", o.gen_tree(n, 3000 + k), None))
+            continue
+        n = int(2 ** rng.uniform(19, 20.5))
+        succ, w = o.gen_list(n, 4000 + k, True)
+        if weights == "distance_overflow":
+            w = np.where(w != 0, np.int64(1 << 29), np.int64(0))
+        elif weights == "beyond_int32" and k == 4:
+            w = np.where(w != 0, np.int64(1 << 33), np.int64(0))
+        jobs.append(("list", succ, w))
+    sv = kascade_amd.Server(ctx, num_slots=1)
+    arrays = []
+    for kind, succ, w in jobs:
+        s = dev(succ)
+        ww = dev(w) if w is not None else None
+        ro = torch.empty(len(succ), dtype=torch.uint64, device="cuda")
+        ra = torch.empty(len(succ), dtype=torch.int64, device="cuda")
+        arrays.append((s, ww, ro, ra))
+    torch.cuda.synchronize()
+    tickets = [sv.submit(kind, s, ww, ro, ra)[0] for (kind, _, _), (s, ww, ro, ra) in zip(jobs, arrays)]
+    for t in tickets:
+        sv.wait(t)
+    st = sv.stats()
+    assert st["jobs_completed"] == len(jobs)
+    assert st["batches_launched"] < len(jobs)
+    for (kind, succ, w), (s, ww, ro, ra) in zip(jobs, arrays):
+        if kind == "list":
+            cr, ck = o.sequential_walk(succ, w)
+        else:
+            cr, ck = o.euler_tour_rooting(succ)
+        assert (host(ro) == cr).all() and (host(ra) == ck).all(), kind
+    sv.close()
