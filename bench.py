@@ -335,7 +335,9 @@ def tree_main(args, ctx):
     parent = ctx.gen_tree(n, args.seed)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     depth = torch.empty(n, dtype=torch.int64, device="cuda")
-    t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth, tour_build=args.tour_build), args.steps,
+    tcfg = dict(spawn=args.spawn, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
+                chase_blocks_per_cu=args.chase_blocks)
+    t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth, tour_build=args.tour_build, **tcfg), args.steps,
                       args.warmup)
     out = {"metric": "vertices rooted/s", "value": n / t, "unit": "vertices/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3, "higher_is_better": True,

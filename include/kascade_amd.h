@@ -119,12 +119,14 @@ typedef enum kc_contract {
  * new rulers are started at unreached elements (candidates at a hashed place
  * in every block of ids, finer than the rulers) and everything resumes, so
  * the longest remaining gaps are split instead of walked to the end by a few
- * lanes.  List regions only (Euler tours start a chase at every tour head
- * instead).  Results are identical either way. */
+ * lanes.  In an Euler tour the candidates are upward edges (the tour
+ * elements that carry claims); AUTO leaves spawn off for tours (2^26-vertex
+ * tree on MI355X: 8.58 ms with spawn against 8.38 without at the tuned 128
+ * elements per ruler).  Results are identical either way. */
 typedef enum kc_spawn {
-  KC_SPAWN_AUTO = 0, /* on for list levels of at least 2^20 elements and 256 or more per ruler */
+  KC_SPAWN_AUTO = 0, /* on for list levels (no tours) of at least 2^20 elements and 256 or more per ruler */
   KC_SPAWN_OFF = 1,
-  KC_SPAWN_ON = 2    /* on every list level with more than 8 elements per ruler */
+  KC_SPAWN_ON = 2    /* on every level with more than 8 elements per ruler, tours included */
 } kc_spawn;
 
 /* ---- context ------------------------------------------------------------ */

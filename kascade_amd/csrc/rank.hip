@@ -382,13 +382,17 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
 
 // Step 2 of the spawn: one thread per candidate block.  An unreached
 // candidate (not a computed ruler, record unclaimed) becomes spawned ruler j
-// while spawn slots last; its chase starts at its successor.
+// while spawn slots last; its chase starts at its successor.  In a tour
+// region only upward edges (even offsets from tour_lo) carry claims, so a
+// candidate there is moved to the upward edge of its vertex: a downward
+// edge's record stays unclaimed after a chase has passed it.
 template <class R>
 __global__ void __launch_bounds__(kThreads)
 spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, uint64_t nF,
                   uint32_t logC, uint32_t cseed, uint64_t cap, uint64_t budget, Rec* __restrict__ sub,
                   uint64_t* __restrict__ sub_orig, unsigned long long* extra_ctr, unsigned long long* spawn_ctr,
-                  SpawnArgs sp, const uint64_t* __restrict__ n_dev, const uint32_t* __restrict__ err) {
+                  SpawnArgs sp, const uint64_t* __restrict__ n_dev, const uint32_t* __restrict__ err,
+                  uint64_t tour_lo) {
   if (*err) return;
   if (n_dev) {
     n = *n_dev < n ? *n_dev : n;
@@ -401,7 +405,18 @@ spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed,
   Rec r{0, 0};
   if (b < nC) {
     c = ruler_of_block(b, n, logC, cseed);
-    if (!is_ruler(c, n, logK, seed)) {
+    bool ok = true;
+    if (c >= tour_lo && ((c - tour_lo) & 1)) {
+      // a downward edge: take the upward edge of the same vertex (c - 1), or
+      // the next one when c - 1 lies in the previous candidate block
+      if (c > (b << logC)) {
+        c--;
+      } else {
+        c++;
+        ok = c < n && c < ((b + 1) << logC);
+      }
+    }
+    if (ok && !is_ruler(c, n, logK, seed)) {
       r = RecOps<R>::decode(rec[c]);
       want = !(r.s & kSpawnBits);
     }
@@ -951,8 +966,12 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
 // two extra launches cost more than they save (2^27 elements, 6 runs each:
 // K = 128 chase 7.15 ms without spawn, 7.37-7.50 with it; K = 256 7.96
 // without, 7.49-7.54 with it, whose 4 MB rank table then makes propagation
-// 0.45 ms shorter than at K = 128).  KC_SPAWN_ON forces it on any list level
-// whose K exceeds 8.
+// 0.45 ms shorter than at K = 128).  KC_SPAWN_ON forces it on any level
+// whose K exceeds 8, tour levels included (candidates on upward edges); AUTO
+// keeps it off for tours, whose chase has no long tail at the tuned K = 128
+// (2^26-vertex tree: level-0 chase 4.14 ms with or without spawn, the whole
+// call 8.58 against 8.38 ms; at K = 256 spawn helps, 4.50 against 4.97 ms,
+// but K = 128 without it stays faster).
 constexpr uint32_t kSpawnShift = 3;    // candidates per ruler: 2^kSpawnShift
 constexpr uint64_t kSpawnActive = 16;  // suspend when fewer than lanes / this still chase
 struct SpawnPlan {
@@ -964,8 +983,8 @@ struct SpawnPlan {
 };
 SpawnPlan spawn_plan(const RankTuning& tune, uint64_t n, uint32_t logK, bool has_tours, unsigned chase_blocks) {
   SpawnPlan p;
-  if (has_tours || tune.spawn == KC_SPAWN_OFF || logK <= kSpawnShift) return p;
-  if (tune.spawn != KC_SPAWN_ON && (n < (uint64_t{1} << 20) || logK < 8)) return p;
+  if (tune.spawn == KC_SPAWN_OFF || logK <= kSpawnShift) return p;
+  if (tune.spawn != KC_SPAWN_ON && (has_tours || n < (uint64_t{1} << 20) || logK < 8)) return p;
   p.on = true;
   p.logC = logK - kSpawnShift;
   p.budget = std::max<uint64_t>(((n + (uint64_t{1} << logK) - 1) >> logK) / 2, 64);
@@ -1015,7 +1034,7 @@ void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trac
   const uint64_t nC = (n + (uint64_t{1} << sp.logC) - 1) >> sp.logC;
   hipLaunchKernelGGL(spawn_mark_kernel<R>, dim3(blocks_for(nC, kThreads)), dim3(kThreads), 0, st, rec, n, logK, seed,
                      nR, nF, sp.logC, seed ^ 0x5bd1e995u, cap, sp.budget, sub, sub_orig, cs.extra, cs.spawned, sa,
-                     n_dev, ctx->d_err);
+                     n_dev, ctx->d_err, tour_lo);
   auto k3 = trace ? chase_kernel<true, kChaseResume, R> : chase_kernel<false, kChaseResume, R>;
   hipLaunchKernelGGL(k3, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
                      cap, sub, sub_orig, cs.resume_work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);

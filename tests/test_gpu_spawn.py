@@ -117,3 +117,31 @@ def test_spawn_errors(ctx):
     root, rank = ctx.rank_list(dev(succ), dev(w), ruler_spacing=512, spawn="on")
     er, ek = o.list_closed_form(n, 8, w)
     assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
+@pytest.mark.parametrize("n", [100003, 1 << 21, (1 << 22) + 5])
+@pytest.mark.parametrize("spacing", [64, 256, 2048])
+@pytest.mark.parametrize("launch", ["auto", "host"])
+def test_spawn_trees(ctx, n, spacing, launch):
+    """Spawn on Euler tours: candidates are moved onto upward edges (the only
+    tour elements that carry claims).  Wide and narrow tours, linked and
+    sorted builds, against the closed form."""
+    say("spawn trees", n, spacing, launch)
+    parent = o.gen_tree(n, 7 + spacing)
+    er, ed = o.tree_closed_form(n, 7 + spacing)
+    for build in ("linked", "sorted"):
+        root, depth = ctx.root_tree(dev(parent), tour_build=build, ruler_spacing=spacing, spawn="on",
+                                    launch=launch)
+        assert (host(root) == er).all() and (host(depth) == ed).all(), build
+
+
+@pytest.mark.parametrize("trees", [2, 3000])
+def test_spawn_tree_forests(ctx, trees):
+    from test_gpu_narrow import fast_forest
+    n = (1 << 21) + 17
+    say("spawn tree forests", trees)
+    parent = fast_forest(n, trees, trees + 5)
+    er, ed = o.euler_tour_rooting(parent)
+    for spacing in (0, 128, 1024):
+        root, depth = ctx.root_tree(dev(parent), ruler_spacing=spacing, spawn="on")
+        assert (host(root) == er).all() and (host(depth) == ed).all(), spacing
