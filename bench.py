@@ -406,7 +406,9 @@ def segmented_main(args, ctx):
                                       weighted=False)
     root = torch.empty(total, dtype=torch.uint32, device="cuda")
     rank = torch.empty(total, dtype=torch.int64, device="cuda")
-    t, phases = timed(ctx, lambda: ctx.rank_lists_segmented(succ, d_off, None, root, rank), args.steps,
+    scfg = dict(spawn=args.spawn, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
+                chase_blocks_per_cu=args.chase_blocks)
+    t, phases = timed(ctx, lambda: ctx.rank_lists_segmented(succ, d_off, None, root, rank, **scfg), args.steps,
                       args.warmup)
     out = {"metric": "elements ranked/s (segmented batch)", "value": total / t, "unit": "elements/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,

@@ -294,7 +294,9 @@ kc_status kc_unpack(void *ctx, const uint64_t *psucc, const int64_t *prank,
  * 256 MB Infinity Cache (a chase ends at every segment's tail, so short
  * segments need no denser rulers of their own; on 20,000 lists of
  * 2^8..2^20 elements spacings 32 / 64 / 128 / 256 measured 124 / 106 / 116 /
- * 130 ms on MI355X).  cfg->local_contraction applies to the batch as a
+ * 130 ms on MI355X).  Batches below 2^30 elements pack 8-byte records
+ * (ids and weights in 32 bits; wider weights or distances fall back to
+ * 16-byte records).  cfg->local_contraction applies to the batch as a
  * whole. */
 kc_status kc_rank_lists_segmented(void *ctx, const uint32_t *succ_local, const uint64_t *seg_off,
                                   uint64_t nseg, const int64_t *weight, uint32_t *root_local,

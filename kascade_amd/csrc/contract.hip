@@ -379,15 +379,18 @@ uint32_t contraction_mode(const kc_rank_cfg* cfg, uint64_t n) {
 
 void contraction_sample(Context* ctx, const Rec* rec, uint64_t n, unsigned long long* out, hipStream_t st) {
   (void)ctx;
+  if (n < kSample) return;  // the strides would be empty (the count stays 0: no contraction)
   hipLaunchKernelGGL(lc_sample_kernel<RecIn>, dim3(kSample / kThreads), dim3(kThreads), 0, st, RecIn{rec}, n, out);
 }
 
 void contraction_sample(Context* ctx, const Rec8* rec, uint64_t n, unsigned long long* out, hipStream_t st) {
   (void)ctx;
+  if (n < kSample) return;  // the strides would be empty (the count stays 0: no contraction)
   hipLaunchKernelGGL(lc_sample_kernel<Rec8In>, dim3(kSample / kThreads), dim3(kThreads), 0, st, Rec8In{rec}, n, out);
 }
 
 bool contraction_sample_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st) {
+  if (n < kSample) return false;  // the strides would be empty
   unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
   KC_HIP(hipMemsetAsync(ctx->d_counters, 0, sizeof(uint64_t), st));
   hipLaunchKernelGGL(lc_sample_kernel<ListIn>, dim3(kSample / kThreads), dim3(kThreads), 0, st, ListIn{succ, nullptr},

@@ -937,12 +937,14 @@ constexpr uint64_t kMallBytes = uint64_t{256} << 20;
 // stay within about an eighth of the level.  The table's spacings above 128
 // (sizes beyond 2^27) were tuned on single lists.  A top level of more than
 // 64 lists takes its rulers in id order, so the chases in flight (`lanes` of
-// them, K ids apart) work in a window of about lanes x K records, and the
-// lists inside that window are all they touch: the spacing is capped so the
-// window fits the MALL (20,000 lists, 2.5e9 elements, 3 chase workgroups per
-// CU: 116 ms at K = 128 (a 402 MB window), 106 ms at 64 (201 MB), 124 ms at
-// 32; with 2 workgroups per CU 106.6 ms at 128 (268 MB)).
-uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails, uint64_t lanes) {
+// them, K ids apart) work in a window of about lanes x K records of
+// record_bytes each, and the lists inside that window are all they touch:
+// the spacing is capped so the window fits the MALL (20,000 lists, 2.5e9
+// elements, 16-byte records, 3 chase workgroups per CU: 116 ms at K = 128
+// (a 402 MB window), 106 ms at 64 (201 MB), 124 ms at 32; with 2 workgroups
+// per CU 106.6 ms at 128 (268 MB)).
+uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t max_tails, uint64_t lanes,
+                       uint64_t record_bytes) {
   uint32_t spacing = level == 0 ? tune.spacing_top : tune.spacing_deep;
   if (!(level == 0 ? tune.spacing_top_fixed : tune.spacing_deep_fixed)) {
     const uint64_t mean = n / std::max<uint64_t>(1, max_tails);
@@ -951,7 +953,7 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
     spacing = std::min(spacing, cap);
     if (level == 0 && max_tails > 64) {
       uint32_t window = 128;
-      while (window > 2 && lanes * window * sizeof(Rec) > kMallBytes) window /= 2;
+      while (window > 2 && lanes * window * record_bytes > kMallBytes) window /= 2;
       spacing = std::min(spacing, window);
     }
   }
@@ -1048,7 +1050,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
   const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
-  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads);
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads, sizeof(Rec));
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t lanes_wanted = nR + nF;
@@ -1239,7 +1241,7 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
   const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
-  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads);
+  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads, sizeof(R));
   const uint32_t logK = log2_floor(spacing);
   const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
   const uint64_t lists = std::min<uint64_t>(max_tails, n);

@@ -72,11 +72,11 @@ def test_tree_full_size(ctx):
     torch.cuda.empty_cache()
 
 
-def segmented_identities_hold(succ, off, root, rank):
+def segmented_identities_hold(succ, off, root, rank, w=None):
     """Segment s owns [off[s], off[s + 1]); a local successor j of element v is
-    the global element off[s] + j.  Off the tails: rank[v] = 1 + rank[g] and
-    root[v] = root[g] (both local, same segment); on a tail: rank 0, root =
-    its own local id."""
+    the global element off[s] + j.  Off the tails: rank[v] = w[v] + rank[g]
+    (w = 1 when no weights are given) and root[v] = root[g] (both local, same
+    segment); on a tail: rank 0, root = its own local id."""
     import torch
     off64 = off.view(torch.int64)
     starts = off64[:-1].contiguous()
@@ -95,7 +95,8 @@ def segmented_identities_hold(succ, off, root, rank):
         tail = g == idx
         rt = root[a:b].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
         rg = root.view(torch.int32)[g].to(torch.int64) & 0xFFFFFFFF
-        ok &= bool(torch.all(torch.where(tail, rank[a:b] == 0, rank[a:b] == 1 + rank[g])).item())
+        ww = 1 if w is None else w[a:b]
+        ok &= bool(torch.all(torch.where(tail, rank[a:b] == 0, rank[a:b] == ww + rank[g])).item())
         ok &= bool(torch.all(torch.where(tail, rt == idx - base, rt == rg)).item())
     return ok
 

@@ -120,3 +120,117 @@ def test_segmented_many_tiny_lists(ctx, algo):
     er, ek = o.sequential_walk(gsucc, w)
     base = np.repeat(off[:-1], sizes.astype(np.int64))
     assert (host(root).astype(np.uint64) == er - base).all() and (host(rank) == ek).all()
+
+
+def host_batch(sizes, seed, weighted=True):
+    """Host arrays of a segmented batch (local u32 successors) and each
+    segment's oracle result."""
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    succ = np.empty(int(off[-1]), np.uint32)
+    w = np.empty(int(off[-1]), np.int64)
+    for s, n in enumerate(sizes):
+        a = int(off[s])
+        ss, ww = o.gen_list(int(n), seed + s, weighted)
+        succ[a:a + n] = ss.astype(np.uint32)
+        w[a:a + n] = ww
+    return off, succ, w
+
+
+def host_expected(off, succ, w):
+    roots, ranks = [], []
+    for s in range(len(off) - 1):
+        a, b = int(off[s]), int(off[s + 1])
+        if a == b:
+            continue
+        r, k = o.sequential_walk(succ[a:b].astype(np.uint64), w[a:b])
+        roots.append(r.astype(np.uint32))
+        ranks.append(k)
+    return np.concatenate(roots), np.concatenate(ranks)
+
+
+@pytest.mark.parametrize("weighted", [True, False])
+def test_segmented_narrow_group(ctx, weighted):
+    """A batch of 2^22 elements or more (below 2^30) is ranked with 8-byte
+    records."""
+    say("seg narrow group", weighted)
+    rng = np.random.default_rng(23)
+    sizes = np.floor(2.0 ** rng.uniform(10, 16, size=1500)).astype(np.uint64)
+    off, seeds, local = make_batch(sizes, 5)
+    assert off[-1] >= 1 << 22
+    succ, w = ctx.gen_lists_segmented(dev(off), dev(seeds), dev(local), weighted=weighted)
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(succ, dev(off), w if weighted else None)
+    after = ctx.record_stats()
+    er, ek = expected(off, seeds, local, weighted)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
+    assert after["narrow_calls"] - before["narrow_calls"] == 1
+
+
+@pytest.mark.parametrize("weight", [1 << 33, 1 << 29])
+def test_segmented_narrow_fallbacks(ctx, weight):
+    """One segment with weights beyond int32 (the batch is packed wide), or
+    whose claim distances outgrow int32 (the narrow ranking is redone
+    wide): results unchanged."""
+    say("seg narrow fallback", weight)
+    rng = np.random.default_rng(29)
+    sizes = [int(x) for x in np.floor(2.0 ** rng.uniform(12, 17, size=120))]
+    off, succ, w = host_batch(sizes, 300)
+    assert off[-1] >= 1 << 22
+    a, b = int(off[7]), int(off[8])
+    w[a:b] = np.where(w[a:b] != 0, np.int64(weight), np.int64(0))
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(dev(succ), dev(off), dev(w))
+    after = ctx.record_stats()
+    er, ek = host_expected(off, succ, w)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
+    assert after["narrow_calls"] == before["narrow_calls"]
+    if weight < (1 << 31):
+        assert after["narrow_redos"] - before["narrow_redos"] == 1
+
+
+def test_segmented_narrow_large(ctx):
+    """About 9e8 elements (below 2^30 ids): one ranking with 8-byte records;
+    every element checked through the local identities."""
+    import torch
+    from test_gpu_fullsize import segmented_identities_hold
+    rng = np.random.default_rng(31)
+    sizes = np.floor(2.0 ** rng.uniform(16, 22, size=900)).astype(np.uint64)
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    assert (1 << 29) < off[-1] < (1 << 30)
+    say("seg narrow large", int(off[-1]))
+    seeds = rng.integers(1, 1 << 62, size=len(sizes), dtype=np.uint64)
+    d_off = torch.from_numpy(off).to("cuda")
+    succ, w = ctx.gen_lists_segmented(d_off, torch.from_numpy(seeds).to("cuda"), None, weighted=True)
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(succ, d_off, w)
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 1
+    assert segmented_identities_hold(succ, d_off, root, rank, w)
+    del succ, w, root, rank
+    ctx.trim()
+    torch.cuda.empty_cache()
+
+
+def test_segmented_segment_beyond_narrow_ids(ctx):
+    """A batch of more than 2^30 elements (one segment beyond 2^30 between two
+    small ones) is ranked with 16-byte records."""
+    import torch
+    from test_gpu_fullsize import segmented_identities_hold
+    sizes = [10, (1 << 30) + 5, 1000]
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    say("seg huge segment", int(off[-1]))
+    d_off = torch.from_numpy(off).to("cuda")
+    seeds = torch.tensor([3, 5, 7], dtype=torch.uint64, device="cuda")
+    succ, w = ctx.gen_lists_segmented(d_off, seeds, None, weighted=False)
+    del w
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(succ, d_off, None)
+    after = ctx.record_stats()
+    assert after["narrow_calls"] == before["narrow_calls"]
+    assert segmented_identities_hold(succ, d_off, root, rank)
+    del succ, root, rank
+    ctx.trim()
+    torch.cuda.empty_cache()
