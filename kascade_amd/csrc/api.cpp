@@ -179,6 +179,12 @@ void Arena::signature(std::vector<uint64_t>& out) const {
 
 uint64_t Arena::held() const { return held_.load(std::memory_order_relaxed); }
 
+void Arena::reserve(size_t bytes) {
+  // only an arena with nothing allocated yet: no kernel can still use its chunks
+  if (fixed_ || top_ != 0 || !chunks_.empty() || bytes == 0) return;
+  new_chunk(round_chunk(bytes));
+}
+
 // -------------------------------------------------------------- Profiler ---
 hipEvent_t Profiler::get() {
   if (!pool_.empty()) {

@@ -67,6 +67,10 @@ class Arena {
   // Frees every chunk (no live blocks).
   void trim();
   uint64_t held() const;
+  // A fresh arena's first chunk, sized up front (a no-op once it holds any):
+  // an arena that grows later does so by consolidating, which waits for the
+  // whole device.
+  void reserve(size_t bytes);
   uint64_t peak() const { return peak_; }
   // Appends what decides the addresses of the next allocations (chunks and
   // the stack position), so equal signatures hand out equal addresses.
