@@ -185,6 +185,7 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
   for (unsigned q = 0; q < kPer; q++) {
     const uint64_t t = chunk_first() + q * kThreads + threadIdx.x;
     bool root = false;
+    uint64_t slot = 0;
     if (t < nt) {
       jc.seek(t);
       const BatchJob& j = jc.job;
@@ -192,8 +193,9 @@ batch_tree_link_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
       const uint64_t p = j.succ[i];
       if (p >= j.n) raise_err(err, kErrRange);
       root = (p == i) || p >= j.n;
-      if (!root) sib[t] = atomicExch(&head[j.base + p], static_cast<uint32_t>(t));
+      slot = j.base + p;
     }
+    link_child(head, sib, slot, static_cast<uint32_t>(t), t < nt && !root);
     wave_append(roots, root);
   }
 }

@@ -88,7 +88,8 @@ tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__
 }
 
 // Linked build: children lists by atomic exchange on the parent's head slot
-// (one random atomic per vertex); sib[v] = the sibling inserted before v.  The
+// (one random atomic per vertex, one per group of a wave's lanes that share
+// a parent); sib[v] = the sibling inserted before v.  The
 // child order is whatever order the exchanges happened in; any cyclic order of
 // a vertex's neighbours gives a valid tour and the same (root, depth).
 __global__ void __launch_bounds__(kThreads)
@@ -96,12 +97,13 @@ tour_link_atomic_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_
                         uint32_t* __restrict__ sib, uint32_t* err, unsigned long long* roots) {
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   bool is_root = false;
+  uint64_t p = 0;
   if (v < n) {
-    const uint64_t p = parent[v];
+    p = parent[v];
     if (p >= n) raise_err(err, kErrRange);
     is_root = (p == v) || p >= n;
-    if (!is_root) sib[v] = atomicExch(&head[p], static_cast<uint32_t>(v));
   }
+  link_child(head, sib, p, static_cast<uint32_t>(v), v < n && !is_root);
   wave_append(roots, is_root);
 }
 

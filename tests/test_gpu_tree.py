@@ -221,3 +221,56 @@ def test_tuning_file_kinds(tmp_path, monkeypatch):
         assert c.tuning_get(1 << 11, kind="tree")["spacing_top"] == 16
     finally:
         c.close()
+
+
+def hub_tree(n, seed, hub_share):
+    """Vertex 0 is the root; a share of the vertices hang directly off it,
+    the rest off a random earlier vertex; relabelled by a seeded permutation."""
+    rng = np.random.default_rng(seed)
+    v = np.arange(n)
+    parent = np.floor(rng.random(n) * v).astype(np.int64)
+    parent[rng.random(n) < hub_share] = 0
+    parent[0] = 0
+    sigma = rng.permutation(n)
+    out = np.empty(n, np.uint64)
+    out[sigma] = sigma[parent]
+    return out
+
+
+@pytest.mark.parametrize("build", ["linked", "sorted"])
+@pytest.mark.parametrize("share", [1.0, 0.5, 0.05])
+def test_root_high_degree(ctx, build, share):
+    """High-degree parents (a star, a hub holding half the vertices, a
+    light hub): the linked build inserts the children of one wave that share
+    a parent with one atomic exchange per group."""
+    n = (1 << 20) + 3
+    say("high degree", build, share)
+    parent = hub_tree(n, 41, share)
+    er, ed = o.euler_tour_rooting(parent)
+    root, depth = ctx.root_tree(dev(parent), tour_build=build)
+    assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+def test_server_star_tree(ctx):
+    """A star tree inside a coalesced serving batch (the batch's linked tour
+    build groups the star's children too)."""
+    import torch
+    import kascade_amd
+    say("server star")
+    jobs = [hub_tree(300000, 5, 1.0), hub_tree(200000, 6, 0.5), o.gen_tree(100000, 7)]
+    sv = kascade_amd.Server(ctx, num_slots=1)
+    arrays = []
+    for p in jobs:
+        s = dev(p)
+        ro = torch.empty(len(p), dtype=torch.uint64, device="cuda")
+        ra = torch.empty(len(p), dtype=torch.int64, device="cuda")
+        arrays.append((s, ro, ra))
+    torch.cuda.synchronize()
+    tickets = [sv.submit("tree", s, None, ro, ra)[0] for s, ro, ra in arrays]
+    for t in tickets:
+        sv.wait(t)
+    assert sv.stats()["batches_launched"] < len(jobs)
+    for p, (s, ro, ra) in zip(jobs, arrays):
+        er, ed = o.euler_tour_rooting(p)
+        assert (host(ro) == er).all() and (host(ra) == ed).all()
+    sv.close()
