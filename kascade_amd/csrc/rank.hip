@@ -159,6 +159,14 @@ struct SpawnArgs {
 };
 
 enum ChaseMode : int { kChasePlain = 0, kChaseSuspend = 1, kChaseResume = 2 };
+
+// The last launch of a device-sized level's chase writes the size of the
+// ruler sublist (rulers + tour heads + tails reached and spawned rulers) from
+// its last workgroup to finish, so no separate launch has to.
+struct LevelEnd {
+  uint64_t* m_out = nullptr;                 // where the sublist size goes (nullptr: not this launch)
+  unsigned long long* blocks_done = nullptr;  // workgroups finished (zeroed with the level's counters)
+};
 constexpr int kIdleQuantum = 16;  // lanes per idle report
 // Iterations between two reads of the stop flag.  Each read is an agent-scope
 // load that goes past L2; in the tail, where few waves remain, reads every 32
@@ -174,8 +182,11 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
              const uint64_t* __restrict__ forced, uint64_t nF, uint64_t tour_lo,
              uint64_t cap, Rec* __restrict__ sub, uint64_t* __restrict__ sub_orig,
              unsigned long long* work_ctr, unsigned long long* extra_ctr, uint32_t* err,
-             uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev, SpawnArgs sp) {
-  if (*err) return;  // an earlier kernel found invalid input: stop, as the host-sized path does
+             uint32_t chunk, unsigned long long* chain, const uint64_t* __restrict__ n_dev, SpawnArgs sp,
+             LevelEnd le) {
+  // an earlier kernel found invalid input: stop, as the host-sized path does
+  // (decided once per workgroup, so every wave reaches the barrier at the end)
+  if (__syncthreads_or(threadIdx.x == 0 && *err != 0)) return;
   using Ops = RecOps<R>;
   bool narrow_over = false;  // a claim distance did not fit the narrow record
   const uint64_t lt = lanemask_lt();
@@ -378,6 +389,18 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
   }
   if constexpr (Ops::kNarrow)
     if (narrow_over) raise_err(err, kErrNarrow);
+  if (le.m_out) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();  // this workgroup's sublist links and extra ids before its report
+      const unsigned long long done = atomicAdd(le.blocks_done, 1ull);
+      if (done + 1 == gridDim.x) {
+        __threadfence();
+        const unsigned long long extra = __hip_atomic_load(extra_ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        le.m_out[0] = nR + nF + extra;
+      }
+    }
+  }
 }
 
 // Step 2 of the spawn: one thread per candidate block.  An unreached
@@ -436,28 +459,21 @@ spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed,
   if (k < sp.cont_cap) sp.cont[k] = Cont{j, r.s, r.w, 0};
 }
 
-// T[i].s (sublist root id) -> this level's id; T8[i] = T[i].w; mixed_roots
-// is set when two rulers have different roots (a forest).
-__global__ void first_root_kernel(const Rec* __restrict__ T, uint64_t* __restrict__ first,
-                                  uint32_t* __restrict__ mixed_roots) {
-  if (threadIdx.x == 0) {
-    first[0] = T[0].s;
-    mixed_roots[0] = 0;
-  }
-}
-
+// The ranked sublist T (root as a sublist id, rank) -> Troot[i] = the root
+// as this level's id, T8[i] = the rank; *mixed_roots (zeroed before) is set
+// when two rulers have different roots (a forest).  T itself is only read,
+// so every thread compares against the same T[0].s.
 __global__ void __launch_bounds__(kThreads)
-map_roots_kernel(Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m,
-                 const uint64_t* __restrict__ first, int64_t* __restrict__ T8,
-                 uint32_t* __restrict__ mixed_roots, const uint64_t* __restrict__ m_dev,
-                 const uint32_t* __restrict__ err) {
+map_roots_kernel(const Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m,
+                 uint64_t* __restrict__ Troot, int64_t* __restrict__ T8, uint32_t* __restrict__ mixed_roots,
+                 const uint64_t* __restrict__ m_dev, const uint32_t* __restrict__ err) {
   if (*err) return;  // the sublist ranking failed: T is not valid
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (m_dev && *m_dev < m) m = *m_dev;
   if (i >= m) return;
   const Rec t = T[i];
-  if (t.s != first[0] && *mixed_roots == 0) *mixed_roots = 1;
-  T[i].s = sub_orig[t.s];
+  if (t.s != T[0].s && *mixed_roots == 0) *mixed_roots = 1;
+  Troot[i] = sub_orig[t.s];
   T8[i] = t.w;
 }
 
@@ -484,7 +500,7 @@ seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_o
 // their record is re-marked with that compact id.
 template <class R>
 __global__ void __launch_bounds__(kThreads)
-propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __restrict__ T,
+propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const uint64_t* __restrict__ Troot,
                  const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
@@ -496,7 +512,7 @@ propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __r
   // All rulers in one list (one root): gather only the 8-byte rank, from a
   // table half the size, so more of it stays in L2.
   const bool one_root = *mixed_roots == 0;
-  const uint64_t root0 = T[0].s;
+  const uint64_t root0 = Troot[0];
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
@@ -505,8 +521,8 @@ propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const Rec* __r
       if (one_root) {
         emit(jo, om, n_out, root, rank, v, root0, T8[r.s & kIdMask] - r.w);
       } else {
-        const Rec t = T[r.s & kIdMask];
-        emit(jo, om, n_out, root, rank, v, t.s, t.w - r.w);
+        const uint64_t id = r.s & kIdMask;
+        emit(jo, om, n_out, root, rank, v, Troot[id], T8[id] - r.w);
       }
     } else if (v < tour_lo) {
       unclaimed = true;
@@ -557,8 +573,8 @@ collect_unclaimed_kernel(Rec* __restrict__ rec, uint64_t n, uint64_t* __restrict
 template <class R>
 __global__ void __launch_bounds__(kThreads)
 unclaimed_build_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u_ids,
-                       const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T,
-                       Rec* __restrict__ urec, Rec* __restrict__ ext,
+                       const Rec* __restrict__ u_orig, uint64_t nu, const uint64_t* __restrict__ Troot,
+                       const int64_t* __restrict__ T8, Rec* __restrict__ urec, Rec* __restrict__ ext,
                        const unsigned long long* __restrict__ nu_dev, const uint32_t* __restrict__ err,
                        const uint64_t* __restrict__ seg_off, uint64_t nseg) {
   if (*err) return;
@@ -580,9 +596,9 @@ unclaimed_build_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u
     ext[k] = Rec{0, 0};
   } else if (ms.s & kRootFlag) {
     // claimed: its final root / rank follow from its ruler's entry
-    const Rec t = T[ms.s & kIdMask];
+    const uint64_t id = ms.s & kIdMask;
     urec[k] = Rec{k, 0};
-    ext[k] = Rec{t.s, r0.w + t.w - ms.w};
+    ext[k] = Rec{Troot[id], r0.w + T8[id] - ms.w};
   } else {
     // not collected: a device-sized level past its capacity, which is redone
     // host-sized; any in-range values do
@@ -698,7 +714,8 @@ small_rank_kernel(const Rec* __restrict__ rec, uint32_t n, uint32_t n_out, uint6
 template <class R>
 __global__ void __launch_bounds__(kSmallThreads)
 unclaimed_small_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u_ids,
-                       const Rec* __restrict__ u_orig, uint64_t nu, const Rec* __restrict__ T, Rec* ext,
+                       const Rec* __restrict__ u_orig, uint64_t nu, const uint64_t* __restrict__ Troot,
+                       const int64_t* __restrict__ T8, Rec* ext,
                        uint64_t n_out, uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                        const unsigned long long* __restrict__ nu_dev, uint32_t* err, JobOut jo,
                        const uint64_t* __restrict__ seg_off, uint64_t nseg) {
@@ -721,8 +738,8 @@ unclaimed_small_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u
         link = static_cast<uint32_t>(ms.s & kIdMask);
         w = r0.w;
       } else if (ms.s & kRootFlag) {
-        const Rec t = T[ms.s & kIdMask];
-        e = Rec{t.s, r0.w + t.w - ms.w};
+        const uint64_t id = ms.s & kIdMask;
+        e = Rec{Troot[id], r0.w + T8[id] - ms.w};
       }
     }
     ext[k] = e;
@@ -846,15 +863,6 @@ pd_write_ds_kernel(const Rec* __restrict__ buf0, const Rec* __restrict__ buf1, u
   const Rec x = ((last + 1) & 1) ? buf1[v] : buf0[v];
   root[out_pos(om, v)] = (x.s & ~kRootFlag) >> om.shift;
   rank[out_pos(om, v)] = x.w;
-}
-
-// m = rulers + tour heads + tails reached, the size of the ruler sublist.
-__global__ void level_size_kernel(const uint64_t* __restrict__ n_dev, uint64_t n, uint32_t logK,
-                                  uint64_t nF, const unsigned long long* __restrict__ extra,
-                                  uint64_t* __restrict__ m_out) {
-  if (threadIdx.x != 0) return;
-  if (n_dev && *n_dev < n) n = *n_dev;
-  m_out[0] = ((n + (uint64_t{1} << logK) - 1) >> logK) + nF + extra[0];
 }
 
 // The doubling state after some rounds, root flags kept (bits.hpp:9-21).
@@ -1015,13 +1023,13 @@ template <class R>
 void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trace, unsigned chase_blocks, R* rec,
                   uint64_t n, uint32_t logK, uint32_t seed, uint64_t nR, const uint64_t* forced, uint64_t nF,
                   uint64_t tour_lo, uint64_t cap, Rec* sub, uint64_t* sub_orig, const ChaseSlots& cs,
-                  uint32_t chunk, const uint64_t* n_dev, int level, hipStream_t st) {
+                  uint32_t chunk, const uint64_t* n_dev, int level, hipStream_t st, LevelEnd le = LevelEnd{}) {
   Phase ph(ctx, kChaseNames[std::min(level, 8)], st);
   SpawnArgs sa;
   if (!sp.on) {
     auto kern = trace ? chase_kernel<true, kChasePlain, R> : chase_kernel<false, kChasePlain, R>;
     hipLaunchKernelGGL(kern, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF,
-                       tour_lo, cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+                       tour_lo, cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa, le);
     return;
   }
   sa.idle_ctr = cs.idle;
@@ -1032,14 +1040,14 @@ void launch_chase(Context* ctx, Scratch& scratch, const SpawnPlan& sp, bool trac
   sa.cont_cap = sp.cont_cap;
   auto k1 = trace ? chase_kernel<true, kChaseSuspend, R> : chase_kernel<false, kChaseSuspend, R>;
   hipLaunchKernelGGL(k1, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
-                     cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+                     cap, sub, sub_orig, cs.work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa, LevelEnd{});
   const uint64_t nC = (n + (uint64_t{1} << sp.logC) - 1) >> sp.logC;
   hipLaunchKernelGGL(spawn_mark_kernel<R>, dim3(blocks_for(nC, kThreads)), dim3(kThreads), 0, st, rec, n, logK, seed,
                      nR, nF, sp.logC, seed ^ 0x5bd1e995u, cap, sp.budget, sub, sub_orig, cs.extra, cs.spawned, sa,
                      n_dev, ctx->d_err, tour_lo);
   auto k3 = trace ? chase_kernel<true, kChaseResume, R> : chase_kernel<false, kChaseResume, R>;
   hipLaunchKernelGGL(k3, dim3(chase_blocks), dim3(kThreads), 0, st, rec, n, logK, seed, nR, forced, nF, tour_lo,
-                     cap, sub, sub_orig, cs.resume_work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa);
+                     cap, sub, sub_orig, cs.resume_work, cs.extra, ctx->d_err, chunk, cs.chain, n_dev, sa, le);
 }
 
 void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* root,
@@ -1102,11 +1110,11 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   rank_records_impl(ctx, sub, m, m, &T[0].s, &T[0].w, OutMap{2, 0}, std::min(max_tails, m), tune, m > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
                     level + 1, st);
   int64_t* T8 = scratch.get<int64_t>(m);
-  uint64_t* first = scratch.get<uint64_t>(1);
+  uint64_t* Troot = scratch.get<uint64_t>(m);
   uint32_t* mixed_roots = scratch.get<uint32_t>(1);
-  hipLaunchKernelGGL(first_root_kernel, dim3(1), dim3(64), 0, st, T, first, mixed_roots);
+  KC_HIP(hipMemsetAsync(mixed_roots, 0, sizeof(uint32_t), st));
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, T,
-                     sub_orig, m, first, T8, mixed_roots, nullptr, ctx->d_err);
+                     sub_orig, m, Troot, T8, mixed_roots, nullptr, ctx->d_err);
 
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
@@ -1117,7 +1125,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel<Rec>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, T, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
+                       n, n_out, Troot, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
                        ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false, jo);
   }
   uint64_t nu = 0;
@@ -1140,14 +1148,14 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   if (nu <= kSmallMax && nu <= tune.small_n && !ctx->trace_on) {
     // small enough for one workgroup (the per-level trace keeps the
     // sub-problem as its own ranking)
-    hipLaunchKernelGGL(unclaimed_small_kernel<Rec>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, nu, T,
-                       ext, n_out, root, rank, om, nullptr, ctx->d_err, jo, jo.seg_off, jo.nseg);
+    hipLaunchKernelGGL(unclaimed_small_kernel<Rec>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, nu,
+                       Troot, T8, ext, n_out, root, rank, om, nullptr, ctx->d_err, jo, jo.seg_off, jo.nseg);
     return;
   }
   Rec* urec = scratch.get<Rec>(nu);
   Rec* ures = scratch.get<Rec>(nu);
   hipLaunchKernelGGL(unclaimed_build_kernel<Rec>, dim3(blocks_for(nu, kThreads)), dim3(kThreads), 0, st,
-                     rec, u_ids, u_orig, nu, T, urec, ext, nullptr, ctx->d_err, jo.seg_off, jo.nseg);
+                     rec, u_ids, u_orig, nu, Troot, T8, urec, ext, nullptr, ctx->d_err, jo.seg_off, jo.nseg);
   // every unclaimed prefix belongs to a different list of this level
   ctx->trace_subproblem = 1;
   rank_records_impl(ctx, urec, nu, nu, &ures[0].s, &ures[0].w, OutMap{2, 0}, std::min(nu, max_tails), tune,
@@ -1203,7 +1211,8 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 // ---------------------------------------------------------------------------
 enum DsSlot : int {
   kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsChain = 4,  // chain: 3 slots
-  kDsSpawnWork = 7, kDsSpawnIdle = 8, kDsSpawned = 9, kDsSpawnCont = 10, kDsSpawnStop = 11, kDsSlots = 12
+  kDsSpawnWork = 7, kDsSpawnIdle = 8, kDsSpawned = 9, kDsSpawnCont = 10, kDsSpawnStop = 11,
+  kDsBlocksDone = 12, kDsMixedRoots = 13, kDsSlots = 14
 };
 
 template <class R>
@@ -1253,26 +1262,26 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   Rec* T = scratch.get<Rec>(cap);
   uint64_t* sub_orig = scratch.get<uint64_t>(cap);
   int64_t* T8 = scratch.get<int64_t>(cap);
-  uint64_t* first = scratch.get<uint64_t>(1);
-  uint32_t* mixed_roots = scratch.get<uint32_t>(1);
+  uint64_t* Troot = scratch.get<uint64_t>(cap);
   unsigned long long* ctr = scratch.get<unsigned long long>(kDsSlots);
+  uint32_t* mixed_roots = reinterpret_cast<uint32_t*>(ctr + kDsMixedRoots);
+  uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
   const uint32_t seed = static_cast<uint32_t>(splitmix64(0xC0FFEEull + level * 0x1234567ull + n));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, ctr, kDsSlots);
   const ChaseSlots cs{ctr + kDsWork,      ctr + kDsExtra,   ctr + kDsChain,     ctr + kDsSpawnWork,
                       ctr + kDsSpawnIdle, ctr + kDsSpawned, ctr + kDsSpawnCont, ctr + kDsSpawnStop};
+  // the chase's last workgroup writes the sublist's size to m_dev
   launch_chase(ctx, scratch, sp, false, chase_blocks, rec, n, logK, seed, nR, tours ? tours->heads : nullptr, nF,
-               tour_lo, cap, sub, sub_orig, cs, tune.chase_chunk, n_dev, level, st);
-  uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
-  hipLaunchKernelGGL(level_size_kernel, dim3(1), dim3(64), 0, st, n_dev, n, logK, nF, ctr + kDsExtra, m_dev);
+               tour_lo, cap, sub, sub_orig, cs, tune.chase_chunk, n_dev, level, st,
+               LevelEnd{m_dev, ctr + kDsBlocksDone});
   if (jo.seg_off)
     hipLaunchKernelGGL(seg_local_tails_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, sub, sub_orig,
                        cap, m_dev, jo.seg_off, jo.nseg, ctx->d_err);
   // rank the ruler sublist (recursion, sparse_ruling_set.cpp:300-309)
   ds_rank(ctx, sub, cap, m_dev, cap, &T[0].s, &T[0].w, OutMap{2, 0}, lists, tune,
           cap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1, st, nullptr);
-  hipLaunchKernelGGL(first_root_kernel, dim3(1), dim3(64), 0, st, T, first, mixed_roots);
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, T, sub_orig, cap,
-                     first, T8, mixed_roots, m_dev, ctx->d_err);
+                     Troot, T8, mixed_roots, m_dev, ctx->d_err);
 
   // Unreached head prefixes: about K elements per list; the capacity leaves
   // a wide margin (a prefix longer than 24 K has probability e^-24 on a
@@ -1282,8 +1291,8 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   Rec* u_orig = scratch.get<Rec>(std::max<uint64_t>(ucap, 1));
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(propagate_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out, T,
-                       T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
+    hipLaunchKernelGGL(propagate_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out,
+                       Troot, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
                        ctx->d_err, n_dev, true, jo);
   }
   if (ucap == 0) return;  // tours only: nothing is left unreached
@@ -1291,14 +1300,14 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   Rec* ext = scratch.get<Rec>(ucap);
   const unsigned long long* nu_dev = ctr + kDsUnclaimed;
   if (ucap <= kSmallMax && ucap <= tune.small_n) {
-    hipLaunchKernelGGL(unclaimed_small_kernel<R>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, ucap, T,
-                       ext, n_out, root, rank, om, nu_dev, ctx->d_err, jo, jo.seg_off, jo.nseg);
+    hipLaunchKernelGGL(unclaimed_small_kernel<R>, dim3(1), dim3(kSmallThreads), 0, st, rec, u_ids, u_orig, ucap,
+                       Troot, T8, ext, n_out, root, rank, om, nu_dev, ctx->d_err, jo, jo.seg_off, jo.nseg);
     return;
   }
   Rec* urec = scratch.get<Rec>(ucap);
   Rec* ures = scratch.get<Rec>(ucap);
   hipLaunchKernelGGL(unclaimed_build_kernel<R>, dim3(blocks_for(ucap, kThreads)), dim3(kThreads), 0, st, rec, u_ids,
-                     u_orig, ucap, T, urec, ext, nu_dev, ctx->d_err, jo.seg_off, jo.nseg);
+                     u_orig, ucap, Troot, T8, urec, ext, nu_dev, ctx->d_err, jo.seg_off, jo.nseg);
   ds_rank(ctx, urec, ucap, reinterpret_cast<const uint64_t*>(nu_dev), ucap, &ures[0].s, &ures[0].w, OutMap{2, 0},
           std::min(ucap, max_tails), tune, ucap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET,
           level + 1, st, nullptr);
