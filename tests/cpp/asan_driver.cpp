@@ -17,7 +17,7 @@ int main() {
   void* ctx = nullptr;
   CK(kc_context_create(0, 0, &ctx));
   printf("step lists\n"); fflush(stdout);
-  for (uint64_t n : {1ull, 100ull, 100003ull, 1ull << 20}) {
+  for (uint64_t n : {1ull, 100ull, 100003ull, 1ull << 20, (1ull << 22) + 3}) {
     uint64_t *s, *r; int64_t *w, *k;
     HK(hipMalloc(&s, n * 8)); HK(hipMalloc(&w, n * 8)); HK(hipMalloc(&r, n * 8)); HK(hipMalloc(&k, n * 8));
     CK(kc_gen_list(ctx, n, 7 + n, 1, 0, s, w, nullptr));
@@ -48,12 +48,18 @@ int main() {
     printf("big cycle status %d\n", (int)st);
   }
   printf("step trees\n"); fflush(stdout);
-  for (uint64_t n : {1ull, 3ull, 5000ull, 1ull << 20}) {
+  for (uint64_t n : {1ull, 3ull, 5000ull, 1ull << 20, (1ull << 21) + 5}) {
     uint64_t *p, *r; int64_t *d;
     HK(hipMalloc(&p, n * 8)); HK(hipMalloc(&r, n * 8)); HK(hipMalloc(&d, n * 8));
     CK(kc_gen_tree(ctx, n, 11 + n, p, nullptr));
     for (int tb : {0, 1, 2}) { kc_tree_cfg tc{}; tc.tour_build = tb; CK(kc_root_tree(ctx, p, n, r, d, &tc, nullptr)); }
     HK(hipFree(p)); HK(hipFree(r)); HK(hipFree(d));
+  }
+  {
+    // a star: the linked build's grouped child insertion
+    const uint64_t n = 300000;
+    std::vector<uint64_t> hp(n, 7), hr(n); std::vector<int64_t> hd(n);
+    CK(kc_root_tree(ctx, hp.data(), n, hr.data(), hd.data(), nullptr, nullptr));
   }
   printf("step segmented\n"); fflush(stdout);
   {
@@ -70,6 +76,21 @@ int main() {
     CK(kc_gen_lists_segmented(ctx, doff, nseg, dseeds, dloc, 1, ds, dw, nullptr));
     for (int launch : {0, 1, 2}) { kc_rank_cfg cfg{}; cfg.launch = launch; CK(kc_rank_lists_segmented(ctx, ds, doff, nseg, dw, dr, dk, &cfg, nullptr)); }
     HK(hipFree(doff)); HK(hipFree(dseeds)); HK(hipFree(dloc)); HK(hipFree(ds)); HK(hipFree(dr)); HK(hipFree(dw)); HK(hipFree(dk));
+  }
+  printf("step segmented narrow\n"); fflush(stdout);
+  {
+    const uint64_t nseg = 1500;
+    std::vector<uint64_t> off(nseg + 1, 0), seeds(nseg);
+    for (uint64_t sidx = 0; sidx < nseg; sidx++) { off[sidx + 1] = off[sidx] + (sidx * 37 % 6000); seeds[sidx] = 9 + sidx; }
+    const uint64_t total = off[nseg];
+    uint64_t *doff, *dseeds; uint32_t *ds, *dr; int64_t *dw, *dk;
+    HK(hipMalloc(&doff, (nseg + 1) * 8)); HK(hipMalloc(&dseeds, nseg * 8));
+    HK(hipMalloc(&ds, total * 4)); HK(hipMalloc(&dr, total * 4)); HK(hipMalloc(&dw, total * 8)); HK(hipMalloc(&dk, total * 8));
+    HK(hipMemcpy(doff, off.data(), (nseg + 1) * 8, hipMemcpyHostToDevice));
+    HK(hipMemcpy(dseeds, seeds.data(), nseg * 8, hipMemcpyHostToDevice));
+    CK(kc_gen_lists_segmented(ctx, doff, nseg, dseeds, nullptr, 1, ds, dw, nullptr));
+    CK(kc_rank_lists_segmented(ctx, ds, doff, nseg, dw, dr, dk, nullptr, nullptr));
+    HK(hipFree(doff)); HK(hipFree(dseeds)); HK(hipFree(ds)); HK(hipFree(dr)); HK(hipFree(dw)); HK(hipFree(dk));
   }
   printf("step server\n"); fflush(stdout);
   for (int mode = 0; mode < 2; mode++) {

@@ -11,7 +11,8 @@ HIPCC=${HIPCC:-/opt/rocm/bin/hipcc}
 mkdir -p "$OUT"
 FLAGS="-O1 -g -fPIC -std=c++20 --offload-arch=gfx950 -Xclang -target-feature -Xclang -packed-fp32-ops -I $ROOT/include -Xarch_host -fsanitize=address -Xarch_host -fno-omit-frame-pointer"
 objs=""
-for f in api.cpp rank.hip gen.hip util.hip euler.hip segmented.hip batch.hip server.cpp probe.hip listutil.hip graph.hip; do
+for f in api.cpp rank.hip gen.hip util.hip euler.hip segmented.hip batch.hip server.cpp probe.hip listutil.hip graph.hip \
+         noalloc.hip contract.hip; do
   "$HIPCC" -x hip $FLAGS -c "$SRC/$f" -o "$OUT/$f.o" &
   objs="$objs $OUT/$f.o"
 done
