@@ -459,22 +459,52 @@ spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed,
   if (k < sp.cont_cap) sp.cont[k] = Cont{j, r.s, r.w, 0};
 }
 
-// The ranked sublist T (root as a sublist id, rank) -> Troot[i] = the root
-// as this level's id, T8[i] = the rank; *mixed_roots (zeroed before) is set
-// when two rulers have different roots (a forest).  T itself is only read,
-// so every thread compares against the same T[0].s.
+// The rulers' final (root, rank), as propagation gathers them.  root and rank
+// hold every value; when the values fit 32 bits, rank4 (the rank as int32)
+// and packed (the root id in the low, the rank in the high 32 bits) let
+// propagation gather 4 bytes per element (one root) or 8 (several) instead
+// of 8 or 16, from a table half the size.  flags (kTableFlags words, zeroed
+// before map_roots_kernel): [0] two rulers have different roots, [1] a rank
+// does not fit int32, [2] a root id does not fit 32 bits.
+constexpr int kTableFlags = 3;
+struct RulerTable {
+  uint64_t* root;
+  int64_t* rank;
+  int32_t* rank4;
+  uint64_t* packed;
+  uint32_t* flags;
+};
+
+// rank4 and packed take 12 bytes per ruler; they live in the memory of the
+// ruler sublist's records (16 bytes per ruler), which are dead once the
+// sublist has been ranked.
+template <class R>
+RulerTable ruler_table(uint64_t* root, int64_t* rank, R* dead_sub, uint64_t cap, uint32_t* flags) {
+  static_assert(sizeof(R) >= 12, "rank4 and packed need 12 bytes per ruler");
+  uint64_t* packed = reinterpret_cast<uint64_t*>(dead_sub);
+  return RulerTable{root, rank, reinterpret_cast<int32_t*>(packed + cap), packed, flags};
+}
+
+// The ranked sublist T (root as a sublist id, rank) -> the ruler table.  T
+// itself is only read, so every thread compares against the same T[0].s;
+// the flags are set with plain stores of 1 (every writer writes the same).
 __global__ void __launch_bounds__(kThreads)
-map_roots_kernel(const Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m,
-                 uint64_t* __restrict__ Troot, int64_t* __restrict__ T8, uint32_t* __restrict__ mixed_roots,
+map_roots_kernel(const Rec* __restrict__ T, const uint64_t* __restrict__ sub_orig, uint64_t m, RulerTable tab,
                  const uint64_t* __restrict__ m_dev, const uint32_t* __restrict__ err) {
   if (*err) return;  // the sublist ranking failed: T is not valid
   uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (m_dev && *m_dev < m) m = *m_dev;
   if (i >= m) return;
   const Rec t = T[i];
-  if (t.s != T[0].s && *mixed_roots == 0) *mixed_roots = 1;
-  Troot[i] = sub_orig[t.s];
-  T8[i] = t.w;
+  const uint64_t root_id = sub_orig[t.s];
+  const int64_t r = t.w;
+  if (t.s != T[0].s && tab.flags[0] == 0) tab.flags[0] = 1;
+  if (r != static_cast<int32_t>(r) && tab.flags[1] == 0) tab.flags[1] = 1;
+  if ((root_id >> 32) && tab.flags[2] == 0) tab.flags[2] = 1;
+  tab.root[i] = root_id;
+  tab.rank[i] = r;
+  tab.rank4[i] = static_cast<int32_t>(r);
+  tab.packed[i] = (root_id & 0xFFFFFFFFull) | (static_cast<uint64_t>(static_cast<uint32_t>(r)) << 32);
 }
 
 // Segmented batches: the tails of the ruler sublist (self loops; the roots of
@@ -500,8 +530,7 @@ seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_o
 // their record is re-marked with that compact id.
 template <class R>
 __global__ void __launch_bounds__(kThreads)
-propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const uint64_t* __restrict__ Troot,
-                 const int64_t* __restrict__ T8, const uint32_t* __restrict__ mixed_roots,
+propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, RulerTable tab,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
                  uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig, uint64_t ucap,
                  unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err,
@@ -509,20 +538,27 @@ propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, const uint64_t
   if (*err) return;
   uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (n_dev && *n_dev < n) n = *n_dev;
-  // All rulers in one list (one root): gather only the 8-byte rank, from a
-  // table half the size, so more of it stays in L2.
-  const bool one_root = *mixed_roots == 0;
-  const uint64_t root0 = Troot[0];
+  // All rulers in one list (one root): gather only the rank, 4 bytes when
+  // every rank fits.  Several roots: one 8-byte (root, rank) entry when both
+  // fit, else the two 8-byte tables.  Smaller tables keep more in L2.
+  const bool one_root = tab.flags[0] == 0;
+  const bool rank32 = tab.flags[1] == 0;
+  const bool packed = rank32 && tab.flags[2] == 0;
+  const uint64_t root0 = tab.root[0];
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
     r = RecOps<R>::decode(load_stream(&rec[v]));
     if (r.s & kRootFlag) {
+      const uint64_t id = r.s & kIdMask;
       if (one_root) {
-        emit(jo, om, n_out, root, rank, v, root0, T8[r.s & kIdMask] - r.w);
+        const int64_t t = rank32 ? tab.rank4[id] : tab.rank[id];
+        emit(jo, om, n_out, root, rank, v, root0, t - r.w);
+      } else if (packed) {
+        const uint64_t e = tab.packed[id];
+        emit(jo, om, n_out, root, rank, v, e & 0xFFFFFFFFull, static_cast<int32_t>(e >> 32) - r.w);
       } else {
-        const uint64_t id = r.s & kIdMask;
-        emit(jo, om, n_out, root, rank, v, Troot[id], T8[id] - r.w);
+        emit(jo, om, n_out, root, rank, v, tab.root[id], tab.rank[id] - r.w);
       }
     } else if (v < tour_lo) {
       unclaimed = true;
@@ -1111,10 +1147,11 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
                     level + 1, st);
   int64_t* T8 = scratch.get<int64_t>(m);
   uint64_t* Troot = scratch.get<uint64_t>(m);
-  uint32_t* mixed_roots = scratch.get<uint32_t>(1);
-  KC_HIP(hipMemsetAsync(mixed_roots, 0, sizeof(uint32_t), st));
+  uint32_t* flags = scratch.get<uint32_t>(kTableFlags);
+  KC_HIP(hipMemsetAsync(flags, 0, kTableFlags * sizeof(uint32_t), st));
+  const RulerTable tab = ruler_table(Troot, T8, sub, m, flags);
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(m, kThreads)), dim3(kThreads), 0, st, T,
-                     sub_orig, m, Troot, T8, mixed_roots, nullptr, ctx->d_err);
+                     sub_orig, m, tab, nullptr, ctx->d_err);
 
   // propagation + collection of unclaimed elements
   zero_counters(ctx, st);
@@ -1125,7 +1162,7 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel<Rec>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, Troot, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap,
+                       n, n_out, tab, root, rank, om, u_ids, u_orig, ucap,
                        ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false, jo);
   }
   uint64_t nu = 0;
@@ -1212,7 +1249,7 @@ void rank_records_impl(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint6
 enum DsSlot : int {
   kDsWork = 0, kDsExtra = 1, kDsUnclaimed = 2, kDsM = 3, kDsChain = 4,  // chain: 3 slots
   kDsSpawnWork = 7, kDsSpawnIdle = 8, kDsSpawned = 9, kDsSpawnCont = 10, kDsSpawnStop = 11,
-  kDsBlocksDone = 12, kDsMixedRoots = 13, kDsSlots = 14
+  kDsBlocksDone = 12, kDsTableFlags = 13, kDsSlots = 15  // table flags: 2 slots (kTableFlags words)
 };
 
 template <class R>
@@ -1264,7 +1301,7 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   int64_t* T8 = scratch.get<int64_t>(cap);
   uint64_t* Troot = scratch.get<uint64_t>(cap);
   unsigned long long* ctr = scratch.get<unsigned long long>(kDsSlots);
-  uint32_t* mixed_roots = reinterpret_cast<uint32_t*>(ctr + kDsMixedRoots);
+  const RulerTable tab = ruler_table(Troot, T8, sub, cap, reinterpret_cast<uint32_t*>(ctr + kDsTableFlags));
   uint64_t* m_dev = reinterpret_cast<uint64_t*>(ctr + kDsM);
   const uint32_t seed = static_cast<uint32_t>(splitmix64(0xC0FFEEull + level * 0x1234567ull + n));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, ctr, kDsSlots);
@@ -1281,7 +1318,7 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   ds_rank(ctx, sub, cap, m_dev, cap, &T[0].s, &T[0].w, OutMap{2, 0}, lists, tune,
           cap > n / 2 ? KC_ALGO_POINTER_DOUBLING : KC_ALGO_SPARSE_RULING_SET, level + 1, st, nullptr);
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, T, sub_orig, cap,
-                     Troot, T8, mixed_roots, m_dev, ctx->d_err);
+                     tab, m_dev, ctx->d_err);
 
   // Unreached head prefixes: about K elements per list; the capacity leaves
   // a wide margin (a prefix longer than 24 K has probability e^-24 on a
@@ -1292,7 +1329,7 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
     hipLaunchKernelGGL(propagate_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out,
-                       Troot, T8, mixed_roots, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
+                       tab, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
                        ctx->d_err, n_dev, true, jo);
   }
   if (ucap == 0) return;  // tours only: nothing is left unreached
