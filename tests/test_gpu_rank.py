@@ -90,6 +90,22 @@ def test_rank_weight_ranges(ctx, algo, kind, n, nlists):
     assert (host(root) == er).all() and (host(rank) == ek).all()
 
 
+@pytest.mark.parametrize("nlists", [1, 2])
+@pytest.mark.parametrize("weight", [2047, 2048, 2049, -2048, -2049])
+def test_rank_near_int32_edge(ctx, weight, nlists):
+    """Constant weights whose ranks end just inside or just beyond int32
+    (|weight| x 2^20 around 2^31): the rulers' ranks are gathered from a
+    4-byte table (one list) or a packed (root, rank) table (two lists) only
+    while every ruler rank fits; either way the results are exact."""
+    say("int32 edge", weight, nlists)
+    n = (1 << 20) * nlists + 5
+    succ, _ = random_forest_lists(n, nlists, 4242 + nlists, weighted=False)
+    w = np.where(succ == np.arange(n, dtype=np.uint64), 0, weight).astype(np.int64)
+    root, rank = ctx.rank_list(dev(succ), dev(w), algorithm="srs")
+    er, ek = o.sequential_walk(succ, w)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
+
+
 @pytest.mark.parametrize("spacing,deep", [(2, 2), (4, 16), (64, 4), (256, 32), (4096, 8)])
 def test_rank_spacings(ctx, spacing, deep):
     n = 200003
