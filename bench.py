@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--spacing", type=int, default=0)
     ap.add_argument("--spacing-deep", type=int, default=0)
     ap.add_argument("--spawn", choices=["auto", "off", "on"], default="auto", help="ruler spawn (kc_spawn)")
+    ap.add_argument("--records", choices=["auto", "wide"], default="auto", help="top-level record width (kc_records)")
     ap.add_argument("--chase-blocks", type=int, default=0,
                     help="resident chase workgroups per CU (0: the tuning table's)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -335,7 +336,7 @@ def tree_main(args, ctx):
     parent = ctx.gen_tree(n, args.seed)
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     depth = torch.empty(n, dtype=torch.int64, device="cuda")
-    tcfg = dict(spawn=args.spawn, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
+    tcfg = dict(spawn=args.spawn, records=args.records, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
                 chase_blocks_per_cu=args.chase_blocks)
     t, phases = timed(ctx, lambda: ctx.root_tree(parent, root, depth, tour_build=args.tour_build, **tcfg), args.steps,
                       args.warmup)
@@ -406,7 +407,7 @@ def segmented_main(args, ctx):
                                       weighted=False)
     root = torch.empty(total, dtype=torch.uint32, device="cuda")
     rank = torch.empty(total, dtype=torch.int64, device="cuda")
-    scfg = dict(spawn=args.spawn, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
+    scfg = dict(spawn=args.spawn, records=args.records, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
                 chase_blocks_per_cu=args.chase_blocks)
     t, phases = timed(ctx, lambda: ctx.rank_lists_segmented(succ, d_off, None, root, rank, **scfg), args.steps,
                       args.warmup)
@@ -494,7 +495,7 @@ def main():
     root = torch.empty(n, dtype=torch.uint64, device="cuda")
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
     cfg = dict(algorithm=args.algorithm, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
-               chase_blocks_per_cu=args.chase_blocks, spawn=args.spawn)
+               chase_blocks_per_cu=args.chase_blocks, spawn=args.spawn, records=args.records)
     rs0 = ctx.record_stats()
     dt, phases = timed(ctx, lambda: ctx.rank_list(succ, w, root, rank, **cfg), args.steps, args.warmup)
     rs1 = ctx.record_stats()

@@ -98,7 +98,8 @@ typedef struct kc_rank_cfg {
   uint32_t launch;          /* kc_launch */
   uint32_t spawn;           /* kc_spawn */
   uint32_t local_contraction; /* kc_contract */
-  uint32_t reserved[2];     /* zero */
+  uint32_t records;         /* kc_records */
+  uint32_t reserved;        /* zero */
 } kc_rank_cfg;
 
 /* Local contraction of list rankings (kc_rank_cfg.local_contraction), the
@@ -128,6 +129,17 @@ typedef enum kc_spawn {
   KC_SPAWN_OFF = 1,
   KC_SPAWN_ON = 2    /* on every level with more than 8 elements per ruler, tours included */
 } kc_spawn;
+
+/* Record width of the top level (kc_rank_cfg.records).  AUTO: 8-byte records
+ * (32-bit successor, 32-bit weight and claim distance; in segmented batches
+ * the successor relative to its element) for sparse-ruling-set rankings from
+ * 2^22 records whose ids, weights and distances fit, 16-byte records
+ * otherwise; WIDE: always 16-byte records.  Results are identical either
+ * way. */
+typedef enum kc_records {
+  KC_RECORDS_AUTO = 0,
+  KC_RECORDS_WIDE = 1
+} kc_records;
 
 /* ---- context ------------------------------------------------------------ */
 kc_status kc_context_create(int device, uint64_t mem_budget_bytes, void **ctx_out);
@@ -200,11 +212,19 @@ uint32_t kc_trace_read(void *ctx, kc_trace_level *out, uint32_t max_entries);
  * chunk, one-workgroup threshold) and returns the number of loaded rows. */
 kc_status kc_tuning_set(void *ctx, const uint32_t *rows, uint32_t nrows);
 uint32_t kc_tuning_get(void *ctx, uint64_t n, uint32_t *out5);
-/* The same per workload kind (kc_job_kind): KC_JOB_LIST is the table above;
- * KC_JOB_TREE is tree rooting, whose classes count vertices (the tour has 2n
- * records) and whose rows the table file prefixes with "tree".  Mixed
- * batches use the list table.  No reference counterpart (the reference picks
- * ruler counts by formula, ruler_selection.hpp:18-79). */
+/* The same per workload kind: KC_TUNING_LIST is the table above;
+ * KC_TUNING_TREE is tree rooting, whose classes count vertices (the tour has
+ * 2n records) and whose rows the table file prefixes with "tree";
+ * KC_TUNING_SEGMENTED is kc_rank_lists_segmented, whose classes count the
+ * batch's elements and whose rows the file prefixes with "seg" (a batch below
+ * its first class uses the list table).  Mixed serving batches use the list
+ * table.  No reference counterpart (the reference picks ruler counts by
+ * formula, ruler_selection.hpp:18-79). */
+typedef enum kc_tuning_kind {
+  KC_TUNING_LIST = 0,  /* = KC_JOB_LIST */
+  KC_TUNING_TREE = 1,  /* = KC_JOB_TREE */
+  KC_TUNING_SEGMENTED = 2
+} kc_tuning_kind;
 kc_status kc_tuning_set_kind(void *ctx, int32_t kind, const uint32_t *rows, uint32_t nrows);
 uint32_t kc_tuning_get_kind(void *ctx, int32_t kind, uint64_t n, uint32_t *out5);
 
@@ -287,16 +307,20 @@ kc_status kc_unpack(void *ctx, const uint64_t *psucc, const int64_t *prank,
  * and its successors are local ids in [0, size_s).  Equivalent to ranking the
  * forest with global ids seg_off[s] + local.  root_local receives local ids.
  * weight == NULL: unit weights as set_initial_ranking_state.  The whole batch
- * is one ranking with one ruler spacing: the autotuned spacing of the
- * batch's total size, capped at the mean segment length / 16 and, when the
- * batch holds more than 64 lists, so that the chases in flight (taken in id
- * order, K ids apart) stay inside a window of records that fits the GPU's
- * 256 MB Infinity Cache (a chase ends at every segment's tail, so short
- * segments need no denser rulers of their own; on 20,000 lists of
- * 2^8..2^20 elements spacings 32 / 64 / 128 / 256 measured 124 / 106 / 116 /
- * 130 ms on MI355X).  Batches below 2^30 elements pack 8-byte records
- * (ids and weights in 32 bits; wider weights or distances fall back to
- * 16-byte records).  cfg->local_contraction applies to the batch as a
+ * is one ranking with one ruler spacing, capped at the mean segment length /
+ * 16 (a chase ends at every segment's tail, so short segments need no denser
+ * rulers of their own).  The knobs come from the segmented tuning table
+ * (KC_TUNING_SEGMENTED) by the batch's total size: from 2^27 elements 256
+ * per ruler at the top level, 16 below, 2 chase workgroups per CU (on 20,000
+ * lists of 2^8..2^20 elements, 2.5e9 in all: 94.6 ms on MI355X; 105.2 with
+ * the list table's knobs).  Smaller batches use the list table with the
+ * top-level spacing capped, when the batch holds more than 64 lists, so that
+ * the chases in flight (taken in id order, K ids apart) stay inside a window
+ * of records that fits the GPU's 256 MB Infinity Cache.  Batches from 2^22
+ * elements pack 8-byte records (the successor stored relative to its
+ * element, the weight in 32 bits) when every segment is below 2^29 elements
+ * and the weights fit; otherwise, or when a claim distance outgrows int32,
+ * 16-byte records.  cfg->local_contraction applies to the batch as a
  * whole. */
 kc_status kc_rank_lists_segmented(void *ctx, const uint32_t *succ_local, const uint64_t *seg_off,
                                   uint64_t nseg, const int64_t *weight, uint32_t *root_local,

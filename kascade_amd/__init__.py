@@ -33,7 +33,8 @@ class RankCfg(ctypes.Structure):
                 ("ruler_spacing_deep", ctypes.c_uint32), ("max_levels", ctypes.c_uint32),
                 ("validate", ctypes.c_uint32), ("chase_blocks_per_cu", ctypes.c_uint32),
                 ("chase_chunk", ctypes.c_uint32), ("launch", ctypes.c_uint32), ("spawn", ctypes.c_uint32),
-                ("local_contraction", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("local_contraction", ctypes.c_uint32), ("records", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 class TreeCfg(ctypes.Structure):
@@ -237,15 +238,19 @@ def _numel(a):
 
 LAUNCHES = {"auto": 0, "device": 1, "host": 2}
 JOB_KINDS = {"list": 0, "tree": 1}  # kc_job_kind
+TUNING_KINDS = {"list": 0, "tree": 1, "segmented": 2}  # kc_tuning_kind
 
 
 SPAWNS = {"auto": 0, "off": 1, "on": 2}  # kc_spawn
 CONTRACTIONS = {"auto": 0, "off": 1, "on": 2}  # kc_contract
+RECORDS = {"auto": 0, "wide": 1}  # kc_records
 
 
 def _cfg(algorithm="auto", ruler_spacing=0, ruler_spacing_deep=0, max_levels=0, validate=False,
-         chase_blocks_per_cu=0, chase_chunk=0, launch="auto", spawn="auto", local_contraction="auto"):
+         chase_blocks_per_cu=0, chase_chunk=0, launch="auto", spawn="auto", local_contraction="auto",
+         records="auto"):
     c = RankCfg()
+    c.records = RECORDS[records] if isinstance(records, str) else int(records)
     c.local_contraction = (CONTRACTIONS[local_contraction] if isinstance(local_contraction, str)
                            else int(local_contraction))
     c.spawn = SPAWNS[spawn] if isinstance(spawn, str) else int(spawn)
@@ -594,14 +599,15 @@ class Context:
     # ---- tuning ---------------------------------------------------------------
     def tuning_set(self, rows, kind="list"):
         """rows: iterable of (log2n, spacing_top, spacing_deep, chase_blocks_per_cu, chase_chunk);
-        kind "list" (classes count elements) or "tree" (classes count vertices)."""
+        kind "list" (classes count elements), "tree" (classes count vertices) or
+        "segmented" (classes count a batch's elements)."""
         flat = [int(x) for r in rows for x in r]
         arr = (ctypes.c_uint32 * max(1, len(flat)))(*flat)
-        self._check(self._lib.kc_tuning_set_kind(self.handle, JOB_KINDS[kind], arr, len(flat) // 5))
+        self._check(self._lib.kc_tuning_set_kind(self.handle, TUNING_KINDS[kind], arr, len(flat) // 5))
 
     def tuning_get(self, n, kind="list"):
         out = (ctypes.c_uint32 * 5)()
-        rows = self._lib.kc_tuning_get_kind(self.handle, JOB_KINDS[kind], n, out)
+        rows = self._lib.kc_tuning_get_kind(self.handle, TUNING_KINDS[kind], n, out)
         return {"spacing_top": out[0], "spacing_deep": out[1], "chase_blocks_per_cu": out[2],
                 "chase_chunk": out[3], "small_n": out[4], "table_rows": rows}
 

@@ -330,30 +330,36 @@ class Staged {
 bool load_tuning_file(Context* ctx, const std::string& path) {
   std::ifstream in(path);
   if (!in) return false;
-  std::vector<TuneRow> rows, tree_rows;
+  std::vector<TuneRow> rows, tree_rows, seg_rows;
   std::string line;
   while (std::getline(in, line)) {
     const size_t hash = line.find('#');
     if (hash != std::string::npos) line.resize(hash);
     std::istringstream ls(line);
-    const bool tree = ls >> std::ws && ls.peek() == 't';
-    if (tree) {
+    int kind = KC_TUNING_LIST;
+    ls >> std::ws;
+    if (ls.peek() == 't' || ls.peek() == 's') {
       std::string word;
-      if (!(ls >> word) || word != "tree") continue;
+      if (!(ls >> word)) continue;
+      if (word == "tree") kind = KC_TUNING_TREE;
+      else if (word == "seg") kind = KC_TUNING_SEGMENTED;
+      else continue;
     }
     TuneRow r{};
     if (!(ls >> r.log2n >> r.spacing_top >> r.spacing_deep >> r.chase_blocks_per_cu >> r.chase_chunk)) continue;
     auto pow2 = [](uint32_t x) { return x >= 2 && (x & (x - 1)) == 0; };
     if (!pow2(r.spacing_top) || !pow2(r.spacing_deep) || r.chase_blocks_per_cu == 0 || r.chase_chunk == 0)
       continue;
-    (tree ? tree_rows : rows).push_back(r);
+    (kind == KC_TUNING_TREE ? tree_rows : kind == KC_TUNING_SEGMENTED ? seg_rows : rows).push_back(r);
   }
-  if (rows.empty() && tree_rows.empty()) return false;
+  if (rows.empty() && tree_rows.empty() && seg_rows.empty()) return false;
   auto by_class = [](const TuneRow& a, const TuneRow& b) { return a.log2n < b.log2n; };
   std::sort(rows.begin(), rows.end(), by_class);
   std::sort(tree_rows.begin(), tree_rows.end(), by_class);
+  std::sort(seg_rows.begin(), seg_rows.end(), by_class);
   ctx->tuning = rows;
   ctx->tuning_tree = tree_rows;
+  ctx->tuning_seg = seg_rows;
   return true;
 }
 
@@ -984,12 +990,12 @@ uint32_t kc_profile_read(void* vctx, uint32_t max_entries, const char** names, d
 extern "C" {
 
 kc_status kc_tuning_set(void* vctx, const uint32_t* rows, uint32_t nrows) {
-  return kc_tuning_set_kind(vctx, KC_JOB_LIST, rows, nrows);
+  return kc_tuning_set_kind(vctx, KC_TUNING_LIST, rows, nrows);
 }
 
 kc_status kc_tuning_set_kind(void* vctx, int32_t kind, const uint32_t* rows, uint32_t nrows) {
   Context* ctx = static_cast<Context*>(vctx);
-  if (!ctx || (nrows && !rows) || (kind != KC_JOB_LIST && kind != KC_JOB_TREE)) return KC_EINVAL;
+  if (!ctx || (nrows && !rows) || kind < KC_TUNING_LIST || kind > KC_TUNING_SEGMENTED) return KC_EINVAL;
   std::vector<kc::TuneRow> t;
   for (uint32_t i = 0; i < nrows; i++) {
     kc::TuneRow r{rows[5 * i], rows[5 * i + 1], rows[5 * i + 2], rows[5 * i + 3], rows[5 * i + 4]};
@@ -999,25 +1005,25 @@ kc_status kc_tuning_set_kind(void* vctx, int32_t kind, const uint32_t* rows, uin
     t.push_back(r);
   }
   std::sort(t.begin(), t.end(), [](const kc::TuneRow& a, const kc::TuneRow& b) { return a.log2n < b.log2n; });
-  (kind == KC_JOB_TREE ? ctx->tuning_tree : ctx->tuning) = t;
+  (kind == KC_TUNING_TREE ? ctx->tuning_tree : kind == KC_TUNING_SEGMENTED ? ctx->tuning_seg : ctx->tuning) = t;
   return KC_OK;
 }
 
 uint32_t kc_tuning_get(void* vctx, uint64_t n, uint32_t* out5) {
-  return kc_tuning_get_kind(vctx, KC_JOB_LIST, n, out5);
+  return kc_tuning_get_kind(vctx, KC_TUNING_LIST, n, out5);
 }
 
 uint32_t kc_tuning_get_kind(void* vctx, int32_t kind, uint64_t n, uint32_t* out5) {
   Context* ctx = static_cast<Context*>(vctx);
-  if (!ctx || !out5 || (kind != KC_JOB_LIST && kind != KC_JOB_TREE)) return 0;
-  const bool tree = kind == KC_JOB_TREE;
-  kc::RankTuning t = kc::tuning_for(ctx, n, nullptr, tree);
+  if (!ctx || !out5 || kind < KC_TUNING_LIST || kind > KC_TUNING_SEGMENTED) return 0;
+  kc::RankTuning t = kc::tuning_for(ctx, n, nullptr, kind);
   out5[0] = t.spacing_top;
   out5[1] = t.spacing_deep;
   out5[2] = t.chase_blocks_per_cu;
   out5[3] = t.chase_chunk;
   out5[4] = t.small_n;
-  return static_cast<uint32_t>((tree ? ctx->tuning_tree : ctx->tuning).size());
+  return static_cast<uint32_t>(
+      (kind == KC_TUNING_TREE ? ctx->tuning_tree : kind == KC_TUNING_SEGMENTED ? ctx->tuning_seg : ctx->tuning).size());
 }
 
 }  // extern "C"

@@ -104,7 +104,7 @@ batch_list_pack_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint64
       } else {
         w = tail ? 0 : 1;
       }
-      rec[e] = RecOps<R>::encode(j.base + s, w);
+      rec[e] = RecOps<R>::encode(j.base + s, w, e);
     }
     wave_append(tails, tail);
   }
@@ -230,8 +230,8 @@ batch_tree_pairs_kernel(const BatchJob* __restrict__ jobs, uint32_t njobs, uint6
         up = Rec{sb != 0xFFFFFFFFu ? nl + 2 * uint64_t{sb} + 1 : nl + 2 * (j.base + p), 1};
         down = Rec{h != 0xFFFFFFFFu ? nl + 2 * uint64_t{h} + 1 : nl + 2 * t, -1};
       }
-      rec[nl + 2 * t] = RecOps<R>::encode(up.s, up.w);
-      rec[nl + 2 * t + 1] = RecOps<R>::encode(down.s, down.w);
+      rec[nl + 2 * t] = RecOps<R>::encode(up.s, up.w, nl + 2 * t);
+      rec[nl + 2 * t + 1] = RecOps<R>::encode(down.s, down.w, nl + 2 * t + 1);
     }
     const uint64_t k = wave_append(nheads, tour_head);
     if (tour_head) heads[k] = nl + 2 * uint64_t{h} + 1;

@@ -35,6 +35,13 @@ struct alignas(8) Rec8 {
   uint32_t s;
   int32_t w;
 };
+// The same 8 bytes with the successor stored relative to the record's own
+// position (flagged ids stay absolute): for segmented batches of any total
+// size, whose successors lie inside their segment.
+struct alignas(8) Rec8D {
+  uint32_t s;
+  int32_t w;
+};
 constexpr uint32_t kRoot8 = 1u << 31;
 constexpr uint32_t kAux8 = 1u << 30;
 constexpr uint64_t kNarrowMaxIds = uint64_t{1} << 30;
@@ -149,6 +156,7 @@ struct Context {
   Profiler prof;
   std::vector<TuneRow> tuning;       // lists; sorted by log2n; empty = built-in table
   std::vector<TuneRow> tuning_tree;  // tree rooting, by vertices; the same
+  std::vector<TuneRow> tuning_seg;   // segmented batches, by total elements; the same
   int device = 0;
   uint64_t budget = 0;
   hipStream_t stream = nullptr;
@@ -262,6 +270,10 @@ struct RankTuning {
   bool spacing_deep_fixed;
   uint32_t launch;  // kc_launch
   uint32_t spawn;   // kc_spawn
+  uint32_t records;  // kc_records
+  // level 0 of a forest caps its spacing by the cache window (level_spacing);
+  // off for rows measured on such forests (the segmented table)
+  bool window_cap;
 };
 // Per-size-class tuning table row: log2(n) class and the knobs picked for it
 // (written by kascade_amd/autotune.py into tuning_table.txt, which contexts
@@ -273,11 +285,14 @@ struct TuneRow {
   uint32_t chase_blocks_per_cu;
   uint32_t chase_chunk;
 };
-// The knobs for ranking n list elements, or (tree) for rooting a tree of n
-// vertices, whose tour has 2n records: tours have their own table.
-RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bool tree = false);
+// The knobs for ranking n list elements (KC_TUNING_LIST), rooting a tree of
+// n vertices (KC_TUNING_TREE; the tour has 2n records) or a segmented batch
+// of n elements in total (KC_TUNING_SEGMENTED; below its first class, the
+// list table): each kind has its own table.
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, int kind = KC_TUNING_LIST);
 // Loads a tuning table file (one row per line, '#' comments; rows prefixed
-// with "tree" are tree-rooting rows); false if absent.
+// with "tree" are tree-rooting rows, "seg" segmented-batch rows); false if
+// absent.
 bool load_tuning_file(Context* ctx, const std::string& path);
 
 // ---- internal entry points (device pointers only) -----------------------
@@ -370,13 +385,22 @@ int resolve_algorithm(const kc_rank_cfg* cfg, uint64_t n, bool tree);
 uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
                              uint64_t max_tails, const RankTuning& tune, hipStream_t st, uint32_t out_shift = 0,
                              const TourRegion* tours = nullptr, const JobOut* jobs_out = nullptr);
+// The same with position-relative narrow records (segmented batches: no
+// tours).  narrow_level0_fits: every id the top level's claims and
+// unreached prefixes can write into such records (sublist and prefix
+// capacities) is below 2^30, for a ranking of n records with `tails` lists.
+uint32_t rank_records_narrow(Context* ctx, Rec8D* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st, const JobOut* jobs_out);
+bool narrow_level0_fits(const Context* ctx, uint64_t n, uint64_t tails, const RankTuning& tune);
 // Narrow records apply to a top level ranked by the ruling set with
-// device-sized levels, below 2^30 records (ids, flags in bits 31 and 30) and
-// from 2^22 (below, the records fit the caches either way: A/B on MI355X,
-// lists of 2^20 0.353 against 0.338 ms wide, 2^22 0.524 both, 2^24 1.29
-// against 1.44, 2^26 4.31 against 4.97).
+// device-sized levels, below max_records (2^30 for absolute ids, flags in
+// bits 31 and 30; position-relative records have no such limit) and from
+// 2^22 (below, the records fit the caches either way: A/B on MI355X, lists
+// of 2^20 0.353 against 0.338 ms wide, 2^22 0.524 both, 2^24 1.29 against
+// 1.44, 2^26 4.31 against 4.97).
 constexpr uint64_t kNarrowMin = uint64_t{1} << 22;
-bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune);
+bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune,
+                   uint64_t max_records = kNarrowMaxIds);
 // The level-0 ids (rulers, tails, spawned rulers) must stay below 2^30 too:
 // rulers are at most records / 2, spawned ones at most a quarter.
 bool narrow_ids_fit(uint64_t records, uint64_t tails);
@@ -404,6 +428,7 @@ struct ContractOut {
 uint32_t contraction_mode(const kc_rank_cfg* cfg, uint64_t n);
 void contraction_sample(Context* ctx, const Rec* rec, uint64_t n, unsigned long long* out, hipStream_t st);
 void contraction_sample(Context* ctx, const Rec8* rec, uint64_t n, unsigned long long* out, hipStream_t st);
+void contraction_sample(Context* ctx, const Rec8D* rec, uint64_t n, unsigned long long* out, hipStream_t st);
 bool contraction_sample_wins(uint64_t sampled_interior);
 // The same decision from the caller's successor array, before any pack
 // (synchronises `st`): for lists of kContractEarlySample elements or more,

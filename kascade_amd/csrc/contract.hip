@@ -58,6 +58,12 @@ struct Rec8In {
   __device__ __forceinline__ uint64_t s_neighbour(uint64_t v) const { return rec[v].s; }
 };
 
+// Narrow records with position-relative successors (segmented batches).
+struct Rec8DIn {
+  const Rec8D* rec;
+  __device__ __forceinline__ uint64_t s_neighbour(uint64_t v) const { return RecOps<Rec8D>::decode(rec[v], v).s; }
+};
+
 // Packed records (pack_records, the segmented pack).
 struct RecIn {
   const Rec* rec;
@@ -387,6 +393,13 @@ void contraction_sample(Context* ctx, const Rec8* rec, uint64_t n, unsigned long
   (void)ctx;
   if (n < kSample) return;  // the strides would be empty (the count stays 0: no contraction)
   hipLaunchKernelGGL(lc_sample_kernel<Rec8In>, dim3(kSample / kThreads), dim3(kThreads), 0, st, Rec8In{rec}, n, out);
+}
+
+void contraction_sample(Context* ctx, const Rec8D* rec, uint64_t n, unsigned long long* out, hipStream_t st) {
+  (void)ctx;
+  if (n < kSample) return;  // the strides would be empty (the count stays 0: no contraction)
+  hipLaunchKernelGGL(lc_sample_kernel<Rec8DIn>, dim3(kSample / kThreads), dim3(kThreads), 0, st, Rec8DIn{rec}, n,
+                     out);
 }
 
 bool contraction_sample_list(Context* ctx, const uint64_t* succ, uint64_t n, hipStream_t st) {

@@ -113,20 +113,25 @@ __device__ __forceinline__ uint64_t segment_of(const uint64_t* __restrict__ seg_
 
 // Level-0 records of either width, seen as wide ones: decode() gives the
 // wide meaning (ids with kRootFlag / kAuxFlag), encode() stores one back.
+// `at` is the record's own position (Rec8D keeps unflagged successors
+// relative to it; the other forms ignore it).
 template <class R>
 struct RecOps;
 template <>
 struct RecOps<Rec> {
   static constexpr bool kNarrow = false;
-  __device__ static __forceinline__ Rec decode(Rec r) { return r; }
-  __device__ static __forceinline__ Rec encode(uint64_t s, int64_t w) { return Rec{s, w}; }
+  static constexpr uint64_t kTag = 0;  // distinguishes the forms in launch keys
+  __device__ static __forceinline__ Rec decode(Rec r, uint64_t) { return r; }
+  __device__ static __forceinline__ Rec encode(uint64_t s, int64_t w, uint64_t) { return Rec{s, w}; }
   __device__ static __forceinline__ void set_s(Rec* p, uint64_t s) { p->s = s; }
   __device__ static __forceinline__ bool fits(int64_t) { return true; }
+  __device__ static __forceinline__ bool fits_succ(uint64_t, uint64_t) { return true; }
 };
 template <>
 struct RecOps<Rec8> {
   static constexpr bool kNarrow = true;
-  __device__ static __forceinline__ Rec decode(Rec8 r) {
+  static constexpr uint64_t kTag = 1;
+  __device__ static __forceinline__ Rec decode(Rec8 r, uint64_t) {
     uint64_t s = r.s & (kAux8 - 1);
     if (r.s & kRoot8) s |= kRootFlag;
     if (r.s & kAux8) s |= kAuxFlag;
@@ -138,11 +143,38 @@ struct RecOps<Rec8> {
     if (s & kAuxFlag) t |= kAux8;
     return t;
   }
-  __device__ static __forceinline__ Rec8 encode(uint64_t s, int64_t w) {
+  __device__ static __forceinline__ Rec8 encode(uint64_t s, int64_t w, uint64_t) {
     return Rec8{narrow_s(s), static_cast<int32_t>(w)};
   }
   __device__ static __forceinline__ void set_s(Rec8* p, uint64_t s) { p->s = narrow_s(s); }
   __device__ static __forceinline__ bool fits(int64_t w) { return w >= INT32_MIN && w <= INT32_MAX; }
+  __device__ static __forceinline__ bool fits_succ(uint64_t s, uint64_t) { return s < kNarrowMaxIds; }
+};
+// Rec8D: a flagged id (a claim, a collected or spawned id) is stored as in
+// Rec8; an unflagged successor as its 30-bit two's-complement offset from
+// the record's own position, so records of any position work as long as
+// every successor lies within 2^29 of its element (segmented batches).
+template <>
+struct RecOps<Rec8D> {
+  static constexpr bool kNarrow = true;
+  static constexpr uint64_t kTag = 2;
+  static constexpr uint32_t kOffMask = kAux8 - 1;
+  __device__ static __forceinline__ Rec decode(Rec8D r, uint64_t at) {
+    if (r.s & (kRoot8 | kAux8)) return RecOps<Rec8>::decode(Rec8{r.s, r.w}, at);
+    const int64_t off = static_cast<int32_t>(r.s << 2) >> 2;  // sign-extend 30 bits
+    return Rec{at + static_cast<uint64_t>(off), r.w};
+  }
+  __device__ static __forceinline__ Rec8D encode(uint64_t s, int64_t w, uint64_t at) {
+    const uint32_t t = (s & (kRootFlag | kAuxFlag)) ? RecOps<Rec8>::narrow_s(s)
+                                                    : static_cast<uint32_t>(s - at) & kOffMask;
+    return Rec8D{t, static_cast<int32_t>(w)};
+  }
+  __device__ static __forceinline__ void set_s(Rec8D* p, uint64_t s) { p->s = RecOps<Rec8>::narrow_s(s); }
+  __device__ static __forceinline__ bool fits(int64_t w) { return w >= INT32_MIN && w <= INT32_MAX; }
+  __device__ static __forceinline__ bool fits_succ(uint64_t s, uint64_t at) {
+    const int64_t off = static_cast<int64_t>(s - at);
+    return off >= -(int64_t{1} << 29) && off < (int64_t{1} << 29);
+  }
 };
 
 // Streaming accesses (read or written once, coalesced): non-temporal, so they
@@ -160,6 +192,10 @@ __device__ __forceinline__ void store_stream(T* p, T v) {
 __device__ __forceinline__ Rec8 load_stream(const Rec8* p) {
   const uint64_t u = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
   return Rec8{static_cast<uint32_t>(u), static_cast<int32_t>(static_cast<uint32_t>(u >> 32))};
+}
+__device__ __forceinline__ Rec8D load_stream(const Rec8D* p) {
+  const uint64_t u = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(p));
+  return Rec8D{static_cast<uint32_t>(u), static_cast<int32_t>(static_cast<uint32_t>(u >> 32))};
 }
 
 }  // namespace kc

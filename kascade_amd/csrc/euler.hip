@@ -48,10 +48,10 @@ tree_keys_kernel(const uint64_t* __restrict__ parent, uint64_t n, uint32_t* __re
     keys[v] = is_root ? static_cast<uint32_t>(n) : static_cast<uint32_t>(p);
     vals[v] = static_cast<uint32_t>(v);
     if (is_root) {
-      tour[2 * v] = RecOps<R>::encode(2 * v, 0);          // terminal dummy
-      tour[2 * v + 1] = RecOps<R>::encode(2 * v + 1, 0);  // unused slot
+      tour[2 * v] = RecOps<R>::encode(2 * v, 0, 2 * v);          // terminal dummy
+      tour[2 * v + 1] = RecOps<R>::encode(2 * v + 1, 0, 2 * v + 1);  // unused slot
     } else {
-      tour[2 * v + 1] = RecOps<R>::encode(2 * v, -1);     // leaf default: down(v) -> up(v)
+      tour[2 * v + 1] = RecOps<R>::encode(2 * v, -1, 2 * v + 1);     // leaf default: down(v) -> up(v)
     }
   }
   wave_append(roots, is_root);
@@ -72,12 +72,12 @@ tour_link_kernel(const uint32_t* __restrict__ keys, const uint32_t* __restrict__
       const bool first = (j == 0) || keys[j - 1] != p;
       const bool last = (j + 1 == n) || keys[j + 1] != p;
       // up(c): next sibling's down edge, or up(p) / p's dummy after the last child
-      tour[2 * uint64_t{c}] = RecOps<R>::encode(last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1);
+      tour[2 * uint64_t{c}] = RecOps<R>::encode(last ? 2 * uint64_t{p} : 2 * uint64_t{vals[j + 1]} + 1, 1, 2 * uint64_t{c});
       // down(p) -> down(first child); a root p has no down edge, and its
       // first child's down edge is where its tour starts
       if (first) {
         if (parent[p] != p)
-          tour[2 * uint64_t{p} + 1] = RecOps<R>::encode(2 * uint64_t{c} + 1, -1);
+          tour[2 * uint64_t{p} + 1] = RecOps<R>::encode(2 * uint64_t{c} + 1, -1, 2 * uint64_t{p} + 1);
         else
           head = true;
       }
@@ -132,8 +132,8 @@ tour_pairs_kernel(const uint64_t* __restrict__ parent, uint64_t n, const uint32_
       up = Rec{s != 0xFFFFFFFFu ? 2 * uint64_t{s} + 1 : 2 * p, 1};
       down = Rec{h != 0xFFFFFFFFu ? 2 * uint64_t{h} + 1 : 2 * v, -1};
     }
-    tour[2 * v] = RecOps<R>::encode(up.s, up.w);
-    tour[2 * v + 1] = RecOps<R>::encode(down.s, down.w);
+    tour[2 * v] = RecOps<R>::encode(up.s, up.w, 2 * v);
+    tour[2 * v + 1] = RecOps<R>::encode(down.s, down.w, 2 * v + 1);
   }
   const uint64_t k = wave_append(nheads, tour_head);
   if (tour_head) heads[k] = 2 * uint64_t{h} + 1;
@@ -353,7 +353,7 @@ void root_tree_device(Context* ctx, const uint64_t* parent, uint64_t n, uint64_t
   uint64_t* heads = scratch.get<uint64_t>(n);
   const bool sorted = cfg && cfg->tour_build == KC_TOUR_SORTED;
   const kc_rank_cfg* rc = cfg ? &cfg->rank : nullptr;
-  RankTuning tune = tuning_for(ctx, n, rc, /*tree=*/true);
+  RankTuning tune = tuning_for(ctx, n, rc, KC_TUNING_TREE);
   const int algo = resolve_algorithm(rc, n, true);
   if (narrow_wanted(ctx, 2 * n, algo, tune)) {
     // an 8-byte tour (weights +-1 and distances below 2n always fit)

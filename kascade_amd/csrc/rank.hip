@@ -109,7 +109,7 @@ pack_kernel(const uint64_t* __restrict__ succ, const int64_t* __restrict__ w, ui
     } else {
       wt = tail ? 0 : 1;
     }
-    rec[v] = RecOps<R>::encode(s, wt);
+    rec[v] = RecOps<R>::encode(s, wt, v);
   }
   wave_append(tails, tail);
 }
@@ -332,8 +332,8 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
         continue;
       }
       // the ruler's own record: claim it with distance 0
-      const Rec rr = Ops::decode(rec[cur]);
-      if (claims(cur)) rec[cur] = Ops::encode(ruler | kRootFlag, 0);
+      const Rec rr = Ops::decode(rec[cur], cur);
+      if (claims(cur)) rec[cur] = Ops::encode(ruler | kRootFlag, 0, cur);
       if constexpr (kTrace) steps = 1;
       if (rr.s == cur) {
         sub[ruler] = Rec{ruler, 0};  // the ruler is a tail
@@ -353,7 +353,7 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
       chain_end();
       continue;
     }
-    const Rec r = Ops::decode(rec[cur]);
+    const Rec r = Ops::decode(rec[cur], cur);
     if (r.s & kSpawnBits) {
       if (kMode == kChaseResume && (r.s & kSpawnBits) == kSpawnBits) {
         const uint64_t id = r.s & kIdMask;
@@ -370,7 +370,7 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
       if (j < cap) {
         sub[j] = Rec{j, 0};
         sub_orig[j] = cur;
-        rec[cur] = Ops::encode(j | kRootFlag, 0);
+        rec[cur] = Ops::encode(j | kRootFlag, 0, cur);
         sub[ruler] = Rec{j, dist};
       } else {
         raise_err(err, kErrNotList);
@@ -380,7 +380,7 @@ chase_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed, uint
     } else {
       if (claims(cur)) {
         if constexpr (Ops::kNarrow) narrow_over |= !Ops::fits(dist);
-        rec[cur] = Ops::encode(ruler | kRootFlag, dist);
+        rec[cur] = Ops::encode(ruler | kRootFlag, dist, cur);
       }
       dist += r.w;
       cur = r.s;
@@ -440,7 +440,7 @@ spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed,
       }
     }
     if (ok && !is_ruler(c, n, logK, seed)) {
-      r = RecOps<R>::decode(rec[c]);
+      r = RecOps<R>::decode(rec[c], c);
       want = !(r.s & kSpawnBits);
     }
   }
@@ -449,7 +449,7 @@ spawn_mark_kernel(R* __restrict__ rec, uint64_t n, uint32_t logK, uint32_t seed,
   const uint64_t j = nR + nF + wave_append(extra_ctr, go);
   if (!go) return;
   if (j >= cap) return;  // cannot happen: cap holds every tail plus the budget
-  rec[c] = RecOps<R>::encode(j | kSpawnBits, 0);
+  rec[c] = RecOps<R>::encode(j | kSpawnBits, 0, c);
   sub_orig[j] = c;
   if (r.s == c) {
     sub[j] = Rec{j, 0};  // a tail: the spawned ruler ends its list
@@ -548,7 +548,7 @@ propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, RulerTable tab
   Rec r{0, 0};
   bool unclaimed = false;
   if (v < n) {
-    r = RecOps<R>::decode(load_stream(&rec[v]));
+    r = RecOps<R>::decode(load_stream(&rec[v]), v);
     if (r.s & kRootFlag) {
       const uint64_t id = r.s & kIdMask;
       if (one_root) {
@@ -626,7 +626,7 @@ unclaimed_build_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u
     ext[k] = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
     return;
   }
-  const Rec ms = RecOps<R>::decode(rec[s]);
+  const Rec ms = RecOps<R>::decode(rec[s], s);
   if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
     urec[k] = Rec{ms.s & kIdMask, r0.w};
     ext[k] = Rec{0, 0};
@@ -769,7 +769,7 @@ unclaimed_small_kernel(const R* __restrict__ rec, const uint64_t* __restrict__ u
     if (r0.s == v) {
       e = Rec{seg_off ? v - seg_off[segment_of(seg_off, nseg, v)] : v, 0};
     } else {
-      const Rec ms = RecOps<R>::decode(rec[r0.s]);
+      const Rec ms = RecOps<R>::decode(rec[r0.s], r0.s);
       if ((ms.s & kAuxFlag) && !(ms.s & kRootFlag)) {
         link = static_cast<uint32_t>(ms.s & kIdMask);
         w = r0.w;
@@ -995,7 +995,7 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
     uint32_t cap = 2;
     while (cap < spacing && uint64_t{cap} * 16 <= mean) cap *= 2;
     spacing = std::min(spacing, cap);
-    if (level == 0 && max_tails > 64) {
+    if (level == 0 && max_tails > 64 && tune.window_cap) {
       uint32_t window = 128;
       while (window > 2 && lanes * window * record_bytes > kMallBytes) window /= 2;
       spacing = std::min(spacing, window);
@@ -1278,23 +1278,55 @@ void ds_pd(Context* ctx, Rec* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
                      ctx->d_err);
 }
 
+// A device-sized ruling-set level's shape, fixed on the host from its
+// capacity n, the number of lists and the tuning: the ruler spacing, the
+// chase grid, the spawn plan, the sublist capacity (every id a claim can
+// carry is below cap) and the unreached-prefix capacity (every collected id
+// is below ucap).
+struct DsLevel {
+  uint32_t spacing, logK;
+  uint64_t nR, lists;
+  unsigned chase_blocks;
+  SpawnPlan sp;
+  uint64_t cap, ucap;
+};
+
+DsLevel ds_level(const Context* ctx, uint64_t n, uint64_t max_tails, const RankTuning& tune, int level,
+                 const TourRegion* tours, size_t record_bytes) {
+  DsLevel L{};
+  const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
+  const uint64_t nF = tours ? tours->nheads : 0;
+  const uint64_t list_end = std::min(n, tour_lo);
+  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
+  L.spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads, record_bytes);
+  L.logK = log2_floor(L.spacing);
+  L.nR = (n + (uint64_t{1} << L.logK) - 1) >> L.logK;
+  L.lists = std::min<uint64_t>(max_tails, n);
+  L.chase_blocks = static_cast<unsigned>(
+      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (L.nR + nF + kThreads - 1) / kThreads)));
+  L.sp = spawn_plan(tune, n, L.logK, tours != nullptr, L.chase_blocks);
+  L.cap = L.nR + nF + L.lists + L.sp.budget;
+  // Unreached head prefixes: about K elements per list; the capacity leaves
+  // a wide margin (a prefix longer than 24 K has probability e^-24 on a
+  // random list) and an overflow is redone host-sized.
+  L.ucap = std::min<uint64_t>(list_end, 2 * uint64_t{L.spacing} * L.lists + 24 * uint64_t{L.spacing} + 4096);
+  return L;
+}
+
 template <class R>
 void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n_out, uint64_t* root,
                   int64_t* rank, OutMap om, uint64_t max_tails, const RankTuning& tune, int level,
                   hipStream_t st, const TourRegion* tours, const JobOut& jo) {
   const uint64_t tour_lo = tours ? tours->lo : ~uint64_t{0};
   const uint64_t nF = tours ? tours->nheads : 0;
-  const uint64_t list_end = std::min(n, tour_lo);
   Scratch scratch(ctx);
-  const uint64_t max_blocks = static_cast<uint64_t>(ctx->num_cus) * tune.chase_blocks_per_cu;
-  const uint32_t spacing = level_spacing(tune, level, n, max_tails, max_blocks * kThreads, sizeof(R));
-  const uint32_t logK = log2_floor(spacing);
-  const uint64_t nR = (n + (uint64_t{1} << logK) - 1) >> logK;
-  const uint64_t lists = std::min<uint64_t>(max_tails, n);
-  const unsigned chase_blocks = static_cast<unsigned>(
-      std::max<uint64_t>(1, std::min<uint64_t>(max_blocks, (nR + nF + kThreads - 1) / kThreads)));
-  const SpawnPlan sp = spawn_plan(tune, n, logK, tours != nullptr, chase_blocks);
-  const uint64_t cap = nR + nF + lists + sp.budget;
+  const DsLevel L = ds_level(ctx, n, max_tails, tune, level, tours, sizeof(R));
+  const uint32_t logK = L.logK;
+  const uint64_t nR = L.nR;
+  const uint64_t lists = L.lists;
+  const unsigned chase_blocks = L.chase_blocks;
+  const SpawnPlan& sp = L.sp;
+  const uint64_t cap = L.cap;
   Rec* sub = scratch.get<Rec>(cap);
   Rec* T = scratch.get<Rec>(cap);
   uint64_t* sub_orig = scratch.get<uint64_t>(cap);
@@ -1320,10 +1352,7 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   hipLaunchKernelGGL(map_roots_kernel, dim3(blocks_for(cap, kThreads)), dim3(kThreads), 0, st, T, sub_orig, cap,
                      tab, m_dev, ctx->d_err);
 
-  // Unreached head prefixes: about K elements per list; the capacity leaves
-  // a wide margin (a prefix longer than 24 K has probability e^-24 on a
-  // random list) and an overflow is redone host-sized.
-  const uint64_t ucap = std::min<uint64_t>(list_end, 2 * uint64_t{spacing} * lists + 24 * uint64_t{spacing} + 4096);
+  const uint64_t ucap = L.ucap;
   uint64_t* u_ids = scratch.get<uint64_t>(std::max<uint64_t>(ucap, 1));
   Rec* u_orig = scratch.get<Rec>(std::max<uint64_t>(ucap, 1));
   {
@@ -1393,23 +1422,46 @@ const TuneRow kBuiltinTuning[] = {
 const TuneRow kBuiltinTreeTuning[] = {
     {10, 4, 16, 8, 64}, {12, 4, 32, 4, 64}, {14, 8, 32, 4, 64},
     {16, 4, 8, 1, 64}, {18, 8, 4, 1, 64}, {20, 16, 4, 1, 64},
-    {22, 32, 8, 2, 64}, {24, 32, 8, 4, 64}, {26, 128, 16, 2, 64},
+    {22, 32, 8, 2, 64}, {24, 32, 8, 4, 64}, {26, 64, 8, 2, 64},
     {28, 256, 16, 2, 64},
 };
 
-RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bool tree) {
+// Segmented batches, by total elements.  Sizes below the first class use the
+// list table, whose level-0 spacing a batch of many lists caps by a cache
+// window (level_spacing); a row of this table is used as measured.  MI355X,
+// batches of lists log-uniform in [2^8, 2^20] with 8-byte records: 2000
+// lists (2.5e8 elements) 11.11 ms against 12.55 with the list table, 8000
+// (1.0e9) 39.9 against 43.8, 20,000 (2.5e9) 94.6 against 105.2.
+const TuneRow kBuiltinSegTuning[] = {
+    {27, 256, 16, 2, 64},
+};
+
+RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, int kind) {
   RankTuning t{};
   t.max_levels = 8;
   t.small_n = kSmallMax;
+  t.window_cap = true;
   uint32_t lg = 0;
   while (lg < 63 && (uint64_t{2} << lg) <= n) lg++;
-  const TuneRow* rows = tree ? kBuiltinTreeTuning : kBuiltinTuning;
-  size_t nrows = tree ? sizeof(kBuiltinTreeTuning) / sizeof(kBuiltinTreeTuning[0])
-                      : sizeof(kBuiltinTuning) / sizeof(kBuiltinTuning[0]);
-  const std::vector<TuneRow>* loaded = ctx ? (tree ? &ctx->tuning_tree : &ctx->tuning) : nullptr;
+  const TuneRow* rows = kBuiltinTuning;
+  size_t nrows = sizeof(kBuiltinTuning) / sizeof(kBuiltinTuning[0]);
+  const std::vector<TuneRow>* loaded = ctx ? &ctx->tuning : nullptr;
+  if (kind == KC_TUNING_TREE) {
+    rows = kBuiltinTreeTuning;
+    nrows = sizeof(kBuiltinTreeTuning) / sizeof(kBuiltinTreeTuning[0]);
+    loaded = ctx ? &ctx->tuning_tree : nullptr;
+  } else if (kind == KC_TUNING_SEGMENTED) {
+    rows = kBuiltinSegTuning;
+    nrows = sizeof(kBuiltinSegTuning) / sizeof(kBuiltinSegTuning[0]);
+    loaded = ctx ? &ctx->tuning_seg : nullptr;
+  }
   if (loaded && !loaded->empty()) {
     rows = loaded->data();
     nrows = loaded->size();
+  }
+  if (kind == KC_TUNING_SEGMENTED) {
+    if (rows[0].log2n > lg) return tuning_for(ctx, n, cfg, KC_TUNING_LIST);
+    t.window_cap = false;
   }
   // the row of the largest size class not above log2(n), else the smallest
   const TuneRow* pick = &rows[0];
@@ -1440,6 +1492,8 @@ RankTuning tuning_for(const Context* ctx, uint64_t n, const kc_rank_cfg* cfg, bo
     t.launch = cfg->launch;
     if (cfg->spawn > KC_SPAWN_ON) throw KcError{KC_EINVAL, "invalid spawn mode"};
     t.spawn = cfg->spawn;
+    if (cfg->records > KC_RECORDS_WIDE) throw KcError{KC_EINVAL, "invalid record width"};
+    t.records = cfg->records;
   }
   return t;
 }
@@ -1468,8 +1522,9 @@ void pack_records_narrow(Context* ctx, const uint64_t* succ, const int64_t* w, u
 
 }  // namespace
 
-bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune) {
-  return records >= kNarrowMin && records < kNarrowMaxIds && algo == KC_ALGO_SPARSE_RULING_SET &&
+bool narrow_wanted(const Context* ctx, uint64_t records, int algo, const RankTuning& tune, uint64_t max_records) {
+  return tune.records != KC_RECORDS_WIDE && records >= kNarrowMin && records < max_records &&
+         algo == KC_ALGO_SPARSE_RULING_SET &&
          tune.launch != KC_LAUNCH_HOST && !ctx->trace_on && tune.max_levels > 0;
 }
 
@@ -1557,7 +1612,7 @@ uint32_t rank_records_t(Context* ctx, R* rec, uint64_t n, uint64_t n_out, uint64
     // allocated, so the capture allocates nothing).
     std::vector<uint64_t> key = launch_key(ctx, rec, n, n_out, root, rank, max_tails, tune, algorithm, out_shift,
                                            tours, jo);
-    key.push_back(sizeof(R));
+    key.push_back(RecOps<R>::kTag);  // Rec8 and Rec8D have the same size
     if (ctx->graph_exec && key == ctx->graph_key) {
       KC_HIP(hipGraphLaunch(ctx->graph_exec, st));
       ctx->graph_replays++;
@@ -1616,6 +1671,17 @@ uint32_t rank_records_narrow(Context* ctx, Rec8* rec, uint64_t n, uint64_t n_out
                              const TourRegion* tours, const JobOut* jobs_out) {
   return rank_records_t(ctx, rec, n, n_out, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, out_shift,
                         tours, jobs_out);
+}
+
+uint32_t rank_records_narrow(Context* ctx, Rec8D* rec, uint64_t n, uint64_t n_out, uint64_t* root, int64_t* rank,
+                             uint64_t max_tails, const RankTuning& tune, hipStream_t st, const JobOut* jobs_out) {
+  return rank_records_t(ctx, rec, n, n_out, root, rank, max_tails, tune, KC_ALGO_SPARSE_RULING_SET, st, 0, nullptr,
+                        jobs_out);
+}
+
+bool narrow_level0_fits(const Context* ctx, uint64_t n, uint64_t tails, const RankTuning& tune) {
+  const DsLevel L = ds_level(ctx, n, tails, tune, 0, nullptr, sizeof(Rec8D));
+  return L.cap < kNarrowMaxIds && L.ucap < kNarrowMaxIds;
 }
 
 void rank_list_device(Context* ctx, const uint64_t* succ, const int64_t* w, uint64_t n,

@@ -41,8 +41,9 @@ __device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ seg_off
   __syncthreads();
 }
 
-// R: Rec, or Rec8 for a narrow batch (a weight outside int32 then sets *wide
-// and the batch is packed again with wide records).
+// R: Rec, or Rec8D for a narrow batch (a weight outside int32, or a
+// successor 2^29 or more away from its element, then sets *wide and the
+// batch is packed again with wide records).
 template <class R>
 __global__ void __launch_bounds__(kThreads)
 seg_pack_kernel(const uint32_t* __restrict__ succ, const uint64_t* __restrict__ seg_off,
@@ -75,7 +76,8 @@ seg_pack_kernel(const uint32_t* __restrict__ succ, const uint64_t* __restrict__ 
       } else {
         wt = tail ? 0 : 1;
       }
-      rec[i] = RecOps<R>::encode(g, wt);
+      if (!RecOps<R>::fits_succ(g, i)) *wide = 1;
+      rec[i] = RecOps<R>::encode(g, wt, i);
     }
     wave_append(tails, tail);
   }
@@ -152,37 +154,39 @@ void rank_segments_wide(Context* ctx, const uint32_t* succ, const uint64_t* seg_
                      root_global, seg_off, nseg, total, root);
 }
 
-// 8-byte records when the batch's size, weights and ids allow (below 2^30
-// ids), else, or when a claim distance outgrows int32, the wide path.
-// Measured on MI355X: 1000 lists (1.26e8 elements) 6.99 against 7.19 ms
-// wide, 200 lists (2.5e7) 1.80 against 1.97 ms.  A batch beyond 2^30 ids
-// stays one wide forest: ranking it as groups of whole segments with 8-byte
-// records (each group below 2^29 elements) was slower for 20,000 lists of
-// 2.5e9 elements (115 against 107 ms: the groups' head-prefix
-// sub-problems cost 12 against 3.4 ms).
+// 8-byte records whose successors are stored relative to their own
+// position (Rec8D), so the batch's total size does not limit them: every
+// segment below 2^29 elements, weights and claim distances inside int32, and
+// the top level's sublist and prefix ids below 2^30; otherwise, or when a
+// claim distance outgrows int32, the wide path.  Measured on MI355X: 1000
+// lists (1.26e8 elements) 6.99 against 7.19 ms wide, 200 lists (2.5e7) 1.80
+// against 1.97 ms (absolute 8-byte ids).  Ranking a batch beyond 2^30
+// elements as groups of whole segments with absolute 8-byte ids (each group
+// below 2^29 elements) was slower for 20,000 lists of 2.5e9 elements (115
+// against 107 ms: the groups' head-prefix sub-problems cost 12 against
+// 3.4 ms); relative ids keep such a batch one forest.
 void rank_segments(Context* ctx, const uint32_t* succ, const uint64_t* seg_off, uint64_t nseg, uint64_t total,
                         const int64_t* w, uint32_t* root, int64_t* rank, const kc_rank_cfg* cfg, int algo,
                         hipStream_t st) {
-  RankTuning tune = tuning_for(ctx, total, cfg);
+  RankTuning tune = tuning_for(ctx, total, cfg, KC_TUNING_SEGMENTED);
   const uint32_t contract = contraction_mode(cfg, total);
-  if (contract != KC_CONTRACT_ON && narrow_wanted(ctx, total, algo, tune)) {
+  if (contract != KC_CONTRACT_ON && narrow_wanted(ctx, total, algo, tune, ~uint64_t{0})) {
     Scratch narrow(ctx);  // released before any wide records are taken
-    Rec8* rec8 = narrow.get<Rec8>(total);
+    Rec8D* rec8 = narrow.get<Rec8D>(total);
     unsigned long long* ctr = reinterpret_cast<unsigned long long*>(ctx->d_counters);
     KC_HIP(hipMemsetAsync(ctx->d_counters, 0, 3 * sizeof(uint64_t), st));
-    hipLaunchKernelGGL(seg_pack_kernel<Rec8>, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st, succ,
+    hipLaunchKernelGGL(seg_pack_kernel<Rec8D>, dim3(blocks_for(total, kPer * kThreads)), dim3(kThreads), 0, st, succ,
                        seg_off, nseg, total, w, rec8, ctx->d_err, ctr, ctr + 2);
     if (contract == KC_CONTRACT_AUTO) contraction_sample(ctx, rec8, total, ctr + 1, st);
     uint64_t c[3] = {0, 0, 0};
-    read_counters(ctx, st, 0, 3, c);  // tails, contraction sample, a weight outside int32
+    read_counters(ctx, st, 0, 3, c);  // tails, contraction sample, a weight or successor that does not fit
     const bool contracts = contract == KC_CONTRACT_AUTO && contraction_sample_wins(c[1]);
-    if (!contracts && c[2] == 0 && narrow_ids_fit(total, c[0])) {
+    if (!contracts && c[2] == 0 && narrow_level0_fits(ctx, total, c[0], tune)) {
       JobOut jo;
       jo.seg_root = root;
       jo.seg_off = seg_off;
       jo.nseg = nseg;
-      const uint32_t redo = rank_records_narrow(ctx, rec8, total, total, nullptr, rank, c[0], tune, st, 0, nullptr,
-                                                &jo);
+      const uint32_t redo = rank_records_narrow(ctx, rec8, total, total, nullptr, rank, c[0], tune, st, &jo);
       if (redo == 0) {
         check_err(ctx, st);
         ctx->narrow_calls++;

@@ -170,3 +170,36 @@ def test_narrow_tree_cycle(ctx):
     root, depth = ctx.root_tree(dev(parent))
     er, ed = o.euler_tour_rooting(parent)
     assert (host(root) == er).all() and (host(depth) == ed).all()
+
+
+def test_records_wide_knob(ctx):
+    """kc_rank_cfg.records = WIDE keeps every top level at 16-byte records
+    (lists, trees, segmented batches); the results are those of AUTO, which
+    uses 8-byte records for all three."""
+    import torch
+    say("records knob")
+    n = 1 << 22
+    succ, w = o.gen_list(n, 41, True)
+    er, ek = o.list_closed_form(n, 41, w)
+    parent = ctx.gen_tree(n, 43)
+    tr, td = o.tree_closed_form(n, 43)
+    sizes = np.full(64, (1 << 16) + 3, np.uint64)
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    seeds = torch.from_numpy(np.arange(1, len(sizes) + 1, dtype=np.uint64)).to("cuda")
+    ssucc, sw = ctx.gen_lists_segmented(dev(off), seeds, None, weighted=True)
+    for records, narrow in [("wide", 0), ("auto", 3)]:
+        before = ctx.record_stats()
+        root, rank = ctx.rank_list(dev(succ), dev(w), records=records)
+        assert (host(root) == er).all() and (host(rank) == ek).all(), records
+        root, depth = ctx.root_tree(parent, records=records)
+        assert (host(root) == tr).all() and (host(depth) == td).all(), records
+        sroot, srank = ctx.rank_lists_segmented(ssucc, dev(off), sw, records=records)
+        if records == "wide":
+            wide_seg = (host(sroot).copy(), host(srank).copy())
+        else:
+            assert (host(sroot) == wide_seg[0]).all() and (host(srank) == wide_seg[1]).all()
+        after = ctx.record_stats()
+        assert after["narrow_calls"] - before["narrow_calls"] == narrow, records
+    with pytest.raises(Exception):
+        ctx.rank_list(dev(succ), dev(w), records=2)

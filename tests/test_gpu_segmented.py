@@ -214,8 +214,10 @@ def test_segmented_narrow_large(ctx):
 
 
 def test_segmented_segment_beyond_narrow_ids(ctx):
-    """A batch of more than 2^30 elements (one segment beyond 2^30 between two
-    small ones) is ranked with 16-byte records."""
+    """A batch with one segment beyond 2^30 elements between two small ones:
+    some successors lie 2^29 or more away from their element, more than an
+    8-byte record's relative successor holds, so the batch is ranked with
+    16-byte records."""
     import torch
     from test_gpu_fullsize import segmented_identities_hold
     sizes = [10, (1 << 30) + 5, 1000]
@@ -234,3 +236,83 @@ def test_segmented_segment_beyond_narrow_ids(ctx):
     del succ, root, rank
     ctx.trim()
     torch.cuda.empty_cache()
+
+
+def test_segmented_narrow_beyond_2e30(ctx):
+    """A batch of more than 2^30 elements whose segments are all below 2^21
+    is still ranked with 8-byte records (successors stored relative to their
+    element); every element checked through the local identities."""
+    import torch
+    from test_gpu_fullsize import segmented_identities_hold
+    rng = np.random.default_rng(37)
+    sizes = np.floor(2.0 ** rng.uniform(16, 21, size=2200)).astype(np.uint64)
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    assert off[-1] > (1 << 30)
+    say("seg narrow beyond 2^30", int(off[-1]))
+    seeds = rng.integers(1, 1 << 62, size=len(sizes), dtype=np.uint64)
+    d_off = torch.from_numpy(off).to("cuda")
+    succ, w = ctx.gen_lists_segmented(d_off, torch.from_numpy(seeds).to("cuda"), None, weighted=True)
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(succ, d_off, w)
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 1
+    assert segmented_identities_hold(succ, d_off, root, rank, w)
+    del succ, w, root, rank
+    ctx.trim()
+    torch.cuda.empty_cache()
+
+
+def test_segmented_narrow_offsets_near_2e29(ctx):
+    """One random list of 2^29 - 3 elements between two small ones: its
+    successors lie up to nearly 2^29 before or after their element, the
+    most an 8-byte record's relative successor holds; the batch is ranked
+    with 8-byte records and every element checked."""
+    import torch
+    from test_gpu_fullsize import segmented_identities_hold
+    sizes = [10, (1 << 29) - 3, 1000]
+    off = np.zeros(len(sizes) + 1, np.uint64)
+    off[1:] = np.cumsum(sizes)
+    say("seg offsets near 2^29", int(off[-1]))
+    d_off = torch.from_numpy(off).to("cuda")
+    seeds = torch.tensor([11, 13, 17], dtype=torch.uint64, device="cuda")
+    succ, w = ctx.gen_lists_segmented(d_off, seeds, None, weighted=False)
+    del w
+    loc = succ[10:10 + sizes[1]].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    delta = loc - torch.arange(sizes[1], device="cuda", dtype=torch.int64)
+    assert int(delta.max()) > (1 << 29) - (1 << 20) and int(delta.min()) < -(1 << 29) + (1 << 20)
+    del loc, delta
+    before = ctx.record_stats()
+    root, rank = ctx.rank_lists_segmented(succ, d_off, None)
+    after = ctx.record_stats()
+    assert after["narrow_calls"] - before["narrow_calls"] == 1
+    assert segmented_identities_hold(succ, d_off, root, rank)
+    del succ, root, rank
+    ctx.trim()
+    torch.cuda.empty_cache()
+
+
+def test_segmented_tuning_table(ctx):
+    """Segmented batches have their own tuning table (classes count a
+    batch's elements; below its first class the list table applies): it
+    changes the knobs of segmented rankings only, never a result."""
+    say("segmented tuning")
+    builtin = ctx.tuning_get(1 << 31, kind="segmented")
+    assert (builtin["spacing_top"], builtin["spacing_deep"], builtin["chase_blocks_per_cu"]) == (256, 16, 2)
+    knobs = ("spacing_top", "spacing_deep", "chase_blocks_per_cu", "chase_chunk")
+    below, as_list = ctx.tuning_get(1 << 20, kind="segmented"), ctx.tuning_get(1 << 20)
+    assert [below[k] for k in knobs] == [as_list[k] for k in knobs]
+    list_before = ctx.tuning_get(1 << 22)
+    rng = np.random.default_rng(43)
+    sizes = [int(x) for x in np.floor(2.0 ** rng.uniform(8, 14, size=900))]
+    off, succ, w = host_batch(sizes, 44)
+    er, ek = host_expected(off, succ, w)
+    try:
+        for rows in ([(12, 4, 4, 1, 64)], [(12, 512, 32, 1, 1024)], [(30, 8, 8, 8, 64)]):
+            ctx.tuning_set(rows, kind="segmented")
+            assert ctx.tuning_get(1 << 22) == list_before
+            root, rank = ctx.rank_lists_segmented(dev(succ), dev(off), dev(w))
+            assert (host(root) == er).all() and (host(rank) == ek).all(), rows
+    finally:
+        ctx.tuning_set([], kind="segmented")
+    assert ctx.tuning_get(1 << 31, kind="segmented")["table_rows"] == 0

@@ -204,12 +204,14 @@ def test_tree_tuning_table(ctx):
 
 
 def test_tuning_file_kinds(tmp_path, monkeypatch):
-    """A table file holds list rows and "tree" rows; malformed rows are skipped."""
+    """A table file holds list rows, "tree" rows and "seg" rows; malformed
+    rows are skipped."""
     import kascade_amd
     say("tuning file kinds")
     f = tmp_path / "tuning.txt"
     f.write_text("# comment\n10 8 8 2 64\n20 64 8 2 64   # 1.0\n"
-                 "tree 10 16 8 1 64\ntree 20 32 16 4 256\nnot a row\ntree 12 3 8 1 64\n")
+                 "tree 10 16 8 1 64\ntree 20 32 16 4 256\nnot a row\ntree 12 3 8 1 64\n"
+                 "seg 24 128 8 3 256\nsegs 25 64 8 2 64\nseg 26 3 8 2 64\n")
     monkeypatch.setenv("KASCADE_AMD_TUNING", str(f))
     c = kascade_amd.Context(0)
     try:
@@ -219,6 +221,11 @@ def test_tuning_file_kinds(tmp_path, monkeypatch):
         assert (t["spacing_top"], t["spacing_deep"], t["chase_blocks_per_cu"], t["chase_chunk"],
                 t["table_rows"]) == (32, 16, 4, 256, 2)
         assert c.tuning_get(1 << 11, kind="tree")["spacing_top"] == 16
+        s = c.tuning_get(1 << 25, kind="segmented")
+        assert (s["spacing_top"], s["spacing_deep"], s["chase_blocks_per_cu"], s["chase_chunk"],
+                s["table_rows"]) == (128, 8, 3, 256, 1)
+        # below the first segmented class: the list table
+        assert c.tuning_get(1 << 21, kind="segmented")["spacing_top"] == 64
     finally:
         c.close()
 
