@@ -585,8 +585,14 @@ def main():
                              "probe": "u64 gather through a random permutation of 2^27, 24 B/element"}
     out["record_access_ceilings"] = {
         pat: {"records_per_s": r, "ms": ms}
-        for pat in ("read", "read_write", "walk_read", "walk_claim")
+        for pat in ("read", "read_write", "walk_read", "walk_claim", "walk_read8", "walk_claim8")
         for r, ms in [ctx.probe_record_access(pat, 1 << 27, 3)]}
+    # the chase phase against its own pattern's floor: walks of 2^19 chains
+    # of 256 steps claiming every record, over records of the level-0 width
+    floor = out["record_access_ceilings"]["walk_claim8" if record_bytes == 8 else "walk_claim"]
+    if n == 1 << 27:
+        out["roofline"]["walk_claim_floor_ms"] = floor["ms"]
+        out["roofline"]["frac_of_walk_claim_floor"] = floor["ms"] / dom_ms
     out["roofline"]["frac_of_gather_ceiling"] = achieved / g_gather
     out["whole_call_frac_of_gather_ceiling"] = whole_gbps / g_gather
     out["whole_call_frac_of_hbm_peak"] = whole_gbps / HBM_PEAK_GBPS

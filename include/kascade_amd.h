@@ -485,9 +485,10 @@ kc_status kc_probe_table_gather(void *ctx, uint64_t n, uint64_t m, uint32_t reps
  * power of two, >= 256): 0 independent random reads, 1 independent random
  * writes, 2 independent read + 16-byte write of the same record, 3 dependent
  * walks (n/256 walkers of 256 steps) reading only, 4 the same walks writing a
- * 16-byte claim into every record (the ruler chase's pattern).  Records per
- * second and the kernel time: the cost of a random write against a random
- * read. */
+ * 16-byte claim into every record (the ruler chase's pattern); 5 and 6 the
+ * walks of 3 and 4 over 8-byte records (the narrow level-0 form; n at most
+ * 2^30).  Records per second and the kernel time: the cost of a random write
+ * against a random read. */
 kc_status kc_probe_record_access(void *ctx, uint64_t n, int pattern, uint32_t reps,
                                  double *records_per_s, double *ms_out);
 

@@ -585,11 +585,12 @@ class Context:
         self._check(self._lib.kc_probe_table_gather(self.handle, n, m, reps, ctypes.byref(g), ctypes.byref(ms)))
         return g.value, ms.value
 
-    RECORD_PATTERNS = ("read", "write", "read_write", "walk_read", "walk_claim")
+    RECORD_PATTERNS = ("read", "write", "read_write", "walk_read", "walk_claim", "walk_read8", "walk_claim8")
 
     def probe_record_access(self, pattern, n=1 << 27, reps=5):
         """Records per second and ms for one access pattern of 16-byte records
-        (see RECORD_PATTERNS) over a random cycle of n records."""
+        (see RECORD_PATTERNS; the "8" walks use 8-byte records) over a random
+        cycle of n records."""
         r = ctypes.c_double()
         ms = ctypes.c_double()
         self._check(self._lib.kc_probe_record_access(self.handle, n, self.RECORD_PATTERNS.index(pattern), reps,

@@ -85,6 +85,30 @@ record_walk_kernel(Rec* __restrict__ rec, const uint64_t* __restrict__ idx, int6
   }
   out[j] = d;
 }
+
+// The same cycle and walks over 8-byte records {u32 successor, i32 weight}
+// (the narrow level-0 form); the claim sets the record's root bit.
+__global__ void __launch_bounds__(kThreads)
+cycle_init8_kernel(Rec8* __restrict__ rec, const uint64_t* __restrict__ idx, uint64_t n) {
+  uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (k < n) rec[idx[k]] = Rec8{static_cast<uint32_t>(idx[(k + 1) & (n - 1)]), 1};
+}
+
+__global__ void __launch_bounds__(kThreads)
+record_walk8_kernel(Rec8* __restrict__ rec, const uint64_t* __restrict__ idx, int64_t* __restrict__ out,
+                    uint64_t walkers, uint64_t L, int claim) {
+  uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (j >= walkers) return;
+  uint64_t cur = idx[j * L];
+  int64_t d = 0;
+  for (uint64_t t = 0; t < L; ++t) {
+    const Rec8 v = rec[cur];
+    if (claim) rec[cur] = Rec8{v.s | kRoot8, v.w};
+    d += v.w;
+    cur = v.s & (kAux8 - 1);
+  }
+  out[j] = d;
+}
 }  // namespace
 }  // namespace kc
 
@@ -200,13 +224,15 @@ extern "C" kc_status kc_probe_table_gather(void* vctx, uint64_t n, uint64_t m, u
 // Access patterns of 16-byte records over a random cycle of n records (n a
 // power of two): 0 independent reads, 1 independent writes, 2 independent
 // read + 16-byte write of the same record, 3 dependent walks (n/256 walkers of
-// 256 steps) reading only, 4 the same walks writing a claim into every record.
+// 256 steps) reading only, 4 the same walks writing a claim into every record;
+// 5 and 6 the walks of 3 and 4 over 8-byte records (n at most 2^30).
 // Reports records per second and the kernel time.
 extern "C" kc_status kc_probe_record_access(void* vctx, uint64_t n, int pattern, uint32_t reps,
                                             double* records_per_s, double* ms_out) {
   auto* ctx = static_cast<kc::Context*>(vctx);
   constexpr uint64_t kWalk = 256;
-  if (!ctx || n < kWalk || (n & (n - 1)) != 0 || reps == 0 || pattern < 0 || pattern > 4)
+  if (!ctx || n < kWalk || (n & (n - 1)) != 0 || reps == 0 || pattern < 0 || pattern > 6 ||
+      (pattern >= 5 && n > kc::kNarrowMaxIds))
     return KC_EINVAL;
   try {
     KC_HIP(hipSetDevice(ctx->device));
@@ -217,10 +243,17 @@ extern "C" kc_status kc_probe_record_access(void* vctx, uint64_t n, int pattern,
     hipStream_t st = ctx->stream;
     const unsigned blocks = kc::blocks_for(n, 256);
     hipLaunchKernelGGL(kc::perm_idx_kernel, dim3(blocks), dim3(256), 0, st, idx, n, 4242ull);
-    hipLaunchKernelGGL(kc::cycle_init_kernel, dim3(blocks), dim3(256), 0, st, rec, idx, n);
+    kc::Rec8* rec8 = reinterpret_cast<kc::Rec8*>(rec);  // patterns 5, 6: the first half of the buffer
+    if (pattern >= 5)
+      hipLaunchKernelGGL(kc::cycle_init8_kernel, dim3(blocks), dim3(256), 0, st, rec8, idx, n);
+    else
+      hipLaunchKernelGGL(kc::cycle_init_kernel, dim3(blocks), dim3(256), 0, st, rec, idx, n);
     const uint64_t walkers = n / kWalk;
     auto launch = [&] {
-      if (pattern <= 2)
+      if (pattern >= 5)
+        hipLaunchKernelGGL(kc::record_walk8_kernel, dim3(kc::blocks_for(walkers, 256)), dim3(256), 0, st,
+                           rec8, idx, out, walkers, kWalk, pattern == 6 ? 1 : 0);
+      else if (pattern <= 2)
         hipLaunchKernelGGL(kc::record_access_kernel, dim3(blocks), dim3(256), 0, st, rec, idx, out, n,
                            pattern);
       else

@@ -96,3 +96,27 @@ def test_compute_stats_segmented_forest(ctx):
     rank[root == np.arange(len(root))] = 0
     got = ctx.compute_stats(dev(root), dev(rank), "extensive")
     assert got == o.compute_stats(root, rank, "extensive")
+
+
+def test_measurement_probes(ctx):
+    """The bench's access-pattern probes run at a small size and report
+    positive rates whose time matches the count; invalid sizes and patterns
+    are rejected."""
+    import kascade_amd
+    say("probes")
+    n = 1 << 16
+    g, ms = ctx.probe_random_gather(n, 2)
+    assert g > 0 and ms > 0
+    g, ms = ctx.probe_random_gather(n, 2, mode=1)
+    assert g > 0 and ms > 0
+    r, ms = ctx.probe_table_gather(n, 1 << 12, 2)
+    assert r > 0 and ms > 0
+    for pat in ctx.RECORD_PATTERNS:
+        r, ms = ctx.probe_record_access(pat, n, 2)
+        assert r > 0 and ms > 0 and abs(r * ms * 1e-3 - n) < 1e-6 * n, pat
+    for bad in [lambda: ctx.probe_record_access("walk_claim", n + 1, 1),
+                lambda: ctx.probe_record_access("walk_claim8", 1 << 31, 1),
+                lambda: ctx._check(ctx._lib.kc_probe_record_access(ctx.handle, n, 7, 1, None, None)),
+                lambda: ctx.probe_random_gather(n, 1, mode=4)]:
+        with pytest.raises(kascade_amd.KascadeError):
+            bad()
