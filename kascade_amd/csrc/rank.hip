@@ -528,7 +528,68 @@ seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_o
 // ruler.  Unclaimed elements (list prefixes no ruler reaches, the analogue of
 // fixup_unreached, post_invert.hpp:195-264) are appended to a compact list and
 // their record is re-marked with that compact id.
+//
+// Two launch shapes.  One element per thread, one counter atomic per wave
+// that has an unclaimed element: the fastest when few waves have one (a
+// wave's stores are left in flight when it exits).  Grouped (kGrouped): a
+// grid of resident size strides over the level, and each wave appends the
+// unclaimed elements of kPropGroup strides with one atomic.  A forest of
+// many short lists has an unreached prefix in most waves, and their atomics
+// on the single counter then dominate (MI355X, 200,000 lists of 2^6..2^10
+// elements, 6.9e7 in all: 10.5 ms one per thread, 0.79 ms grouped); where
+// they are rare the grouped loop is slower, because every stride waits for
+// the previous stride's stores (a load's wait counter covers stores too):
+// 20,000 lists of 2^8..2^20 with spawn, 2.5e9 elements, 11.3 against 18.9 ms.
+constexpr unsigned kPropGroup = 32;
+
 template <class R>
+__device__ __forceinline__ bool propagate_element(R* __restrict__ rec, uint64_t v, const RulerTable& tab, int table,
+                                                  uint64_t root0, uint64_t n_out, uint64_t* __restrict__ root,
+                                                  int64_t* __restrict__ rank, OutMap om, uint64_t tour_lo,
+                                                  uint32_t* err, const JobOut& jo) {
+  const Rec r = RecOps<R>::decode(load_stream(&rec[v]), v);
+  if (r.s & kRootFlag) {
+    const uint64_t id = r.s & kIdMask;
+    if (table == 0) {
+      emit(jo, om, n_out, root, rank, v, root0, tab.rank4[id] - r.w);
+    } else if (table == 1) {
+      emit(jo, om, n_out, root, rank, v, root0, tab.rank[id] - r.w);
+    } else if (table == 2) {
+      const uint64_t e = tab.packed[id];
+      emit(jo, om, n_out, root, rank, v, e & 0xFFFFFFFFull, static_cast<int32_t>(e >> 32) - r.w);
+    } else {
+      emit(jo, om, n_out, root, rank, v, tab.root[id], tab.rank[id] - r.w);
+    }
+    return false;
+  }
+  if (v < tour_lo) return true;
+  if (((v - tour_lo) & 1) == 0) {
+    // an unreached upward edge of a tour: an isolated root, else a cycle
+    if (r.s == v)
+      emit(jo, om, n_out, root, rank, v, v, 0);
+    else
+      raise_err(err, kErrCycle);
+  }
+  return false;
+}
+
+// Appends element v (unclaimed) at compact slot k.
+template <class R>
+__device__ __forceinline__ void collect_element(R* __restrict__ rec, uint64_t v, uint64_t k,
+                                                uint64_t* __restrict__ u_ids, Rec* __restrict__ u_orig,
+                                                uint64_t ucap, uint32_t* err, bool device_sized) {
+  if (k < ucap) {
+    u_ids[k] = v;
+    u_orig[k] = RecOps<R>::decode(rec[v], v);  // unclaimed: unchanged since read
+    RecOps<R>::set_s(&rec[v], k | kAuxFlag);
+  } else if (device_sized) {
+    // beyond the capacity: host-sized levels collect the rest in a second
+    // pass; device-sized ones flag the call to be redone host-sized
+    raise_err(err, kErrOverflow);
+  }
+}
+
+template <class R, bool kGrouped>
 __global__ void __launch_bounds__(kThreads)
 propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, RulerTable tab,
                  uint64_t* __restrict__ root, int64_t* __restrict__ rank, OutMap om,
@@ -536,49 +597,49 @@ propagate_kernel(R* __restrict__ rec, uint64_t n, uint64_t n_out, RulerTable tab
                  unsigned long long* u_ctr, uint64_t tour_lo, uint32_t* err,
                  const uint64_t* __restrict__ n_dev, bool device_sized, JobOut jo) {
   if (*err) return;
-  uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
   if (n_dev && *n_dev < n) n = *n_dev;
   // All rulers in one list (one root): gather only the rank, 4 bytes when
-  // every rank fits.  Several roots: one 8-byte (root, rank) entry when both
-  // fit, else the two 8-byte tables.  Smaller tables keep more in L2.
-  const bool one_root = tab.flags[0] == 0;
+  // every rank fits (table 0; else 1).  Several roots: one 8-byte (root,
+  // rank) entry when both fit (2), else the two 8-byte tables (3).  Smaller
+  // tables keep more in L2.
   const bool rank32 = tab.flags[1] == 0;
-  const bool packed = rank32 && tab.flags[2] == 0;
+  const int table = tab.flags[0] == 0 ? (rank32 ? 0 : 1) : (rank32 && tab.flags[2] == 0 ? 2 : 3);
   const uint64_t root0 = tab.root[0];
-  Rec r{0, 0};
-  bool unclaimed = false;
-  if (v < n) {
-    r = RecOps<R>::decode(load_stream(&rec[v]), v);
-    if (r.s & kRootFlag) {
-      const uint64_t id = r.s & kIdMask;
-      if (one_root) {
-        const int64_t t = rank32 ? tab.rank4[id] : tab.rank[id];
-        emit(jo, om, n_out, root, rank, v, root0, t - r.w);
-      } else if (packed) {
-        const uint64_t e = tab.packed[id];
-        emit(jo, om, n_out, root, rank, v, e & 0xFFFFFFFFull, static_cast<int32_t>(e >> 32) - r.w);
-      } else {
-        emit(jo, om, n_out, root, rank, v, tab.root[id], tab.rank[id] - r.w);
+  if constexpr (!kGrouped) {
+    const uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    const bool unclaimed =
+        v < n && propagate_element(rec, v, tab, table, root0, n_out, root, rank, om, tour_lo, err, jo);
+    const uint64_t k = wave_append(u_ctr, unclaimed);
+    if (unclaimed) collect_element(rec, v, k, u_ids, u_orig, ucap, err, device_sized);
+  } else {
+    const unsigned lane = __lane_id();
+    const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x) / 64;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kThreads;
+    for (uint64_t g = wave * 64; g < n; g += kPropGroup * stride) {
+      uint32_t mask = 0;  // bit q: element g + q * stride + lane is unclaimed
+      for (unsigned q = 0; q < kPropGroup; q++) {
+        const uint64_t wb = g + q * stride;
+        if (wb >= n) break;  // uniform over the wave
+        if (wb + lane < n &&
+            propagate_element(rec, wb + lane, tab, table, root0, n_out, root, rank, om, tour_lo, err, jo))
+          mask |= 1u << q;
       }
-    } else if (v < tour_lo) {
-      unclaimed = true;
-    } else if (((v - tour_lo) & 1) == 0) {
-      // an unreached upward edge of a tour: an isolated root, else a cycle
-      if (r.s == v)
-        emit(jo, om, n_out, root, rank, v, v, 0);
-      else
-        raise_err(err, kErrCycle);
+      const uint32_t cnt = static_cast<uint32_t>(__popc(mask));
+      uint32_t inc = cnt;  // inclusive prefix sum over the wave
+      for (unsigned o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o);
+        if (lane >= o) inc += y;
+      }
+      const uint32_t total = __shfl(inc, 63);
+      if (total == 0) continue;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(u_ctr, static_cast<unsigned long long>(total));
+      base = __shfl(base, 0);
+      uint64_t k = base + inc - cnt;
+      for (; mask != 0; mask &= mask - 1, k++)
+        collect_element(rec, g + uint64_t(__ffs(mask) - 1) * stride + lane, k, u_ids, u_orig, ucap, err,
+                        device_sized);
     }
-  }
-  const uint64_t k = wave_append(u_ctr, unclaimed);
-  if (unclaimed && k < ucap) {
-    u_ids[k] = v;
-    u_orig[k] = r;
-    RecOps<R>::set_s(&rec[v], k | kAuxFlag);
-  } else if (unclaimed && device_sized) {
-    // beyond the capacity: host-sized levels collect the rest in a second
-    // pass; device-sized ones flag the call to be redone host-sized
-    raise_err(err, kErrOverflow);
   }
 }
 
@@ -1041,6 +1102,30 @@ SpawnPlan spawn_plan(const RankTuning& tune, uint64_t n, uint32_t logK, bool has
   return p;
 }
 
+// Launches propagation in the shape that suits the expected number of
+// unreached elements: about `lists` prefixes of K elements each, or K / 8
+// when spawn starts rulers inside them.  Grouped when they would put an
+// unclaimed element into a quarter of the waves or more.
+template <class R>
+void launch_propagate(Context* ctx, hipStream_t st, uint64_t n, uint64_t lists, uint32_t spacing, bool spawn,
+                      R* rec, uint64_t n_out, const RulerTable& tab, uint64_t* root, int64_t* rank, OutMap om,
+                      uint64_t* u_ids, Rec* u_orig, uint64_t ucap, unsigned long long* u_ctr, uint64_t tour_lo,
+                      const uint64_t* n_dev, bool device_sized, const JobOut& jo) {
+  const uint64_t list_end = std::min(n, tour_lo);
+  const uint64_t expected = std::min<uint64_t>(list_end, lists * (spawn ? spacing >> kSpawnShift : spacing));
+  if (expected * 256 >= n && n > 0) {
+    const unsigned grid = static_cast<unsigned>(
+        std::min<uint64_t>(blocks_for(n, kThreads), static_cast<uint64_t>(ctx->num_cus) * 8));
+    hipLaunchKernelGGL((propagate_kernel<R, true>), dim3(grid), dim3(kThreads), 0, st, rec, n, n_out, tab, root,
+                       rank, om, u_ids, u_orig, ucap, u_ctr, tour_lo, ctx->d_err, n_dev, device_sized, jo);
+  } else {
+    hipLaunchKernelGGL((propagate_kernel<R, false>), dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n,
+                       n_out, tab, root, rank, om, u_ids, u_orig, ucap, u_ctr, tour_lo, ctx->d_err, n_dev,
+                       device_sized, jo);
+  }
+}
+
+
 // Counter slots of one level's chase.
 struct ChaseSlots {
   unsigned long long* work;
@@ -1161,9 +1246,8 @@ void srs_level(Context* ctx, Rec* rec, uint64_t n, uint64_t n_out, uint64_t* roo
   Rec* u_orig = scratch.get<Rec>(ucap);
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(propagate_kernel<Rec>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec,
-                       n, n_out, tab, root, rank, om, u_ids, u_orig, ucap,
-                       ctr + kSlotUnclaimed, tour_lo, ctx->d_err, nullptr, false, jo);
+    launch_propagate(ctx, st, n, std::min(max_tails, n), spacing, sp.on, rec, n_out, tab, root, rank, om, u_ids,
+                     u_orig, ucap, ctr + kSlotUnclaimed, tour_lo, nullptr, false, jo);
   }
   uint64_t nu = 0;
   read_counters(ctx, st, kSlotUnclaimed, 1, &nu);
@@ -1357,9 +1441,8 @@ void ds_srs_level(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint6
   Rec* u_orig = scratch.get<Rec>(std::max<uint64_t>(ucap, 1));
   {
     Phase ph(ctx, kPropNames[std::min(level, 8)], st);
-    hipLaunchKernelGGL(propagate_kernel<R>, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, st, rec, n, n_out,
-                       tab, root, rank, om, u_ids, u_orig, ucap, ctr + kDsUnclaimed, tour_lo,
-                       ctx->d_err, n_dev, true, jo);
+    launch_propagate(ctx, st, n, lists, L.spacing, sp.on, rec, n_out, tab, root, rank, om, u_ids, u_orig, ucap,
+                     ctr + kDsUnclaimed, tour_lo, n_dev, true, jo);
   }
   if (ucap == 0) return;  // tours only: nothing is left unreached
   Phase ph(ctx, "unclaimed", st);

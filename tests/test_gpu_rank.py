@@ -296,3 +296,22 @@ def test_tuning_table(ctx):
         ctx.tuning_set([])
     assert ctx.tuning_get(1 << 20)["spacing_top"] in (8, 16, 32, 64, 128)
     assert before["small_n"] == 8192
+
+
+@pytest.mark.parametrize("launch", ["device", "host"])
+@pytest.mark.parametrize("nlists", [30, 60000])
+@pytest.mark.parametrize("spacing,spawn", [(4, "off"), (256, "on"), (1024, "off")])
+def test_forest_propagation_shapes(ctx, launch, nlists, spacing, spawn):
+    """Propagation runs one element per thread when few waves meet an
+    unreached list prefix (30 lists), and strides a resident grid, one
+    counter atomic per wave and 32 strides, when many do (60,000 short
+    lists).  Both forests ranked with dense and sparse rulers, with and
+    without spawn, host- and device-sized; results against the sequential
+    walk."""
+    say("propagation shapes", launch, nlists, spacing, spawn)
+    n = 3_000_017
+    succ, w = random_forest_lists(n, nlists, 99 + nlists, weighted=True)
+    root, rank = ctx.rank_list(dev(succ), dev(w), algorithm="srs", ruler_spacing=spacing, spawn=spawn,
+                               launch=launch)
+    er, ek = o.sequential_walk(succ, w)
+    assert (host(root) == er).all() and (host(rank) == ek).all()
