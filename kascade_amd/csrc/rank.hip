@@ -1505,7 +1505,7 @@ const TuneRow kBuiltinTuning[] = {
 const TuneRow kBuiltinTreeTuning[] = {
     {10, 4, 16, 8, 64}, {12, 4, 32, 4, 64}, {14, 8, 32, 4, 64},
     {16, 4, 8, 1, 64}, {18, 8, 4, 1, 64}, {20, 16, 4, 1, 64},
-    {22, 32, 8, 2, 64}, {24, 32, 8, 4, 64}, {26, 64, 8, 2, 64},
+    {22, 32, 8, 2, 64}, {24, 32, 16, 3, 64}, {26, 64, 8, 2, 64},
     {28, 256, 16, 2, 64},
 };
 
