@@ -530,16 +530,17 @@ seg_local_tails_kernel(const Rec* __restrict__ sub, uint64_t* __restrict__ sub_o
 // their record is re-marked with that compact id.
 //
 // Two launch shapes.  One element per thread, one counter atomic per wave
-// that has an unclaimed element: the fastest when few waves have one (a
-// wave's stores are left in flight when it exits).  Grouped (kGrouped): a
-// grid of resident size strides over the level, and each wave appends the
-// unclaimed elements of kPropGroup strides with one atomic.  A forest of
-// many short lists has an unreached prefix in most waves, and their atomics
-// on the single counter then dominate (MI355X, 200,000 lists of 2^6..2^10
-// elements, 6.9e7 in all: 10.5 ms one per thread, 0.79 ms grouped); where
-// they are rare the grouped loop is slower, because every stride waits for
-// the previous stride's stores (a load's wait counter covers stores too):
-// 20,000 lists of 2^8..2^20 with spawn, 2.5e9 elements, 11.3 against 18.9 ms.
+// that has an unclaimed element: the fastest when few waves have one.
+// Grouped (kGrouped): a grid of resident size strides over the level, and
+// each wave appends the unclaimed elements of kPropGroup strides with one
+// atomic.  A forest of many short lists has an unreached prefix in most
+// waves, and their atomics on the single counter then dominate (MI355X,
+// 200,000 lists of 2^6..2^10 elements, 6.9e7 in all: 10.5 ms one per
+// thread, 0.79 ms grouped); where they are rare every looped shape measured
+// slower (20,000 lists of 2^8..2^20 with spawn, 2.5e9 elements: 11.3 ms one
+// per thread, 18.9 grouped, 13.4-15.1 with 4-16 elements per thread), most
+// likely because each load's wait also waits for the previous element's
+// streaming stores (one wait counter covers both on this GPU).
 constexpr unsigned kPropGroup = 32;
 
 template <class R>
