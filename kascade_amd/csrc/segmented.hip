@@ -44,43 +44,86 @@ __device__ __forceinline__ void chunk_range(const uint64_t* __restrict__ seg_off
 // R: Rec, or Rec8D for a narrow batch (a weight outside int32, or a
 // successor 2^29 or more away from its element, then sets *wide and the
 // batch is packed again with wide records).
+// The block's segment offsets are staged in LDS (when at most kSegCache + 1
+// of them) and the successors and weights are loaded before the first
+// store, so no load waits for a store (one wait counter covers both: the
+// pack had run at 4.5 TB/s with its offset searches in global memory).
+constexpr uint64_t kSegCache = 255;
+
 template <class R>
 __global__ void __launch_bounds__(kThreads)
 seg_pack_kernel(const uint32_t* __restrict__ succ, const uint64_t* __restrict__ seg_off,
                 uint64_t nseg, uint64_t total, const int64_t* __restrict__ w,
                 R* __restrict__ rec, uint32_t* err, unsigned long long* tails, unsigned long long* wide) {
   __shared__ uint64_t s_range[2];
-  const uint64_t first = static_cast<uint64_t>(blockIdx.x) * kPer * kThreads;
-  chunk_range(seg_off, nseg, total, first, s_range);
-  uint64_t lo = s_range[0];
-  const uint64_t hi = s_range[1];
+  __shared__ uint64_t s_off[kSegCache + 1];
+  const uint64_t block_first = static_cast<uint64_t>(blockIdx.x) * kPer * kThreads;
+  chunk_range(seg_off, nseg, total, block_first, s_range);
+  const uint64_t seg_lo = s_range[0], seg_hi = s_range[1];
+  const uint64_t cnt = seg_hi - seg_lo + 2;  // offsets seg_lo .. seg_hi + 1
+  const bool cached = cnt <= kSegCache + 1;
+  if (cached)
+    for (uint64_t t = threadIdx.x; t < cnt; t += kThreads) s_off[t] = seg_off[seg_lo + t];
+  const uint64_t first = block_first + threadIdx.x;
+  uint32_t loc[kPer];
+#pragma unroll
   for (unsigned q = 0; q < kPer; q++) {
-    const uint64_t i = first + q * kThreads + threadIdx.x;
-    bool tail = false;
-    if (i < total) {
-      lo = find_segment(seg_off, lo, hi, i);
-      const uint64_t base = seg_off[lo];
-      const uint64_t size = seg_off[lo + 1] - base;
-      uint64_t loc = succ[i];
-      if (loc >= size) {
-        raise_err(err, kErrRange);
-        loc = i - base;
-      }
-      const uint64_t g = base + loc;
-      tail = (g == i);
-      int64_t wt;
-      if (w) {
-        wt = w[i];
-        if (tail && wt != 0) raise_err(err, kErrRootWeight);
-        if (!RecOps<R>::fits(wt)) *wide = 1;
-      } else {
-        wt = tail ? 0 : 1;
-      }
-      if (!RecOps<R>::fits_succ(g, i)) *wide = 1;
-      rec[i] = RecOps<R>::encode(g, wt, i);
-    }
-    wave_append(tails, tail);
+    const uint64_t i = first + q * kThreads;
+    loc[q] = i < total ? succ[i] : 0;
   }
+  int64_t wt[kPer];
+  if (w) {
+#pragma unroll
+    for (unsigned q = 0; q < kPer; q++) {
+      const uint64_t i = first + q * kThreads;
+      wt[q] = i < total ? w[i] : 0;
+    }
+  }
+  __syncthreads();
+  uint64_t lo = 0;  // segment index relative to seg_lo (cached) or absolute
+  if (!cached) lo = seg_lo;
+  uint32_t ntails = 0;
+#pragma unroll
+  for (unsigned q = 0; q < kPer; q++) {
+    const uint64_t i = first + q * kThreads;
+    if (i >= total) break;
+    uint64_t base, size;
+    if (cached) {
+      uint64_t a = lo, b = cnt - 2;  // invariant: s_off[a] <= i < s_off[b + 1]
+      while (a < b) {
+        const uint64_t mid = (a + b + 1) / 2;
+        if (s_off[mid] <= i) a = mid; else b = mid - 1;
+      }
+      lo = a;
+      base = s_off[lo];
+      size = s_off[lo + 1] - base;
+    } else {
+      lo = find_segment(seg_off, lo, seg_hi, i);
+      base = seg_off[lo];
+      size = seg_off[lo + 1] - base;
+    }
+    uint64_t l = loc[q];
+    if (l >= size) {
+      raise_err(err, kErrRange);
+      l = i - base;
+    }
+    const uint64_t g = base + l;
+    const bool tail = g == i;
+    ntails += tail;
+    int64_t x;
+    if (w) {
+      x = wt[q];
+      if (tail && x != 0) raise_err(err, kErrRootWeight);
+      if (!RecOps<R>::fits(x)) *wide = 1;
+    } else {
+      x = tail ? 0 : 1;
+    }
+    if (!RecOps<R>::fits_succ(g, i)) *wide = 1;
+    rec[i] = RecOps<R>::encode(g, x, i);
+  }
+  // one atomic per wave for its tails
+  for (int o = 32; o > 0; o >>= 1) ntails += __shfl_xor(ntails, o);
+  if (__lane_id() == 0 && ntails) atomicAdd(tails, static_cast<unsigned long long>(ntails));
 }
 
 __global__ void __launch_bounds__(kThreads)
