@@ -6,14 +6,16 @@
 For every size class c it generates a seeded random weighted list of 2^c
 elements (at least 12288) on the device, with --kind tree a random rooted
 tree of 2^c vertices (at least 6144; its Euler tour has twice as many
-records), or with --kind segmented a batch of seeded random weighted lists
-of 2^8..2^20 elements each (log-uniform) with about 2^c elements in all,
-and picks, by coordinate descent over the knobs
+records), or with --kind segmented a batch of seeded random lists of
+2^8..2^20 elements each (log-uniform, 30 % locality-biased, unit weights)
+with about 2^c elements in all, and picks, by coordinate descent over the
+knobs
 
     ruler spacing at the top level, ruler spacing below it,
     resident chase workgroups per CU, rulers per work-counter atomic,
 
-the configuration with the lowest median time over `reps` calls.  Every
+starting from the knobs the context currently uses for that size, the
+configuration with the lowest median time over `reps` calls.  Every
 candidate's output must equal the first configuration's output bit for bit
 (outputs are unique, so any difference is a bug and aborts the sweep).  The
 table is written as text (tree rows prefixed with "tree", segmented-batch
@@ -61,7 +63,9 @@ PREFIX = {"list": "", "tree": "tree ", "segmented": "seg "}
 
 def segmented_batch(ctx, total, seed):
     """Offsets of lists of 2^8..2^20 elements (log-uniform, seeded) adding up
-    to at least `total`, and the batch's successors and weights."""
+    to at least `total`, 30 % of them locality-biased, unit weights (the
+    composition of the 20,000-list batch benchmark), and the batch's
+    successors."""
     import numpy as np
     import torch
     rng = np.random.default_rng(seed)
@@ -71,9 +75,11 @@ def segmented_batch(ctx, total, seed):
     off = np.zeros(len(sizes) + 1, np.uint64)
     off[1:] = np.cumsum(np.array(sizes, np.uint64))
     seeds = rng.integers(1, 1 << 62, size=len(sizes), dtype=np.uint64)
+    local = (rng.random(len(sizes)) < 0.3).astype(np.uint8)
     d_off = torch.from_numpy(off).to("cuda")
-    succ, w = ctx.gen_lists_segmented(d_off, torch.from_numpy(seeds).to("cuda"), None, weighted=True)
-    return d_off, succ, w
+    succ, _ = ctx.gen_lists_segmented(d_off, torch.from_numpy(seeds).to("cuda"), torch.from_numpy(local).to("cuda"),
+                                      weighted=False)
+    return d_off, succ, None
 
 
 def tune_class(ctx, c, reps, kind="list", seed=12345):
@@ -95,7 +101,11 @@ def tune_class(ctx, c, reps, kind="list", seed=12345):
     rank = torch.empty(n, dtype=torch.int64, device="cuda")
     ref_root = torch.empty_like(root)
     ref_rank = torch.empty_like(rank)
-    best = dict(START)
+    # start from the context's current row for this size (coordinate descent
+    # finds a local optimum: from a fixed start it can miss a joint optimum
+    # of spacing and workgroups that the current row already holds)
+    cur = ctx.tuning_get(n, kind=kind)
+    best = {k: cur[k] for k in START} if cur["spacing_top"] else dict(START)
     best_t = measure(run, ref_root, ref_rank, best, reps)
     for _ in range(2):
         for knob, values in KNOBS.items():
