@@ -409,8 +409,11 @@ def segmented_main(args, ctx):
     rank = torch.empty(total, dtype=torch.int64, device="cuda")
     scfg = dict(spawn=args.spawn, records=args.records, ruler_spacing=args.spacing, ruler_spacing_deep=args.spacing_deep,
                 chase_blocks_per_cu=args.chase_blocks)
+    rs0 = ctx.record_stats()
     t, phases = timed(ctx, lambda: ctx.rank_lists_segmented(succ, d_off, None, root, rank, **scfg), args.steps,
                       args.warmup)
+    rs1 = ctx.record_stats()
+    knobs = ctx.tuning_get(total, kind="segmented")
     out = {"metric": "elements ranked/s (segmented batch)", "value": total / t, "unit": "elements/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t * 1e3,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
@@ -419,7 +422,11 @@ def segmented_main(args, ctx):
                       "segments": args.segments, "total_elements": total, "seed": args.seed},
            "effective_gbps": (16 * total + 8 * args.segments) / t / 1e9,
            "phases_ms_per_step": {k: v[0] / args.steps for k, v in sorted(phases.items())},
-           "launch": ctx.launch_stats()}
+           "launch": ctx.launch_stats(),
+           # 8 when every warm-up, timed and profiled call used 8-byte level-0 records
+           "level0_record_bytes": 8 if rs1["narrow_calls"] - rs0["narrow_calls"] == args.warmup + 2 * args.steps
+           else 16,
+           "tuning": {k: knobs[k] for k in ("spacing_top", "spacing_deep", "chase_blocks_per_cu", "chase_chunk")}}
     if not args.no_check:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as o
