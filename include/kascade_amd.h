@@ -122,7 +122,7 @@ typedef enum kc_contract {
  * the longest remaining gaps are split instead of walked to the end by a few
  * lanes.  In an Euler tour the candidates are upward edges (the tour
  * elements that carry claims); AUTO leaves spawn off for tours (2^26-vertex
- * tree on MI355X: 8.58 ms with spawn against 8.38 without at the tuned 128
+ * tree on MI355X: 8.58 ms with spawn against 8.38 without at 128
  * elements per ruler).  Results are identical either way. */
 typedef enum kc_spawn {
   KC_SPAWN_AUTO = 0, /* on for list levels (no tours) of at least 2^20 elements and 256 or more per ruler */

@@ -1076,10 +1076,10 @@ uint32_t level_spacing(const RankTuning& tune, int level, uint64_t n, uint64_t m
 // without, 7.49-7.54 with it, whose 4 MB rank table then makes propagation
 // 0.45 ms shorter than at K = 128).  KC_SPAWN_ON forces it on any level
 // whose K exceeds 8, tour levels included (candidates on upward edges); AUTO
-// keeps it off for tours, whose chase has no long tail at the tuned K = 128
-// (2^26-vertex tree: level-0 chase 4.14 ms with or without spawn, the whole
-// call 8.58 against 8.38 ms; at K = 256 spawn helps, 4.50 against 4.97 ms,
-// but K = 128 without it stays faster).
+// keeps it off for tours, whose chase has no long tail at the tree rows'
+// K of 128 and below (2^26-vertex tree at K = 128: level-0 chase 4.14 ms
+// with or without spawn, the whole call 8.58 against 8.38 ms; at K = 256
+// spawn helps, 4.50 against 4.97 ms, but K = 128 without it stays faster).
 constexpr uint32_t kSpawnShift = 3;    // candidates per ruler: 2^kSpawnShift
 constexpr uint64_t kSpawnActive = 16;  // suspend when fewer than lanes / this still chase
 struct SpawnPlan {
