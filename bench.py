@@ -570,16 +570,18 @@ def main():
         # the profiled configuration must be the one this run used
         lvl0 = out["ruler_trace"][0]
         spawn_on = lvl0.get("spawned", 0) > 0
+        blocks = args.chase_blocks or ctx.tuning_get(n)["chase_blocks_per_cu"]
         matches = (str(lvl0.get("spacing")) == traffic_conf.get("spacing")
                    and ("on" if spawn_on else "off") == traffic_conf.get("spawn")
-                   and str(record_bytes) == traffic_conf.get("records"))
+                   and str(record_bytes) == traffic_conf.get("records")
+                   and str(blocks) == traffic_conf.get("blocks"))
         out["roofline"]["traffic_source"] = CHASE_PROFILE
         out["roofline"]["traffic_config_matches"] = matches
         if not matches:
             out["roofline"]["traffic"] = None
             out["roofline"]["traffic_note"] = (f"profile measured {traffic_conf}, this run used spacing "
                                                f"{lvl0.get('spacing')} spawn {'on' if spawn_on else 'off'} "
-                                               f"records {record_bytes}")
+                                               f"records {record_bytes} blocks {blocks}")
             traffic = None
     if not args.no_host_buffers:
         out["host_buffers"] = host_buffer_rates(ctx, succ, w, cfg)
