@@ -1498,7 +1498,7 @@ void ds_rank(Context* ctx, R* rec, uint64_t n, const uint64_t* n_dev, uint64_t n
 const TuneRow kBuiltinTuning[] = {
     {10, 4, 8, 4, 64}, {12, 4, 2, 8, 64}, {14, 4, 32, 2, 64},
     {16, 16, 16, 8, 64}, {18, 4, 4, 4, 64}, {20, 16, 4, 1, 64},
-    {22, 16, 8, 1, 64}, {24, 32, 8, 2, 64}, {25, 64, 8, 2, 64}, {26, 64, 16, 2, 64},
+    {22, 16, 8, 1, 64}, {24, 16, 16, 2, 64}, {25, 32, 8, 2, 64}, {26, 64, 8, 2, 64},
     {27, 256, 8, 2, 64}, {28, 512, 8, 3, 64}, {30, 512, 8, 3, 64},
 };
 
